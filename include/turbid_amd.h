@@ -1,0 +1,124 @@
+/*
+ * turbid_amd.h -- C ABI of the MI355X pseudospectral Fourier-Chebyshev RK3
+ * substep (TURBID FlatChannelSolver hot path) on one GPU.
+ *
+ * The reference has no FFI: its boundary is the C++ virtual interface
+ * Phase::{Configure, Predict(stage), Correct(stage), Update, Save, Load}
+ * (reference src/phase.h:60-73) driven by FlatChannelSolver::Execute
+ * (src/solver_channel_flat.cc:199-293), with the two phases sharing static
+ * variables (src/variable.h:70-85).  One tbd_ctx therefore owns the carrier
+ * AND the scalar phase on the device.  The reference-side binding is the
+ * C++ host in turbid_amd/csrc/host/ (GPU-backed FlatChannelSolver), which
+ * calls only the functions below.
+ *
+ * Conventions (mirroring the reference's error behaviour at a C boundary):
+ *   - every int-returning call returns 0 on success and a negative code on
+ *     failure; the message ("Class::Method > Message" style) is available
+ *     from tbd_last_error() (thread-local);
+ *   - host pointers are borrowed for the duration of the call;
+ *   - spectral arrays use the RealCplxCube order (src/cube_real_cplx.cc:26-91):
+ *     interleaved complex double, index iz + Nz*(iy + nyc*(ix + mx*d)),
+ *     nxc = Nx/3*2, nyc = Ny/3*2, mx = nxc/2+1 (src/base.cc:233-258);
+ *   - physical arrays: double, index x + Nx*(y + Ny*(z + Nz*d));
+ *   - calls on one context are synchronous with respect to the host unless
+ *     stated; one host thread per context.
+ */
+#ifndef TURBID_AMD_H_
+#define TURBID_AMD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Plain mirror of the ini keys the path reads: [Domain], [Space Stepper],
+ * [Time Stepper], [Phase Carrier], [Phase Scalar] (src/configure.h:63-117).
+ * Replaces the Configuration::Load(IniFile&) structs of
+ * src/phase_carrier_flat.cc:27-61, src/phase_scalar_flat.cc:27-54,
+ * src/domain.cc:20-35, src/stepper_time.cc:20-41. */
+typedef struct tbd_config {
+  int nx, ny, nz;                 /* grid_point_number */
+  int nphase;                     /* total_phase_number: 1 carrier, 2 + sediment */
+  double lx, ly, lz;              /* dimensionless_length (lz must be 2) */
+  double slope;                   /* top_bottom_slope */
+  int slope_direction;            /* 0 none, 1 x, 2 y */
+  int mapping;                    /* 0 None 1 TwoEndTangent 2 TwoEndSin 3 OneEndTangent */
+  double mapping_coefficient;
+  double dt;                      /* marching_time_step (step mode) / initial_time_step */
+  int output_on_step;             /* output_based_on_step */
+  int force_type;                 /* 0 None 1 Constant 2 Oscillatory 3 Pulsatile 4 Damping */
+  int force_direction[2];
+  double force_amplitude[4];
+  double reynolds, froude, averaged_concentration, specific_gravity;
+  int affected_by_scalar_phase;
+  double vt_u1[5], vt_u2[5], vt_u3[5], vt_p[5];   /* vertical_type_* */
+  double schmidt, settling_velocity, erosion_rate, critical_shear_stress;
+  int scalar_bc;                  /* 0 ZeroFlux, 1 ErosionDeposition */
+  int precision;                  /* 0 FP64 throughout; 1 mixed (FP32 transforms,
+                                     nonlinear products and their D1 contractions) */
+} tbd_config;
+
+typedef struct tbd_ctx tbd_ctx;
+
+/* Builds one DNS on the current device: operators, factorisations, device
+ * buffers.  Replaces the constructors + Configure of ChebyshevSpaceStepper
+ * (src/stepper_space_chebyshev.cc:101-250), CmDz2DecompositionCplx
+ * (src/cmdz2_decomposition.cc:19-227) and both phases
+ * (src/phase_carrier_flat.cc:178-246, src/phase_scalar_flat.cc:176-221). */
+int tbd_create(const tbd_config* cfg, tbd_ctx** out);
+void tbd_destroy(tbd_ctx* ctx);
+const char* tbd_last_error(void);
+
+/* State transfer.  Replaces Variable::Load/Save's cube I/O
+ * (src/variable_carrier_flat.cc:254-377, src/variable_scalar_flat.cc:59-212):
+ * uvw: 3*S complex, p: S complex, c: S complex (NULL when nphase == 1).
+ * Upload also performs the Configure-time work (physical velocity, zero
+ * H and wall-gradient history, src/boundary_carrier_flat.cc:24-54, 255-269). */
+int tbd_upload_state(tbd_ctx* ctx, const double* uvw, const double* p, const double* c);
+int tbd_download_state(tbd_ctx* ctx, double* uvw, double* p, double* c);
+/* Physical carrier velocity (uvwnonlinear), 3*Nx*Ny*Nz doubles. */
+int tbd_download_physical_velocity(tbd_ctx* ctx, double* uvw);
+
+/* One RK3 substage: FlatCarrierPhase::Predict + Correct then
+ * FlatScalarPhase::Predict + Correct (src/solver_channel_flat.cc:223-234).
+ * time = TimeStepper time after Update; alternation = TimeStepper flag. */
+int tbd_substage(tbd_ctx* ctx, int stage, double time, int alternation);
+/* The three substages of one step (src/solver_channel_flat.cc:223-250). */
+int tbd_step(tbd_ctx* ctx, double time, int alternation);
+/* Time mode: new dt (TimeStepper::Update2 + Phase::Update reassembly,
+ * src/stepper_time.cc:120-208, src/phase_carrier_flat.cc:250-265). */
+int tbd_set_dt(tbd_ctx* ctx, double dt);
+/* Phase::featureglobal_ after the last stage 2 (src/phase_carrier_flat.cc:633-703). */
+int tbd_features(tbd_ctx* ctx, double* out50);
+int tbd_sync(tbd_ctx* ctx);
+
+/* Kernel timing with HIP events on the context's stream.
+ * class: 0 plane transforms, 1 z-contractions, 2 Helmholtz solves, 3 other. */
+int tbd_profile_enable(tbd_ctx* ctx, int enable);
+int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms);
+int tbd_profile_reset(tbd_ctx* ctx);
+
+/* Operators built by the context (pinned against closed forms in tests):
+ * gridz[nz], d1/d2 nz*nz column-major (stepper_space_chebyshev.cc:122-228). */
+int tbd_get_operators(tbd_ctx* ctx, double* gridz, double* d1, double* d2);
+/* which: 0 u1, 1 u2, 2 u3, 3 p, 4 scalar; V~, V~^-1 nz*nz column-major,
+ * lambda[nz], fix[4*nz] (cmdz2_decomposition.cc:47-139). */
+int tbd_get_factorisation(tbd_ctx* ctx, int which, double* V, double* Vinv, double* lambda, double* fix);
+
+/* Stand-alone device kernels on host buffers (parity tests of single pieces).
+ * DftForward / DftBackward of nv components (src/cube_real_cplx.cc:525-555);
+ * fp32 != 0 runs the FP32 transform on FP64 data. */
+int tbd_dft_forward(int nx, int ny, int nz, int nv, int fp32, const double* phys, double* spec);
+int tbd_dft_backward(int nx, int ny, int nz, int nv, int fp32, const double* spec, double* phys);
+/* Y = M X for ncol complex columns of length nz, M nz*nz column-major
+ * (RealCplxCube::Multiply(cmat, result), src/cube_real_cplx.cc:461-471). */
+int tbd_zcontract(int nz, long long ncol, int fp32, const double* m, const double* x, double* y);
+/* Helmholtz solve with the context's factorisation `which` at `stage` on
+ * its own fields layout: x, y each S complex (one component). mode 0 forward
+ * fix, 1 pressure (src/phase_carrier_flat.cc:861-994). */
+int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TURBID_AMD_H_ */

@@ -1,0 +1,147 @@
+"""GPU parity against the FP64 CPU restatement (oracle/) through the C ABI.
+
+Tolerances (relative L2 unless stated):
+  FP64 transforms / contractions / solves: 1e-12
+  FP32 transforms / contractions: 2e-6
+  FP64 RK3 steps (several steps, both nonlinear forms): 1e-9
+  mixed-precision RK3 steps: 1e-5 (fields), the north-star's bound
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from turbid_amd import runtime as RT
+from turbid_amd import workloads as W
+
+from helpers import rel_l2, small_config
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(1234)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 5), (48, 96, 3), (384, 256, 2), (8, 6, 4), (3, 3, 3)])
+def test_dft_forward_fp64(shape):
+    nx, ny, nz = shape
+    u = RNG.standard_normal((2, nz, ny, nx))
+    ref = O.dft_forward(u)
+    got = RT.dft_forward(u)
+    assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 5), (48, 96, 3), (256, 256, 2)])
+def test_dft_backward_fp64(shape):
+    nx, ny, nz = shape
+    nxc, nyc, mx = RT.dims(nx, ny, nz)
+    s = RNG.standard_normal((2, mx, nyc, nz)) + 1j * RNG.standard_normal((2, mx, nyc, nz))
+    ref = O.dft_backward(s, nx, ny)
+    got = RT.dft_backward(s, nx, ny)
+    assert rel_l2(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 3), (96, 48, 2), (384, 384, 2)])
+def test_dft_fp32(shape):
+    nx, ny, nz = shape
+    u = RNG.standard_normal((1, nz, ny, nx))
+    ref = O.dft_forward(u)
+    got = RT.dft_forward(u, fp32=True)
+    assert rel_l2(got, ref) < 2e-6
+    back = RT.dft_backward(ref, nx, ny, fp32=True)
+    assert rel_l2(back, O.dft_backward(ref, nx, ny)) < 2e-6
+
+
+@pytest.mark.parametrize("nz,ncol", [(65, 100), (257, 300), (20, 17), (33, 1)])
+def test_zcontract(nz, ncol):
+    m = RNG.standard_normal((nz, nz))
+    x = RNG.standard_normal((ncol, nz)) + 1j * RNG.standard_normal((ncol, nz))
+    ref = (m @ x.T).T
+    got = RT.zcontract(m, x)
+    assert rel_l2(got, ref) < 1e-13
+    got32 = RT.zcontract(m, x, fp32=True)
+    assert rel_l2(got32, ref) < 2e-6
+
+
+def _pair(cfg):
+    o = O.Oracle(cfg)
+    g = RT.Solver(cfg)
+    uvw, p, c = W.initial_state(cfg)
+    o.set_state(uvw, p, c if cfg["nphase"] > 1 else None)
+    g.upload(uvw, p, c if cfg["nphase"] > 1 else None)
+    return o, g
+
+
+@pytest.mark.parametrize("which,mode", [(0, 0), (2, 0), (3, 1), (4, 0)])
+def test_helmholtz_vs_oracle(which, mode):
+    cfg = small_config(nx=12, ny=12, nz=65)
+    o, g = _pair(cfg)
+    S = g.S
+    x = RNG.standard_normal(S) + 1j * RNG.standard_normal(S)
+    got = g.helmholtz(which, 1, mode, x)
+    nz, P = g.nz, g.P
+    # oracle: per column with the solver's own multipliers
+    kx = 2 * np.pi / cfg["lx"] * np.arange(g.mx)
+    nyc = g.nyc
+    ky1 = 2 * np.pi / cfg["ly"]
+    ky = np.array([ky1 * iy if iy <= nyc // 2 else -ky1 * (nyc - iy) for iy in range(nyc)])
+    cols = np.arange(P)
+    k2 = -(kx[cols // nyc] ** 2) + -(ky[cols % nyc] ** 2)
+    ref = o.helmholtz(which, 1, mode, x.reshape(P, nz), k2).reshape(-1)
+    assert rel_l2(got, ref) < 1e-11
+
+
+def test_operators_match_oracle():
+    cfg = small_config(nx=12, ny=12, nz=65)
+    o, g = _pair(cfg)
+    go, d1o, d2o = o.operators()
+    gg, d1g, d2g = g.operators()
+    assert np.array_equal(go, gg)
+    assert np.array_equal(d1o, d1g)
+    assert rel_l2(d2g, d2o) < 1e-15
+
+
+def _run_pair(cfg, nsteps):
+    o, g = _pair(cfg)
+    st = RT.Stepper(cfg["dt"])
+    feat = np.zeros(50)
+    for n in range(nsteps):
+        st.update(feat[19])
+        for s in range(3):
+            o.substage(s, st.time, st.alternation)
+            g.substage(s, st.time, st.alternation)
+        st.toggle()
+        feat = g.features()
+    return o, g
+
+
+@pytest.mark.parametrize("nphase,erosion,top", [(2, True, "freeslip"), (1, False, "freeslip"), (2, False, "noslip")])
+def test_steps_fp64(nphase, erosion, top):
+    cfg = small_config(nphase=nphase, erosion=erosion, top=top)
+    o, g = _run_pair(cfg, 4)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    assert rel_l2(ug, uo) < 1e-9
+    assert rel_l2(pg, po) < 1e-8
+    if nphase > 1:
+        assert rel_l2(cg, co) < 1e-9
+    fo, fg = o.features(), g.features()
+    idx = [0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 14, 15, 19, 21, 28, 29]
+    np.testing.assert_allclose(fg[idx], fo[idx], rtol=1e-8, atol=1e-12)
+
+
+def test_steps_buoyancy_fp64():
+    cfg = small_config(affected=1)
+    cfg.update(froude=2.0, specific_gravity=1.5)
+    o, g = _run_pair(cfg, 2)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    assert rel_l2(ug, uo) < 1e-9
+    assert rel_l2(cg, co) < 1e-9
+
+
+def test_steps_mixed():
+    cfg = small_config(precision=1)
+    o, g = _run_pair(cfg, 4)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    assert rel_l2(ug, uo) < 1e-5
+    assert rel_l2(cg, co) < 1e-5

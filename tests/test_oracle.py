@@ -1,0 +1,369 @@
+"""Pins of the FP64 CPU restatement (oracle/) -- CPU only.
+
+Each check is chosen so a plausible restatement mistake (a dropped term, a
+sign or index slip, a transposed operand) fails it: golden vectors from
+numpy/LAPACK, closed forms, brute-force solves of the bordered collocation
+system, the analytic Stokes layer, invariants.  The reference ships no tests
+or golden vectors for this path (its only known-answer check is the analytic
+laminar profile, tmtools/get_loflow.m:17-19, used in
+examples/oscillatory_flow_re180_laminar/mcode/e_analysis.m:59-61).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from turbid_amd import workloads as W
+
+from helpers import rel_l2, small_config
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# --------------------------------------------------------------------------
+# plane transforms (DftForward/DftBackward, cube_real_cplx.cc:525-555)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("tag", ["a", "b", "c"])
+def test_dft_golden(tag):
+    g = np.load(os.path.join(GOLD, "fft_golden.npz"))
+    u = g["fwd_in_" + tag]
+    assert rel_l2(O.dft_forward(u), g["fwd_out_" + tag]) < 1e-14
+    s = g["bwd_in_" + tag]
+    nx, ny = u.shape[3], u.shape[2]
+    assert rel_l2(O.dft_backward(s, nx, ny), g["bwd_out_" + tag]) < 1e-14
+
+
+def test_dft_single_mode_closed_form():
+    nx, ny, nz = 48, 24, 2
+    lx, ly = 2.0, 1.0
+    nxc, nyc, mx = O.dims(nx, ny, nz)
+    x = lx / nx * np.arange(nx)
+    y = ly / ny * np.arange(ny)
+    X, Y = np.meshgrid(x, y)
+    for p, q in ((3, 2), (0, 5), (4, -7), (15, 7)):
+        u = np.cos(2 * np.pi * (p * X / lx + q * Y / ly))
+        s = O.dft_forward(np.broadcast_to(u, (1, nz, ny, nx)).copy())[0]
+        expect = np.zeros((mx, nyc), complex)
+
+        def put(kx, ky, v):
+            if kx < mx:
+                iy = ky if ky >= 0 else ky + nyc
+                if -(nyc // 2) <= ky < nyc // 2 and 0 <= iy < nyc:
+                    expect[kx, iy] += v
+        if p == 0:
+            put(0, q, 0.5)
+            put(0, -q, 0.5)
+        else:
+            put(p, q, 0.5)
+        for z in range(nz):
+            np.testing.assert_allclose(s[:, :, z], expect, atol=1e-14)
+
+
+def test_dft_roundtrip_bandlimited():
+    # band-limited fields (no ky = -nyc/2 row, Hermitian kx = 0 plane) survive
+    # backward then forward exactly
+    cfg = small_config(nx=36, ny=24, nz=5)
+    uvw, _, _ = W.initial_state(cfg)
+    back = O.dft_backward(uvw, 36, 24)
+    assert rel_l2(O.dft_forward(back), uvw) < 1e-14
+
+
+def test_dealiasing_three_halves():
+    # the product of two retained modes transformed on the 3/2 grid carries no
+    # aliased energy on the retained modes: compare with the exact product
+    # spectrum computed on a 2x finer grid
+    # (N = 32: not a multiple of 3; for N % 3 == 0 the kept mode kx = N/3 is
+    # itself aliased by the reference's truncation nxc = N/3*2)
+    rng = np.random.default_rng(5)
+    nx = ny = 32
+    nxc, nyc, mx = O.dims(nx, ny, 1)
+    nxc2, nyc2, mx2 = O.dims(2 * nx, 2 * ny, 1)
+    yf = [iy if iy < nyc // 2 else iy - nyc + nyc2 for iy in range(nyc)]
+
+    def band_limited():
+        s = np.zeros((1, mx, nyc, 1), complex)
+        s[0, :, :nyc // 2] = rng.standard_normal((mx, nyc // 2, 1)) + 1j * rng.standard_normal((mx, nyc // 2, 1))
+        s[0, :, nyc // 2 + 1:] = rng.standard_normal((mx, nyc // 2 - 1, 1)) + 1j * rng.standard_normal((mx, nyc // 2 - 1, 1))
+        phys = O.dft_backward(s, nx, ny)
+        return O.dft_forward(phys)   # projected to the Hermitian representable set
+
+    def on_fine(s):                  # same Fourier modes on the 2x grid
+        f = np.zeros((1, mx2, nyc2, 1), complex)
+        f[:, :mx, yf, :] = s
+        return O.dft_backward(f, 2 * nx, 2 * ny)
+    a, b = band_limited(), band_limited()
+    prod = O.dft_backward(a, nx, ny) * O.dft_backward(b, nx, ny)
+    got = O.dft_forward(prod)
+    fine = O.dft_forward(on_fine(a) * on_fine(b))      # exact on the fine grid
+    exact = fine[:, :mx, yf, :]
+    assert rel_l2(got[:, :, : nyc // 2], exact[:, :, : nyc // 2]) < 1e-13
+    assert rel_l2(got[:, :, nyc // 2 + 1:], exact[:, :, nyc // 2 + 1:]) < 1e-13
+
+
+# --------------------------------------------------------------------------
+# Chebyshev operators (stepper_space_chebyshev.cc:101-228)
+# --------------------------------------------------------------------------
+def trefethen_cheb(N):
+    x = np.cos(np.pi * np.arange(N + 1) / N)
+    c = np.hstack([2, np.ones(N - 1), 2]) * (-1) ** np.arange(N + 1)
+    X = np.tile(x, (N + 1, 1)).T
+    D = np.outer(c, 1 / c) / (X - X.T + np.eye(N + 1))
+    return D - np.diag(D.sum(axis=1)), x
+
+
+@pytest.mark.parametrize("nz", [9, 33, 65])
+def test_chebyshev_operators(nz):
+    z, d1, d2 = O.chebyshev(nz)
+    assert z[0] == -1.0 and abs(z[-1] - 1.0) < 1e-15 and np.all(np.diff(z) > 0)
+    for m in range(nz):
+        pm = z ** m
+        dp = m * z ** (m - 1) if m > 0 else np.zeros(nz)
+        d2p = m * (m - 1) * z ** (m - 2) if m > 1 else np.zeros(nz)
+        scale = max(1.0, m * m)
+        assert np.abs(d1 @ pm - dp).max() < 1e-11 * scale * nz
+        assert np.abs(d2 @ pm - d2p).max() < 1e-9 * scale * scale * nz
+    assert np.abs(d1.sum(axis=1)).max() < 1e-12 * nz * nz
+    assert np.abs(d2.sum(axis=1)).max() < 1e-9 * nz ** 4
+    # centro-antisymmetry of D1, centro-symmetry of D2
+    assert np.abs(d1 + d1[::-1, ::-1]).max() < 1e-12 * nz * nz
+    assert np.abs(d2 - d2[::-1, ::-1]).max() < 1e-9 * nz ** 4
+    D, x = trefethen_cheb(nz - 1)
+    P = np.eye(nz)[::-1]
+    assert np.abs(d1 - P @ D @ P).max() < 1e-11 * nz * nz
+
+
+# --------------------------------------------------------------------------
+# eigen-factorisation and Helmholtz solves (cmdz2_decomposition.cc:19-297)
+# --------------------------------------------------------------------------
+BC = {
+    "dirichlet_dirichlet": (1.0, 0.0, 1.0, 0.0, 0.0),
+    "dirichlet_neumann": (1.0, 0.0, 0.0, 1.0, 0.0),
+    "neumann_neumann": (0.0, 1.0, 0.0, 1.0, 0.0),
+}
+
+
+def _oracle_with(vt_u1, nz=33, nphase=2):
+    cfg = small_config(nx=6, ny=6, nz=nz, nphase=nphase)
+    cfg["vt_u1"] = vt_u1
+    return O.Oracle(cfg), cfg
+
+
+@pytest.mark.parametrize("nz", [33, 65])
+@pytest.mark.parametrize("name", ["dirichlet_dirichlet", "dirichlet_neumann", "neumann_neumann", "robin_settling"])
+def test_eigenvalues_golden(nz, name):
+    g = np.load(os.path.join(GOLD, "eig_golden.npz"))
+    ref = np.sort(g["%s_%d" % (name, nz)].real)
+    if name == "robin_settling":
+        o, _ = _oracle_with(BC["dirichlet_neumann"], nz=nz)
+        _, _, lam, _ = o.decomposition(4)
+    else:
+        o, _ = _oracle_with(BC[name], nz=nz)
+        _, _, lam, _ = o.decomposition(0)
+    got = np.sort(lam[1:-1])
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() < 1e-10 * scale
+
+
+@pytest.mark.parametrize("name", list(BC))
+def test_eigen_residuals(name):
+    o, _ = _oracle_with(BC[name], nz=65)
+    V, Vi, lam, _ = o.decomposition(0)
+    n = V.shape[0]
+    assert np.abs(V @ Vi - np.eye(n)).max() < 1e-9
+    lam_in = lam[1:-1]
+    Vin = V[1:-1, 1:-1]
+    # rebuild the interior operator from the oracle's D1/D2 and check A V = V L
+    _, d1, d2 = o.operators()
+    vt = BC[name]
+    C = d2.copy()
+    C[0] = vt[1] * d1[0]
+    C[-1] = vt[3] * d1[-1]
+    C[0, 0] += vt[0]
+    C[-1, -1] += vt[2]
+    Binv = np.linalg.inv(C[[0, -1]][:, [0, -1]])
+    A = C[1:-1, 1:-1] - C[1:-1][:, [0, -1]] @ Binv @ C[[0, -1]][:, 1:-1]
+    res = np.abs(A @ Vin - Vin * lam_in).max() / np.abs(A).max()
+    assert res < 1e-10
+
+
+def _bordered_solve(o, which, vt, k2, sigma, rhs):
+    """Brute force: assemble the N x N bordered collocation system and LU-solve."""
+    _, d1, d2 = o.operators()
+    n = d1.shape[0]
+    A = d2 + (k2 + sigma) * np.eye(n)
+    A[0] = vt[1] * d1[0]
+    A[0, 0] += vt[0]
+    A[-1] = vt[3] * d1[-1]
+    A[-1, -1] += vt[2]
+    return np.linalg.solve(A, rhs)
+
+
+@pytest.mark.parametrize("which,name", [(0, "dirichlet_neumann"), (2, "dirichlet_dirichlet")])
+def test_helmholtz_vs_bordered_lu(which, name):
+    cfg = small_config(nx=6, ny=6, nz=33)
+    cfg["vt_u1"] = BC[name]
+    cfg["vt_u3"] = BC[name]
+    o = O.Oracle(cfg)
+    rng = np.random.default_rng(11)
+    n = 33
+    ncol = 5
+    x = rng.standard_normal((ncol, n)) + 1j * rng.standard_normal((ncol, n))
+    k2 = -np.array([0.0, 1.0, 4.5, 20.0, 300.0])
+    stage = 1
+    sigma = -cfg["reynolds"] / cfg["dt"] / (5.0 / 24.0)
+    got = o.helmholtz(which, stage, 0, x, k2, col0_is_mean=False)
+    for c in range(ncol):
+        ref = _bordered_solve(o, which, BC[name], k2[c], sigma, x[c])
+        # Dirichlet bottom: the sequential top fix is exact
+        assert rel_l2(got[c], ref) < 1e-11
+
+
+def test_helmholtz_quirk_sequential_top_fix():
+    # Neumann bottom: the top value is rebuilt from the already solved bottom
+    # value (cube_real_cplx.cc:483-488 applied at phase_carrier_flat.cc:984-985)
+    cfg = small_config(nx=6, ny=6, nz=33)
+    o = O.Oracle(cfg)
+    rng = np.random.default_rng(12)
+    n = 33
+    x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    k2 = -np.array([1.0, 9.0, 50.0])
+    got = o.helmholtz(3, 0, 1, x.copy(), k2, col0_is_mean=False)
+    _, _, _, fix = o.decomposition(3)
+    fbt = fix[3]
+    for c in range(3):
+        rhs = x[c].copy()
+        rhs[0] = rhs[-1] = 0.0
+        ref = _bordered_solve(o, 3, BC["neumann_neumann"], k2[c], 0.0, rhs)
+        # interior and bottom agree with the exact solve
+        assert rel_l2(got[c][:-1], ref[:-1]) < 1e-10
+        # top = fbt . [y0_solved, y_interior, f_top]
+        top = fbt[0] * ref[0] + fbt[1:-1] @ ref[1:-1] + fbt[-1] * 0.0
+        assert abs(got[c][-1] - top) < 1e-10 * max(1.0, abs(top))
+
+
+def test_pressure_null_mode_gated():
+    o, _ = _oracle_with(BC["dirichlet_neumann"], nz=33)
+    _, _, lam, _ = o.decomposition(3)
+    i = 1 + np.argmin(np.abs(lam[1:-1]))
+    assert abs(lam[i]) < 1e-9 * np.abs(lam[1:-1]).max()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((1, 33)) + 0j
+    y = o.helmholtz(3, 0, 1, x.copy(), np.array([0.0]), col0_is_mean=True)
+    assert np.all(np.isfinite(y))
+
+
+# --------------------------------------------------------------------------
+# time stepping: RK3 coefficients, laminar Stokes layer, scalar invariants
+# --------------------------------------------------------------------------
+def test_rk3_coefficients_order():
+    # low-storage scheme on y' = lam y with the reference's coefficients
+    # (stepper_time_rk3.cc:29-31): H = c1 H + dt f(y); y += c2 H
+    c1 = [0.0, -5.0 / 9.0, -153.0 / 128.0]
+    c2 = [1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0]
+    c3 = [1.0 / 6.0, 5.0 / 24.0, 1.0 / 8.0]
+    assert abs(sum(2 * c for c in c3) - 1.0) < 1e-15
+    lam = -1.3
+    errs = []
+    for n in (20, 40, 80):
+        dt = 1.0 / n
+        y = 1.0
+        for _ in range(n):
+            H = 0.0
+            for s in range(3):
+                H = c1[s] * H + dt * lam * y
+                y = y + c2[s] * H
+        errs.append(abs(y - math.exp(lam)))
+    order = math.log(errs[0] / errs[1], 2), math.log(errs[1] / errs[2], 2)
+    assert order[0] > 2.8 and order[1] > 2.8
+
+
+@pytest.mark.parametrize("top", ["freeslip", "noslip"])
+def test_stokes_layer_analytic(top):
+    cfg = W.base_config(6, 6, 65, 4 * math.pi, 2 * math.pi, 180.0, W.TWO_PI / 2000.0, nphase=1, top=top)
+    uvw, p, _ = W.initial_state(cfg, noise_u=0.0)
+    o = O.Oracle(cfg)
+    o.set_state(uvw, p)
+    for _ in range(300):
+        o.step()
+    _, t, _, _ = o.stepper()
+    u, _, _ = o.get_state()
+    U = u.reshape(3, o.mx, o.nyc, o.nz)[0, 0, 0].real
+    ref = W.stokes_profile(180.0, W.cgl_grid(65), t, top=top)
+    assert np.abs(U - ref).max() < 2e-6
+    # no spurious transverse or wall-normal mean flow
+    assert np.abs(u.reshape(3, o.mx, o.nyc, o.nz)[1:, 0, 0]).max() < 1e-12
+
+
+def test_stokes_layer_second_order_in_time():
+    errs = []
+    for ndt in (500, 1000):
+        cfg = W.base_config(6, 6, 49, 4 * math.pi, 2 * math.pi, 180.0, W.TWO_PI / ndt, nphase=1)
+        uvw, p, _ = W.initial_state(cfg, noise_u=0.0)
+        o = O.Oracle(cfg)
+        o.set_state(uvw, p)
+        for _ in range(ndt // 10):
+            o.step()
+        _, t, _, _ = o.stepper()
+        u, _, _ = o.get_state()
+        U = u.reshape(3, o.mx, o.nyc, o.nz)[0, 0, 0].real
+        errs.append(np.abs(U - W.stokes_profile(180.0, W.cgl_grid(49), t)).max())
+    assert errs[0] / errs[1] > 3.0   # CN: 4x per halving (>= 3 with start-up)
+
+
+def test_settling_diffusion_equilibrium():
+    # no flow, zero-flux walls: c ~ exp(-Re Sc ws z) is steady
+    cfg = small_config(nx=6, ny=6, nz=33, noise_u=0.0, noise_c=0.0, erosion=False)
+    cfg.update(force_type=0, scalar_bc=0, settling_velocity=2e-4)
+    re, sc, ws = cfg["reynolds"], cfg["schmidt"], cfg["settling_velocity"]
+    uvw, p, c = W.initial_state(cfg, noise_u=0.0, noise_c=0.0)
+    uvw[:] = 0
+    z = W.cgl_grid(33)
+    prof = np.exp(-re * sc * ws * (z + 1))
+    c[:] = 0
+    c[0, 0, 0, :] = prof
+    o = O.Oracle(cfg)
+    o.set_state(uvw, p, c)
+    for _ in range(20):
+        o.step()
+    _, _, cc = o.get_state()
+    C = cc.reshape(o.mx, o.nyc, o.nz)[0, 0].real
+    assert rel_l2(C, prof) < 1e-9
+
+
+def test_uniform_concentration_invariant():
+    # no settling, zero-flux walls, any flow: a uniform concentration is an
+    # exact solution (advection of a constant vanishes, D2 of a constant is 0)
+    cfg = small_config(nx=12, ny=12, nz=33, noise_u=0.05, noise_c=0.0, erosion=False)
+    cfg.update(scalar_bc=0, settling_velocity=0.0)
+    uvw, p, c = W.initial_state(cfg)
+    o = O.Oracle(cfg)
+    o.set_state(uvw, p, c)
+    for _ in range(6):
+        o.step()
+    _, _, cc = o.get_state()
+    C = cc.reshape(o.mx, o.nyc, o.nz)
+    # round-off amplified by the 1e7-scaled Robin rows stays below 1e-8
+    assert np.abs(C[0, 0].real - 1.0).max() < 1e-8
+    C[0, 0] = 0
+    assert np.abs(C).max() < 1e-8
+
+
+def test_divergence_free_after_correct():
+    cfg = small_config(nx=12, ny=12, nz=33)
+    uvw, p, c = W.initial_state(cfg)
+    o = O.Oracle(cfg)
+    o.set_state(uvw, p, c)
+    for _ in range(3):
+        o.step()
+    u, _, _ = o.get_state()
+    u = u.reshape(3, o.mx, o.nyc, o.nz)
+    _, d1, _ = o.operators()
+    kx = 2 * np.pi / cfg["lx"] * np.arange(o.mx)
+    ky1 = 2 * np.pi / cfg["ly"]
+    ky = np.array([ky1 * iy if iy <= o.nyc // 2 else -ky1 * (o.nyc - iy) for iy in range(o.nyc)])
+    div = (1j * kx[:, None, None] * u[0] + 1j * ky[None, :, None] * u[1] + u[2] @ d1.T)
+    # interior collocation points, relative to the velocity gradient scale
+    grad = np.abs(u[0] @ d1.T).max()
+    assert np.abs(div[:, :, 1:-1]).max() < 1e-8 * grad

@@ -1,0 +1,273 @@
+// C ABI (include/turbid_amd.h).  Exceptions never cross the boundary: they
+// become negative return codes with the message kept for tbd_last_error().
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/turbid_amd.h"
+#include "context.h"
+#include "fft.h"
+#include "pointwise.h"
+#include "zgemm.h"
+
+struct tbd_ctx {
+  tbd::Context* c;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  } catch (...) {
+    g_err = "unknown error";
+    return -2;
+  }
+}
+
+// RAII device buffer for the stand-alone entry points
+struct DBuf {
+  void* p = nullptr;
+  explicit DBuf(size_t n) { TBD_HIP(hipMalloc(&p, n < 256 ? 256 : n)); }
+  ~DBuf() { if (p) (void)hipFree(p); }
+  template <class T> T* as() { return (T*)p; }
+};
+}  // namespace
+
+extern "C" {
+
+const char* tbd_last_error(void) { return g_err.c_str(); }
+
+int tbd_create(const tbd_config* cfg, tbd_ctx** out) {
+  return guard([&] {
+    TBD_CHECK(cfg && out, "tbd_create > null argument");
+    *out = nullptr;
+    tbd_ctx* c = new tbd_ctx;
+    try {
+      c->c = new tbd::Context(*cfg);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+void tbd_destroy(tbd_ctx* ctx) {
+  if (!ctx) return;
+  try { delete ctx->c; } catch (...) {}
+  delete ctx;
+}
+
+int tbd_upload_state(tbd_ctx* ctx, const double* uvw, const double* p, const double* c) {
+  return guard([&] {
+    TBD_CHECK(ctx && uvw && p, "tbd_upload_state > null argument");
+    ctx->c->upload(uvw, p, c);
+  });
+}
+
+int tbd_download_state(tbd_ctx* ctx, double* uvw, double* p, double* c) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_download_state > null context");
+    ctx->c->download(uvw, p, c);
+  });
+}
+
+int tbd_download_physical_velocity(tbd_ctx* ctx, double* uvw) {
+  return guard([&] {
+    TBD_CHECK(ctx && uvw, "tbd_download_physical_velocity > null argument");
+    ctx->c->download_physical_velocity(uvw);
+  });
+}
+
+int tbd_substage(tbd_ctx* ctx, int stage, double time, int alternation) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_substage > null context");
+    ctx->c->substage(stage, time, alternation);
+  });
+}
+
+int tbd_step(tbd_ctx* ctx, double time, int alternation) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_step > null context");
+    for (int s = 0; s < 3; ++s) ctx->c->substage(s, time, alternation);
+  });
+}
+
+int tbd_set_dt(tbd_ctx* ctx, double dt) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_set_dt > null context");
+    ctx->c->set_dt(dt);
+  });
+}
+
+int tbd_features(tbd_ctx* ctx, double* out50) {
+  return guard([&] {
+    TBD_CHECK(ctx && out50, "tbd_features > null argument");
+    ctx->c->features(out50);
+  });
+}
+
+int tbd_sync(tbd_ctx* ctx) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_sync > null context");
+    ctx->c->sync();
+  });
+}
+
+int tbd_profile_enable(tbd_ctx* ctx, int enable) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_profile_enable > null context");
+    ctx->c->prof.on = enable != 0;
+  });
+}
+
+int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms) {
+  return guard([&] {
+    TBD_CHECK(ctx && kclass >= 0 && kclass < 4, "tbd_profile_read > bad argument");
+    ctx->c->sync();
+    ctx->c->prof.collect();
+    if (launches) *launches = ctx->c->prof.launches[kclass];
+    if (total_ms) *total_ms = ctx->c->prof.ms[kclass];
+  });
+}
+
+int tbd_profile_reset(tbd_ctx* ctx) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_profile_reset > null context");
+    ctx->c->sync();
+    ctx->c->prof.reset();
+  });
+}
+
+int tbd_get_operators(tbd_ctx* ctx, double* gridz, double* d1, double* d2) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_get_operators > null context");
+    const auto& ch = ctx->c->ch;
+    const size_t n = ch.nz;
+    if (gridz) std::memcpy(gridz, ch.gridz.data(), sizeof(double) * n);
+    if (d1) std::memcpy(d1, ch.d1.data(), sizeof(double) * n * n);
+    if (d2) std::memcpy(d2, ch.d2.data(), sizeof(double) * n * n);
+  });
+}
+
+int tbd_get_factorisation(tbd_ctx* ctx, int which, double* V, double* Vinv, double* lambda, double* fix) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_get_factorisation > null context");
+    TBD_CHECK(which >= 0 && which < (ctx->c->cfg.nphase > 1 ? 5 : 4), "tbd_get_factorisation > bad index");
+    const auto& f = ctx->c->fac[which];
+    const size_t n = f.nz;
+    if (V) std::memcpy(V, f.vnorm.data(), sizeof(double) * n * n);
+    if (Vinv) std::memcpy(Vinv, f.vinv.data(), sizeof(double) * n * n);
+    if (lambda) std::memcpy(lambda, f.lambda.data(), sizeof(double) * n);
+    if (fix) std::memcpy(fix, f.fix.data(), sizeof(double) * 4 * n);
+  });
+}
+
+int tbd_dft_forward(int nx, int ny, int nz, int nv, int fp32, const double* phys, double* spec) {
+  return guard([&] {
+    TBD_CHECK(phys && spec && nv >= 1, "tbd_dft_forward > bad argument");
+    const tbd::Dims d = tbd::Dims::make(nx, ny, nz);
+    tbd::FftPlan p;
+    p.create(d);
+    DBuf dphys(sizeof(double) * d.R * nv), dspec(sizeof(double2) * d.S * nv);
+    DBuf tmp(sizeof(double2) * tbd::fft_tmp_elems(d, nv));
+    TBD_HIP(hipMemcpy(dphys.p, phys, sizeof(double) * d.R * nv, hipMemcpyHostToDevice));
+    if (fp32) {
+      std::vector<float> h(phys, phys + d.R * nv);
+      DBuf fphys(sizeof(float) * d.R * nv);
+      TBD_HIP(hipMemcpy(fphys.p, h.data(), sizeof(float) * d.R * nv, hipMemcpyHostToDevice));
+      DBuf fspec(sizeof(float2) * d.S * nv);
+      tbd::fft_forward<float, float2>(p, fphys.as<float>(), d.R, fspec.as<float2>(), d.S, nv, 0, nz,
+                                      tmp.as<float2>(), 0);
+      std::vector<float> o(2 * d.S * nv);
+      TBD_HIP(hipMemcpy(o.data(), fspec.p, sizeof(float2) * d.S * nv, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < o.size(); ++i) spec[i] = o[i];
+    } else {
+      tbd::fft_forward<double, double2>(p, dphys.as<double>(), d.R, dspec.as<double2>(), d.S, nv, 0, nz,
+                                        tmp.as<double2>(), 0);
+      TBD_HIP(hipMemcpy(spec, dspec.p, sizeof(double2) * d.S * nv, hipMemcpyDeviceToHost));
+    }
+    TBD_HIP(hipDeviceSynchronize());
+    p.destroy();
+  });
+}
+
+int tbd_dft_backward(int nx, int ny, int nz, int nv, int fp32, const double* spec, double* phys) {
+  return guard([&] {
+    TBD_CHECK(phys && spec && nv >= 1, "tbd_dft_backward > bad argument");
+    const tbd::Dims d = tbd::Dims::make(nx, ny, nz);
+    tbd::FftPlan p;
+    p.create(d);
+    DBuf dspec(sizeof(double2) * d.S * nv), tmp(sizeof(double2) * tbd::fft_tmp_elems(d, nv));
+    TBD_HIP(hipMemcpy(dspec.p, spec, sizeof(double2) * d.S * nv, hipMemcpyHostToDevice));
+    if (fp32) {
+      DBuf fphys(sizeof(float) * d.R * nv);
+      tbd::fft_backward<float, double2>(p, dspec.as<double2>(), d.S, fphys.as<float>(), d.R, nv, 0, nz,
+                                        tmp.as<float2>(), 0);
+      std::vector<float> o(d.R * nv);
+      TBD_HIP(hipMemcpy(o.data(), fphys.p, sizeof(float) * d.R * nv, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < o.size(); ++i) phys[i] = o[i];
+    } else {
+      DBuf dphys(sizeof(double) * d.R * nv);
+      tbd::fft_backward<double, double2>(p, dspec.as<double2>(), d.S, dphys.as<double>(), d.R, nv, 0, nz,
+                                         tmp.as<double2>(), 0);
+      TBD_HIP(hipMemcpy(phys, dphys.p, sizeof(double) * d.R * nv, hipMemcpyDeviceToHost));
+    }
+    TBD_HIP(hipDeviceSynchronize());
+    p.destroy();
+  });
+}
+
+int tbd_zcontract(int nz, long long ncol, int fp32, const double* m, const double* x, double* y) {
+  return guard([&] {
+    TBD_CHECK(nz >= 4 && ncol >= 1 && m && x && y, "tbd_zcontract > bad argument");
+    const int lda = (nz + 3) / 4 * 4;
+    std::vector<double> rm((size_t)nz * lda, 0.0);
+    for (int i = 0; i < nz; ++i)
+      for (int j = 0; j < nz; ++j) rm[(size_t)i * lda + j] = m[i + (size_t)nz * j];
+    std::vector<double> kx(1, 0.0), ky(ncol, 0.0);
+    DBuf dk(sizeof(double) * 1), dky(sizeof(double) * ncol);
+    TBD_HIP(hipMemcpy(dk.p, kx.data(), sizeof(double), hipMemcpyHostToDevice));
+    TBD_HIP(hipMemcpy(dky.p, ky.data(), sizeof(double) * ncol, hipMemcpyHostToDevice));
+    const size_t n2 = (size_t)nz * ncol;
+    DBuf dx(sizeof(double2) * n2), dy(sizeof(double2) * n2);
+    TBD_HIP(hipMemcpy(dx.p, x, sizeof(double2) * n2, hipMemcpyHostToDevice));
+    tbd::ZArgs a;
+    a.mode = tbd::Z_PLAIN; a.nz = nz; a.lda = lda; a.P = ncol; a.nyc = (int)ncol;
+    a.x = dx.p; a.y = dy.p; a.kx = dk.as<double>(); a.ky = dky.as<double>();
+    if (fp32) {
+      std::vector<float> rf(rm.begin(), rm.end());
+      DBuf dm(sizeof(float) * rf.size()), fy(sizeof(float2) * n2);
+      TBD_HIP(hipMemcpy(dm.p, rf.data(), sizeof(float) * rf.size(), hipMemcpyHostToDevice));
+      a.op = dm.p;
+      a.y = fy.p;
+      tbd::zgemm<float, double2, float2>(a, 1, 0);
+      std::vector<float> o(2 * n2);
+      TBD_HIP(hipMemcpy(o.data(), fy.p, sizeof(float2) * n2, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < o.size(); ++i) y[i] = o[i];
+    } else {
+      DBuf dm(sizeof(double) * rm.size());
+      TBD_HIP(hipMemcpy(dm.p, rm.data(), sizeof(double) * rm.size(), hipMemcpyHostToDevice));
+      a.op = dm.p;
+      tbd::zgemm<double, double2, double2>(a, 1, 0);
+      TBD_HIP(hipMemcpy(y, dy.p, sizeof(double2) * n2, hipMemcpyDeviceToHost));
+    }
+    TBD_HIP(hipDeviceSynchronize());
+  });
+}
+
+int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x, double* y) {
+  return guard([&] {
+    TBD_CHECK(ctx && x && y && stage >= 0 && stage < 3 && (mode == 0 || mode == 1), "tbd_helmholtz > bad argument");
+    ctx->c->helmholtz_host(which, stage, mode, x, y);
+  });
+}
+
+}  // extern "C"

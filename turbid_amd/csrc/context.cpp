@@ -1,0 +1,605 @@
+// Substage orchestration on one stream (see context.h).  Each step below
+// cites the reference statement it implements.
+#include "context.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "pointwise.h"
+#include "zgemm.h"
+
+namespace tbd {
+
+static const double kPi = 3.14159265358979323846264338327950288;
+
+// ---------------------------------------------------------------------------
+// profiler: event pairs around launches of a kernel class
+// ---------------------------------------------------------------------------
+hipEvent_t Profiler::get() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  TBD_HIP(hipEventCreate(&e));
+  return e;
+}
+void Profiler::begin(int kclass, hipStream_t st, Rec& r) {
+  r.kclass = kclass;
+  r.a = get();
+  r.b = get();
+  TBD_HIP(hipEventRecord(r.a, st));
+}
+void Profiler::end(Rec& r, hipStream_t st) {
+  TBD_HIP(hipEventRecord(r.b, st));
+  recs.push_back(r);
+}
+void Profiler::collect() {
+  for (auto& r : recs) {
+    TBD_HIP(hipEventSynchronize(r.b));
+    float t = 0.f;
+    TBD_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    launches[r.kclass] += 1;
+    ms[r.kclass] += t;
+    pool.push_back(r.a);
+    pool.push_back(r.b);
+  }
+  recs.clear();
+}
+void Profiler::reset() {
+  collect();
+  for (int i = 0; i < 4; ++i) { launches[i] = 0; ms[i] = 0; }
+}
+Profiler::~Profiler() {
+  for (auto& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+template <class F>
+void Context::launch(int kclass, F&& f) {
+  if (!prof.on) {
+    f();
+    return;
+  }
+  Profiler::Rec r;
+  prof.begin(kclass, st_, r);
+  f();
+  prof.end(r, st_);
+}
+
+enum { KC_FFT = 0, KC_ZGEMM = 1, KC_HELM = 2, KC_OTHER = 3 };
+
+void* Context::alloc(size_t bytes) {
+  void* p = nullptr;
+  TBD_HIP(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+  allocs_.push_back(p);
+  fill_zero(p, std::max<size_t>(bytes, 256), st_);
+  return p;
+}
+
+// row-major copy of a column-major n*n matrix with leading dimension lda
+static std::vector<double> rowmajor(const std::vector<double>& cm, int n, int lda) {
+  std::vector<double> r((size_t)n * lda, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) r[(size_t)i * lda + j] = cm[i + (size_t)n * j];
+  return r;
+}
+
+Context::Context(const tbd_config& c) : cfg(c) {
+  TBD_CHECK(cfg.nx >= 2 && cfg.ny >= 2 && cfg.nz >= 4, "Context > grid too small");
+  TBD_CHECK(std::fabs(cfg.lz - 2) <= 10E-12,
+            "Domain(mpi_t,dcp_t,cfg_t) > Dimensionless z-length Should be Equal to Two!");
+  TBD_CHECK(cfg.nphase == 1 || cfg.nphase == 2, "Context > total_phase_number must be 1 or 2");
+  TBD_CHECK(cfg.precision == 0 || cfg.precision == 1, "Context > precision must be 0 (FP64) or 1 (mixed)");
+  d = Dims::make(cfg.nx, cfg.ny, cfg.nz);
+  TBD_CHECK(d.nyc >= 2 && d.mx >= 1, "Context > grid too small for the 3/2 rule");
+  TBD_HIP(hipStreamCreate(&st_));
+  fft_.create(d);
+  dt_ = olddt_ = cfg.dt;
+  const double c1[3] = {0.0 / 1.0, -5.0 / 9.0, -153.0 / 128.0};    // stepper_time_rk3.cc:29-31
+  const double c2[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+  const double c3[3] = {1.0 / 6.0, 5.0 / 24.0, 1.0 / 8.0};
+  for (int s = 0; s < 3; ++s) { coef_[0][s] = c1[s]; coef_[1][s] = c2[s]; coef_[2][s] = c3[s]; }
+
+  // operators
+  ch = make_chebyshev(cfg.nz, cfg.mapping, cfg.mapping_coefficient);
+  fac[0] = make_factorisation(ch, cfg.vt_u1);
+  fac[1] = std::equal(cfg.vt_u1, cfg.vt_u1 + 4, cfg.vt_u2) ? fac[0] : make_factorisation(ch, cfg.vt_u2);
+  for (int i = 0; i < 5; ++i) fac[1].vt[i] = cfg.vt_u2[i];
+  fac[2] = make_factorisation(ch, cfg.vt_u3);
+  fac[3] = make_factorisation(ch, cfg.vt_p);
+  if (cfg.nphase > 1) {   // Robin rows 1e7*(Re Sc ws, 1) on both walls (phase_scalar_flat.cc:194-200)
+    double bc[5] = {cfg.reynolds * cfg.schmidt * cfg.settling_velocity, 1,
+                    cfg.reynolds * cfg.schmidt * cfg.settling_velocity, 1, 0};
+    for (int i = 0; i < 4; ++i) bc[i] = bc[i] * 10E+6;
+    fac[4] = make_factorisation(ch, bc);
+  }
+  opentop_ = std::fabs(cfg.vt_u1[2] - 0) <= 10E-12 && std::fabs(cfg.vt_u1[3] - 1) <= 10E-12;   // boundary.cc:59-71
+
+  const int nz = d.nz;
+  lda_ = (nz + 3) / 4 * 4;
+  auto up = [&](const std::vector<double>& h) {
+    double* p = (double*)alloc(sizeof(double) * h.size());
+    TBD_HIP(hipMemcpy(p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    return p;
+  };
+  d1_ = up(rowmajor(ch.d1, nz, lda_));
+  d2_ = up(rowmajor(ch.d2, nz, lda_));
+  {
+    std::vector<double> r = rowmajor(ch.d1, nz, lda_);
+    std::vector<float> f(r.begin(), r.end());
+    d1f_ = (float*)alloc(sizeof(float) * f.size());
+    TBD_HIP(hipMemcpy(d1f_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  }
+  for (int k = 0; k < (cfg.nphase > 1 ? 5 : 4); ++k) {
+    vinv_[k] = up(rowmajor(fac[k].vinv, nz, lda_));
+    vnorm_[k] = up(rowmajor(fac[k].vnorm, nz, lda_));
+    lam_[k] = up(fac[k].lambda);
+    fix_[k] = up(fac[k].fix);
+  }
+  // Fourier multipliers with the reference's sign at iy = nyc/2 (stepper_space.cc:181-213)
+  std::vector<double> kx(d.mx), ky(d.nyc);
+  const double kx1 = 2 * kPi / cfg.lx, ky1 = 2 * kPi / cfg.ly;
+  for (int ix = 0; ix < d.mx; ++ix) kx[ix] = kx1 * (ix - 0);
+  for (int iy = 0; iy < d.nyc; ++iy) ky[iy] = (iy <= d.nyc / 2) ? +ky1 * (iy - 0) : -ky1 * (d.nyc - iy);
+  kx_ = up(kx);
+  ky_ = up(ky);
+  gridz_ = up(ch.gridz);
+  // grid lengths (domain.cc:304-332)
+  std::vector<double> gx(d.nx), gy(d.ny);
+  for (int i = 0; i < d.nx; ++i) gx[i] = cfg.lx / d.nx * i;
+  for (int i = 0; i < d.ny; ++i) gy[i] = cfg.ly / d.ny * i;
+  glx_ = 1e300;
+  gly_ = 1e300;
+  for (int i = 0; i + 1 < d.nx; ++i) glx_ = std::min(glx_, gx[i + 1] - gx[i]);
+  for (int i = 0; i + 1 < d.ny; ++i) gly_ = std::min(gly_, gy[i + 1] - gy[i]);
+  std::vector<double> glz(nz);
+  const std::vector<double>& g = ch.gridzmap;
+  for (int i = 0; i < nz; ++i) {
+    if (i == 0) glz[i] = (g[1] - g[0]) / 2;
+    else if (i == nz - 1) glz[i] = (g[nz - 1] - g[nz - 2]) / 2;
+    else glz[i] = std::min(g[i + 1] - g[i], g[i] - g[i - 1]);
+  }
+  glz_ = up(glz);
+
+  // buffers
+  const size_t S16 = sizeof(double2) * d.S;
+  const size_t ts = cfg.precision == 1 ? sizeof(float) : sizeof(double);
+  const size_t Sc = 2 * ts * d.S;     // spectral component in the transform precision
+  const size_t Rt = ts * d.R;
+  uvw_ = (double2*)alloc(3 * S16);
+  p_ = (double2*)alloc(S16);
+  huvw_ = (double2*)alloc(3 * S16);
+  suvw_ = (double2*)alloc(3 * S16);
+  rhs_ = (double2*)alloc(3 * S16);
+  rhsp_ = (double2*)alloc(S16);
+  grad_ = alloc(9 * Sc);
+  adv_ = alloc(3 * Sc);
+  prod_ = alloc(6 * Sc);
+  uvwnl_ = alloc(3 * Rt);
+  gphys_ = alloc(9 * Rt);
+  aphys_ = alloc(3 * Rt);
+  tmp_ = alloc(std::max((size_t)9 * 2 * ts, sizeof(double2)) * fft_tmp_elems(d, 1));
+  for (int s = 0; s < 2; ++s)
+    for (int b = 0; b < 2; ++b) {
+      wall_[s][b] = (double2*)alloc(sizeof(double2) * 3 * d.P);
+      wall2_[s][b] = (double2*)alloc(sizeof(double2) * 2 * d.P);
+    }
+  if (cfg.nphase > 1) {
+    c_ = (double2*)alloc(S16);
+    hc_ = (double2*)alloc(S16);
+    rhsc_ = (double2*)alloc(S16);
+    advc_ = alloc(Sc);
+    cphys_ = (double*)alloc(sizeof(double) * d.R);
+    const size_t plane = (size_t)d.nx * d.ny;
+    dudz_bot_ = alloc(ts * plane);
+    dvdz_bot_ = alloc(ts * plane);
+    ephys_ = (double*)alloc(sizeof(double) * plane);
+    ehat_ = (double2*)alloc(sizeof(double2) * d.P);
+    wrows_ = (double2*)alloc(sizeof(double2) * 4 * d.P);
+  }
+  fl_ = (double*)alloc(sizeof(double) * 64);
+  fg_ = (double*)alloc(sizeof(double) * 64);
+  wallmean_ = (double*)alloc(sizeof(double) * 8);
+  TBD_HIP(hipStreamSynchronize(st_));
+}
+
+Context::~Context() {
+  if (st_) (void)hipStreamSynchronize(st_);
+  for (void* p : allocs_) (void)hipFree(p);
+  fft_.destroy();
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Context::sync() { TBD_HIP(hipStreamSynchronize(st_)); }
+
+void Context::set_dt(double dt) {
+  TBD_CHECK(dt > 0, "Context::set_dt > dt must be positive");
+  dt_ = dt;
+}
+
+void Context::upload(const double* uvw, const double* p, const double* c) {
+  const size_t S16 = sizeof(double2) * d.S;
+  TBD_HIP(hipMemcpyAsync(uvw_, uvw, 3 * S16, hipMemcpyHostToDevice, st_));
+  TBD_HIP(hipMemcpyAsync(p_, p, S16, hipMemcpyHostToDevice, st_));
+  if (cfg.nphase > 1) {
+    TBD_CHECK(c != nullptr, "FlatScalarVariable::Load > concentration missing");
+    TBD_HIP(hipMemcpyAsync(c_, c, S16, hipMemcpyHostToDevice, st_));
+    fill_zero(hc_, S16, st_);
+  }
+  fill_zero(huvw_, 3 * S16, st_);
+  for (int s = 0; s < 2; ++s)
+    for (int b = 0; b < 2; ++b) {
+      fill_zero(wall_[s][b], sizeof(double2) * 3 * d.P, st_);
+      fill_zero(wall2_[s][b], sizeof(double2) * 2 * d.P, st_);
+    }
+  olddt_ = dt_;
+  // uvwnonlinear = DftBackward(uvw) (phase_carrier_flat.cc:233-234)
+  if (cfg.precision == 1)
+    fft_backward<float, double2>(fft_, uvw_, d.S, (float*)uvwnl_, d.R, 3, 0, d.nz, (float2*)tmp_, st_);
+  else
+    fft_backward<double, double2>(fft_, uvw_, d.S, (double*)uvwnl_, d.R, 3, 0, d.nz, (double2*)tmp_, st_);
+  sync();
+}
+
+void Context::download(double* uvw, double* p, double* c) {
+  sync();
+  const size_t S16 = sizeof(double2) * d.S;
+  if (uvw) TBD_HIP(hipMemcpy(uvw, uvw_, 3 * S16, hipMemcpyDeviceToHost));
+  if (p) TBD_HIP(hipMemcpy(p, p_, S16, hipMemcpyDeviceToHost));
+  if (c && cfg.nphase > 1) TBD_HIP(hipMemcpy(c, c_, S16, hipMemcpyDeviceToHost));
+}
+
+void Context::download_physical_velocity(double* out) {
+  sync();
+  if (cfg.precision == 1) {
+    std::vector<float> h(3 * d.R);
+    TBD_HIP(hipMemcpy(h.data(), uvwnl_, sizeof(float) * 3 * d.R, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < 3 * d.R; ++i) out[i] = h[i];
+  } else {
+    TBD_HIP(hipMemcpy(out, uvwnl_, sizeof(double) * 3 * d.R, hipMemcpyDeviceToHost));
+  }
+}
+
+void Context::features(double* out50) {
+  sync();
+  TBD_HIP(hipMemcpy(out50, fg_, sizeof(double) * 50, hipMemcpyDeviceToHost));
+}
+
+// driven force on the mean mode (phase_carrier_flat.cc:360-502)
+void Context::forces(double time, double& f1, double& f2) const {
+  f1 = f2 = 0;
+  const double t = time;
+  const double dt = dt_;
+  const double tm = cfg.output_on_step ? (t - dt / 2) : (t + dt / 2);
+  const int d0 = cfg.force_direction[0], d1 = cfg.force_direction[1];
+  const double* a = cfg.force_amplitude;
+  switch (cfg.force_type) {
+    case 0: break;
+    case 1:
+      if (d0 == 1) f1 = a[0]; else if (d0 == 2) f2 = a[0];
+      break;
+    case 2: {
+      const double f = a[1];
+      if (d1 == 1) f1 = f * std::cos(f * tm); else if (d1 == 2) f2 = f * std::cos(f * tm);
+    } break;
+    case 3: {
+      if (d0 == 1) f1 = a[0]; else if (d0 == 2) f2 = a[0];
+      const double f = a[1];
+      if (d1 == 1) f1 = f1 + f * std::cos(f * tm); else if (d1 == 2) f2 = f2 + f * std::cos(f * tm);
+    } break;
+    case 4: {
+      if (d0 == 1) f1 = a[0]; else if (d0 == 2) f2 = a[0];
+      const double f = a[1];
+      const double T = a[0], t0 = a[1], last = a[2], tar = a[3];
+      const double damp = -std::log(tar) / (last + 0.25) / T;
+      const double gv = (f * std::cos(f * tm) - damp * std::sin(f * tm)) * std::exp(-damp * (t - t0));
+      if (d1 == 1) f1 = f1 + gv; else if (d1 == 2) f2 = f2 + gv;
+    } break;
+    default:
+      throw Error("FlatCarrierPhase::Predict() > Not Supported Force Type!");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FlatCarrierPhase::Predict (phase_carrier_flat.cc:272-559)
+// ---------------------------------------------------------------------------
+template <typename T>
+void Context::carrier_predict(int stage, double time, int alternation) {
+  typedef C2<T> CT;
+  const double c1 = coef_[0][stage], c2 = coef_[1][stage], c3 = coef_[2][stage];
+  const long long S = d.S, R = d.R;
+  CT* grad = (CT*)grad_;
+  CT* adv = (CT*)adv_;
+  CT* prod = (CT*)prod_;
+  T* uvwnl = (T*)uvwnl_;
+  T* gphys = (T*)gphys_;
+  T* aphys = (T*)aphys_;
+  CT* tmp = (CT*)tmp_;
+  const T* d1op = std::is_same<T, float>::value ? (const T*)d1f_ : (const T*)d1_;
+
+  // wall gradients of the mean mode for the log (dudxyz(2) rows 0, N-1 col 0)
+  launch(KC_OTHER, [&] { wall_mean_rows(d1_, lda_, d.nz, uvw_, S, wallmean_, st_); });
+  if (!alternation) {
+    // convective form (phase_carrier_flat.cc:784-806)
+    ZArgs a;
+    a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+    a.x = uvw_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
+    launch(KC_ZGEMM, [&] { zgemm<T, double2, CT>(a, 3, st_); });
+    launch(KC_FFT, [&] { fft_backward<T, CT>(fft_, grad, S, gphys, R, 9, 0, d.nz, tmp, st_); });
+    launch(KC_OTHER, [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
+    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, aphys, R, adv, S, 3, 0, d.nz, tmp, st_); });
+    if (cfg.nphase > 1 && cfg.scalar_bc == 1) {
+      const size_t pl = (size_t)d.nx * d.ny * sizeof(T);
+      TBD_HIP(hipMemcpyAsync(dudz_bot_, gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
+      TBD_HIP(hipMemcpyAsync(dvdz_bot_, gphys + 5 * R, pl, hipMemcpyDeviceToDevice, st_));
+    }
+  } else {
+    // divergence form (phase_carrier_flat.cc:725-779): six distinct products
+    launch(KC_OTHER, [&] { products_div<T>(uvwnl, gphys, R, st_); });
+    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, gphys, R, prod, S, 6, 0, d.nz, tmp, st_); });
+    // adv_i = (i kx F_i0 + i ky F_i1) + D1 F_i2 ; rows (uu,uv,uw) (uv,vv,vw) (uw,vw,ww)
+    const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int f = 0; f < 3; ++f) {
+      ZArgs a;
+      a.mode = Z_GRADTMP; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+      a.x = prod + rows[f][2] * S; a.e0 = prod + rows[f][0] * S; a.e1 = prod + rows[f][1] * S;
+      a.y = adv + f * S; a.kx = kx_; a.ky = ky_;
+      launch(KC_ZGEMM, [&] { zgemm<T, CT, CT>(a, 1, st_); });
+    }
+    // Gradient(u), Gradient(v) + DftBackward for the erosion map: only the
+    // bottom plane of du/dz, dv/dz is read (phase_scalar_flat.cc:349-351)
+    if (cfg.nphase > 1 && cfg.scalar_bc == 1) {
+      launch(KC_OTHER, [&] {
+        zrows_wall(d1_, lda_, d.nz, uvw_, d.P, wrows_, wrows_ + d.P, st_);
+        zrows_wall(d1_, lda_, d.nz, uvw_ + S, d.P, wrows_ + 2 * d.P, wrows_ + 3 * d.P, st_);
+      });
+      const long long plane = (long long)d.nx * d.ny;
+      launch(KC_FFT, [&] {
+        fft_backward<T, double2>(fft_, wrows_, 2 * d.P, (T*)dudz_bot_, plane, 1, 0, 1, tmp, st_, 1);
+        fft_backward<T, double2>(fft_, wrows_ + 2 * d.P, 2 * d.P, (T*)dvdz_bot_, plane, 1, 0, 1, tmp, st_, 1);
+      });
+    }
+  }
+  // H = c1 H - dt adv, buoyancy, mean-mode force (:294-504)
+  HUpd u;
+  u.c1 = c1; u.dt = dt_; u.nz = d.nz;
+  u.buoy = (cfg.nphase > 1 && cfg.affected_by_scalar_phase) ? 1 : 0;
+  u.K = dt_ * cfg.averaged_concentration / std::pow(cfg.froude, 2) * (cfg.specific_gravity - 1);
+  u.sn = std::sin(cfg.slope); u.cs = std::cos(cfg.slope); u.slope_dir = cfg.slope_direction;
+  forces(time, u.f1, u.f2);
+  launch(KC_OTHER, [&] { h_update<CT>(huvw_, adv, c_, u, 3, S, st_); });
+  // lap = D2 u - kx^2 u - ky^2 u ; rhs = sigma (c2 H + u) - lap (:517-536)
+  {
+    const double redtc3 = -cfg.reynolds / dt_ / c3;
+    ZArgs a;
+    a.mode = Z_RHS; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d2_;
+    a.x = uvw_; a.xs = S; a.y = rhs_; a.ys = S; a.h = huvw_; a.hs = S; a.kx = kx_; a.ky = ky_;
+    a.s1 = redtc3; a.s2 = redtc3 * c2;
+    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 3, st_); });
+  }
+  // boundary rows from the extrapolated wall pressure gradient (boundary_carrier_flat.cc:185-246)
+  {
+    const double c3m0 = coef_[2][stage], c3m1 = coef_[2][(stage + 2) % 3], c3m2 = coef_[2][(stage + 1) % 3];
+    double xm0 = 0, xm1 = 0, xm2 = 0;
+    switch (stage) {
+      case 0: xm0 = 2 * (0 + c3m0) * dt_; xm1 = 0; xm2 = 2 * (0 - c3m1) * olddt_; break;
+      case 1: xm0 = 2 * (c3m0 + c3m1 + 0) * dt_; xm1 = 2 * (0 + c3m1 + 0) * dt_; xm2 = 0; break;
+      default: xm0 = 2 * (c3m0 + c3m1 + c3m2) * dt_; xm1 = 2 * (0 + c3m1 + c3m2) * dt_; xm2 = 2 * (0 + +0 + c3m2) * dt_;
+    }
+    BcArgs b;
+    b.nz = d.nz; b.P = d.P; b.S = S;
+    b.s = (xm0 - xm1) / (xm1 - xm2);
+    b.k = 2 * dt_ * c3m0;
+    b.m1b = wall_[wslot_][0]; b.m1t = wall_[wslot_][1];
+    b.m2b = wall_[1 - wslot_][0]; b.m2t = wall_[1 - wslot_][1];
+    b.n1t = wall2_[wslot_][1]; b.n2t = wall2_[1 - wslot_][1];
+    b.open_top = opentop_ ? 1 : 0;
+    b.top0 = cfg.vt_u1[4]; b.top1 = cfg.vt_u2[4]; b.top2 = cfg.vt_u3[4];
+    launch(KC_OTHER, [&] { bc_carrier(rhs_, b, st_); });
+  }
+  // intermediate velocity: three Helmholtz solves (:554, :867-931)
+  {
+    HArgs h;
+    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 0; h.kx = kx_; h.ky = ky_;
+    for (int f = 0; f < 3; ++f) {
+      h.f[f].vinv = vinv_[f]; h.f[f].vnorm = vnorm_[f]; h.f[f].lambda = lam_[f]; h.f[f].fix = fix_[f];
+      h.f[f].sigma = -cfg.reynolds / dt_ / coef_[2][stage];
+      h.f[f].nullrow = -1;
+      h.f[f].x = rhs_ + f * S;
+      h.f[f].y = suvw_ + f * S;
+    }
+    launch(KC_HELM, [&] { helmholtz(h, 3, st_); });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FlatCarrierPhase::Correct (phase_carrier_flat.cc:566-709)
+// ---------------------------------------------------------------------------
+template <typename T>
+void Context::carrier_correct(int stage) {
+  const double c3 = coef_[2][stage];
+  const long long S = d.S, R = d.R;
+  // div s, rhs_p = div / (2 dt c3) (:586-589)
+  {
+    ZArgs a;
+    a.mode = Z_DIV; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1_;
+    a.x = suvw_ + 2 * S; a.e0 = suvw_; a.e1 = suvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
+    a.q = 0.5 / dt_ / c3;
+    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+  }
+  // pressure solve (:594, :955-987)
+  {
+    HArgs h;
+    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 1; h.kx = kx_; h.ky = ky_;
+    h.f[0].vinv = vinv_[3]; h.f[0].vnorm = vnorm_[3]; h.f[0].lambda = lam_[3]; h.f[0].fix = fix_[3];
+    h.f[0].sigma = 0.0;
+    h.f[0].nullrow = fac[3].nullrow;
+    h.f[0].x = rhsp_;
+    h.f[0].y = p_;
+    launch(KC_HELM, [&] { helmholtz(h, 1, st_); });
+  }
+  // grad p, boundary Update (m2 <- m1, m1 <- walls), u = s - 2 dt c3 grad p,
+  // mean w := 0 (:597-620, boundary_carrier_flat.cc:131-170)
+  if (stage == 2) olddt_ = dt_;
+  wslot_ = 1 - wslot_;
+  {
+    ZArgs a;
+    a.mode = Z_CORRECT; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1_;
+    a.x = p_; a.e0 = suvw_; a.es = S; a.y = uvw_; a.ys = S; a.kx = kx_; a.ky = ky_;
+    a.cf = 2 * dt_ * c3;
+    a.wall_bot = wall_[wslot_][0]; a.wall_top = wall_[wslot_][1];
+    a.wall2_bot = wall2_[wslot_][0]; a.wall2_top = wall2_[wslot_][1];
+    a.open_top = opentop_ ? 1 : 0;
+    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+  }
+  // uvwnonlinear = DftBackward(uvw) (:625-626)
+  launch(KC_FFT, [&] { fft_backward<T, double2>(fft_, uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_, st_); });
+  if (stage == 2) {   // diagnostics (:633-703)
+    fill_zero(fl_, sizeof(double) * 20, st_);
+    launch(KC_OTHER, [&] {
+      diagnostics<T>((const T*)uvwnl_, R, d.nx, d.ny, d.nz, glx_, gly_, glz_, fl_, st_);
+    });
+    TBD_HIP(hipMemcpyAsync(fg_, fl_, sizeof(double) * 50, hipMemcpyDeviceToDevice, st_));
+    // 10..15: dU/dz, dV/dz, dW/dz on the mean column at bottom, then top
+    const int src[6] = {0, 2, 4, 1, 3, 5};
+    for (int i = 0; i < 6; ++i)
+      TBD_HIP(hipMemcpyAsync(fg_ + 10 + i, wallmean_ + src[i], sizeof(double), hipMemcpyDeviceToDevice, st_));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FlatScalarPhase::Predict + Correct (phase_scalar_flat.cc:245-533)
+// ---------------------------------------------------------------------------
+template <typename T>
+void Context::scalar_correct(int stage, int alternation) {
+  typedef C2<T> CT;
+  const double c1 = coef_[0][stage], c2 = coef_[1][stage], c3 = coef_[2][stage];
+  const long long S = d.S, R = d.R;
+  T* uvwnl = (T*)uvwnl_;
+  T* gphys = (T*)gphys_;
+  T* aphys = (T*)aphys_;
+  CT* grad = (CT*)grad_;
+  CT* prod = (CT*)prod_;
+  CT* advc = (CT*)advc_;
+  CT* tmp = (CT*)tmp_;
+  const T* d1op = std::is_same<T, float>::value ? (const T*)d1f_ : (const T*)d1_;
+  // advecting velocity = carrier velocity + settling shift on the mean mode
+  // (:256-279); its physical form is the carrier's plus constants
+  const double ws = cfg.settling_velocity;
+  const T su = (T)(cfg.slope_direction == 1 ? ws * std::sin(cfg.slope) : 0.0);
+  const T sv = (T)(cfg.slope_direction == 2 ? ws * std::sin(cfg.slope) : 0.0);
+  const T sw = (T)(-ws * std::cos(cfg.slope));
+  if (alternation) {   // divergence form (:563-580)
+    launch(KC_FFT, [&] { fft_backward<T, double2>(fft_, c_, S, aphys, R, 1, 0, d.nz, tmp, st_); });
+    launch(KC_OTHER, [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
+    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp, st_); });
+    ZArgs a;
+    a.mode = Z_GRADTMP; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+    a.x = prod + 2 * S; a.e0 = prod; a.e1 = prod + S; a.y = advc; a.kx = kx_; a.ky = ky_;
+    launch(KC_ZGEMM, [&] { zgemm<T, CT, CT>(a, 1, st_); });
+  } else {             // convective form (:584-592)
+    ZArgs a;
+    a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+    a.x = c_; a.xs = S; a.y = grad + 6 * S; a.ys = S; a.kx = kx_; a.ky = ky_;
+    launch(KC_ZGEMM, [&] { zgemm<T, double2, CT>(a, 1, st_); });
+    launch(KC_FFT, [&] { fft_backward<T, CT>(fft_, grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp, st_); });
+    launch(KC_OTHER, [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
+    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, aphys, R, advc, S, 1, 0, d.nz, tmp, st_); });
+  }
+  // H = c1 H - dt adv (:313-315)
+  {
+    HUpd u;
+    u.c1 = c1; u.dt = dt_; u.nz = 0;
+    launch(KC_OTHER, [&] { h_update<CT>(hc_, advc, c_, u, 1, S, st_); });
+  }
+  // lap, rhs (:319-329)
+  {
+    const double redtc3 = -cfg.reynolds * cfg.schmidt / dt_ / c3;
+    ZArgs a;
+    a.mode = Z_RHS; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d2_;
+    a.x = c_; a.xs = S; a.y = rhsc_; a.ys = S; a.h = hc_; a.hs = S; a.kx = kx_; a.ky = ky_;
+    a.s1 = redtc3; a.s2 = redtc3 * c2;
+    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+  }
+  // boundary rows + erosion/deposition (boundary_scalar_flat.cc:151-152, :340-384)
+  const int erosion = cfg.scalar_bc == 1 ? 1 : 0;
+  if (erosion) {
+    const long long plane = (long long)d.nx * d.ny;
+    launch(KC_OTHER, [&] {
+      erosion_map<T>((const T*)dudz_bot_, (const T*)dvdz_bot_, ephys_, plane, cfg.reynolds,
+                     cfg.critical_shear_stress, cfg.erosion_rate, st_);
+    });
+    launch(KC_FFT, [&] {
+      fft_forward<double, double2>(fft_, ephys_, plane, ehat_, d.P, 1, 0, 1, (double2*)tmp_, st_, 1);
+    });
+  }
+  const double wsc = cfg.settling_velocity * std::cos(cfg.slope);
+  launch(KC_OTHER, [&] {
+    bc_scalar(rhsc_, d.nz, d.P, ehat_, c_, wsc, cfg.averaged_concentration, cfg.reynolds, cfg.schmidt, erosion,
+              st_);
+  });
+  if (erosion) {   // feature 28: mean erosion flux, 29: mean deposition flux (:377-380)
+    TBD_HIP(hipMemcpyAsync(fl_ + 28, ehat_, sizeof(double), hipMemcpyDeviceToDevice, st_));
+    launch(KC_OTHER, [&] { scaled_real(c_, wsc, cfg.averaged_concentration, fl_ + 29, st_); });
+  }
+  // Helmholtz (:399, :640-661)
+  {
+    HArgs h;
+    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 0; h.kx = kx_; h.ky = ky_;
+    h.f[0].vinv = vinv_[4]; h.f[0].vnorm = vnorm_[4]; h.f[0].lambda = lam_[4]; h.f[0].fix = fix_[4];
+    h.f[0].sigma = -cfg.reynolds * cfg.schmidt / dt_ / coef_[2][stage];
+    h.f[0].nullrow = -1;
+    h.f[0].x = rhsc_;
+    h.f[0].y = c_;
+    launch(KC_HELM, [&] { helmholtz(h, 1, st_); });
+  }
+  // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
+  launch(KC_FFT, [&] { fft_backward<double, double2>(fft_, c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_, st_); });
+  launch(KC_OTHER, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, st_); });
+  launch(KC_FFT, [&] { fft_forward<double, double2>(fft_, cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_, st_); });
+  launch(KC_OTHER, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });
+}
+
+void Context::substage(int stage, double time, int alternation) {
+  TBD_CHECK(stage >= 0 && stage < 3, "Context::substage > stage must be 0, 1 or 2");
+  if (cfg.precision == 1) {
+    carrier_predict<float>(stage, time, alternation);
+    carrier_correct<float>(stage);
+    if (cfg.nphase > 1) scalar_correct<float>(stage, alternation);
+  } else {
+    carrier_predict<double>(stage, time, alternation);
+    carrier_correct<double>(stage);
+    if (cfg.nphase > 1) scalar_correct<double>(stage, alternation);
+  }
+  if (stage == 2 && cfg.nphase > 1) {
+    // the log keeps the scalar values of stage 1 (copied at the carrier's stage 2)
+  }
+}
+
+void Context::helmholtz_host(int which, int stage, int mode, const double* x, double* y) {
+  TBD_CHECK(which >= 0 && which < (cfg.nphase > 1 ? 5 : 4), "Context::helmholtz > bad factorisation index");
+  const size_t S16 = sizeof(double2) * d.S;
+  double2* dx = (double2*)alloc(S16);
+  double2* dy = (double2*)alloc(S16);
+  TBD_HIP(hipMemcpyAsync(dx, x, S16, hipMemcpyHostToDevice, st_));
+  HArgs h;
+  h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = mode; h.kx = kx_; h.ky = ky_;
+  h.f[0].vinv = vinv_[which]; h.f[0].vnorm = vnorm_[which]; h.f[0].lambda = lam_[which]; h.f[0].fix = fix_[which];
+  double sig = 0.0;
+  if (which < 3) sig = -cfg.reynolds / dt_ / coef_[2][stage];
+  else if (which == 4) sig = -cfg.reynolds * cfg.schmidt / dt_ / coef_[2][stage];
+  h.f[0].sigma = sig;
+  h.f[0].nullrow = which == 3 ? fac[3].nullrow : -1;
+  h.f[0].x = dx;
+  h.f[0].y = dy;
+  helmholtz(h, 1, st_);
+  TBD_HIP(hipMemcpyAsync(y, dy, S16, hipMemcpyDeviceToHost, st_));
+  sync();
+  // temporary buffers are released with the context
+}
+
+}  // namespace tbd

@@ -1,0 +1,112 @@
+// One DNS on one GPU: operators, device buffers and the substage sequence of
+// FlatCarrierPhase / FlatScalarPhase (reference src/phase_carrier_flat.cc,
+// src/phase_scalar_flat.cc), launched on one HIP stream.
+#pragma once
+
+#include <vector>
+
+#include "../../include/turbid_amd.h"
+#include "common.h"
+#include "fft.h"
+#include "setup.h"
+
+namespace tbd {
+
+struct Profiler {
+  bool on = false;
+  struct Rec { int kclass; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  long long launches[4] = {0, 0, 0, 0};
+  double ms[4] = {0, 0, 0, 0};
+  hipEvent_t get();
+  void begin(int kclass, hipStream_t st, Rec& r);
+  void end(Rec& r, hipStream_t st);
+  void collect();
+  void reset();
+  ~Profiler();
+};
+
+class Context {
+ public:
+  explicit Context(const tbd_config& cfg);
+  ~Context();
+
+  void upload(const double* uvw, const double* p, const double* c);
+  void download(double* uvw, double* p, double* c);
+  void download_physical_velocity(double* out);
+  void substage(int stage, double time, int alternation);
+  void set_dt(double dt);
+  void features(double* out50);
+  void sync();
+  void helmholtz_host(int which, int stage, int mode, const double* x, double* y);
+
+  tbd_config cfg;
+  Dims d;
+  Chebyshev ch;
+  Factorisation fac[5];
+  Profiler prof;
+
+ private:
+  template <typename T> void carrier_predict(int stage, double time, int alternation);
+  template <typename T> void carrier_correct(int stage);
+  template <typename T> void scalar_correct(int stage, int alternation);
+  void forces(double time, double& f1, double& f2) const;
+  template <class F> void launch(int kclass, F&& f);
+  void* alloc(size_t bytes);
+
+  hipStream_t st_ = nullptr;
+  FftPlan fft_;
+  int lda_ = 0;
+  bool opentop_ = false;
+  double dt_ = 0, olddt_ = 0;
+  double coef_[3][3];
+
+  // operators (row-major, padded lda)
+  double* d1_ = nullptr;
+  double* d2_ = nullptr;
+  float* d1f_ = nullptr;
+  double* vinv_[5] = {nullptr};
+  double* vnorm_[5] = {nullptr};
+  double* lam_[5] = {nullptr};
+  double* fix_[5] = {nullptr};
+  double* kx_ = nullptr;
+  double* ky_ = nullptr;
+  double* gridz_ = nullptr;
+  double* glz_ = nullptr;
+  double glx_ = 0, gly_ = 0;
+
+  // spectral FP64 state and work (S complex per component)
+  double2 *uvw_ = nullptr, *p_ = nullptr, *c_ = nullptr;
+  double2 *huvw_ = nullptr, *hc_ = nullptr, *suvw_ = nullptr;
+  double2 *rhs_ = nullptr, *rhsp_ = nullptr, *rhsc_ = nullptr;
+  // nonlinear-term spectral work in the transform precision (C2<T>)
+  void* grad_ = nullptr;   // 9 components
+  void* adv_ = nullptr;    // 3
+  void* prod_ = nullptr;   // 6
+  void* advc_ = nullptr;   // 1
+  // physical work in the transform precision (T)
+  void* uvwnl_ = nullptr;  // 3
+  void* gphys_ = nullptr;  // 9
+  void* aphys_ = nullptr;  // 3
+  double* cphys_ = nullptr;
+  void* tmp_ = nullptr;    // transform intermediate (bytes for 9 components)
+  // wall rows of grad p: m1/m2 (bottom, top) ping-pong, and open-top rows
+  double2* wall_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [slot][bottom/top], 3P each
+  double2* wall2_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // 2P each
+  int wslot_ = 0;          // slot holding m1
+  // erosion
+  void* dudz_bot_ = nullptr;   // T, plane
+  void* dvdz_bot_ = nullptr;
+  double* ephys_ = nullptr;
+  double2* ehat_ = nullptr;
+  double2* wrows_ = nullptr;   // 4P: bottom/top rows of D1 u, D1 v
+  // features
+  double* fl_ = nullptr;       // featurelocal_ (64)
+  double* fg_ = nullptr;       // featureglobal_ (64)
+  double* wallmean_ = nullptr; // 6: D1 u/v/w rows 0 and nz-1 on the mean column
+
+  std::vector<void*> allocs_;
+};
+
+}  // namespace tbd

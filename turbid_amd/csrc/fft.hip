@@ -1,0 +1,422 @@
+// LDS-staged Stockham transforms for the plane DFTs (see fft.h).
+//
+// A workgroup owns a batch of equal-length sequences in LDS.  Each stage of
+// radix R in {4, 2, 3} reads all butterfly inputs into registers, barriers,
+// and writes the outputs back in self-sorting (Stockham) order, so a single
+// LDS buffer suffices.  Truncation, zero padding, the real<->complex packing
+// of two rows into one complex sequence, the 1/(Nx*Ny) scale and the
+// transposes to and from the z-fastest spectral layout are fused into the
+// global loads/stores of the passes.
+#include "fft.h"
+
+#include <cmath>
+#include <vector>
+
+namespace tbd {
+
+FftLen make_fftlen(int n) {
+  FftLen L;
+  L.n = n;
+  int m = n;
+  while (m % 4 == 0 && L.nstage < 12) { L.radix[L.nstage++] = 4; m /= 4; }
+  while (m % 2 == 0 && L.nstage < 12) { L.radix[L.nstage++] = 2; m /= 2; }
+  while (m % 3 == 0 && L.nstage < 12) { L.radix[L.nstage++] = 3; m /= 3; }
+  TBD_CHECK(m == 1, "FFT length " + std::to_string(n) + " must be of the form 2^a 3^b");
+  return L;
+}
+
+void FftPlan::create(const Dims& dims) {
+  d = dims;
+  lx = make_fftlen(d.nx);
+  ly = make_fftlen(d.ny);
+  auto mk_tw = [](int n, double2** t64, float2** t32) {
+    std::vector<double2> a(n);
+    std::vector<float2> b(n);
+    for (int t = 0; t < n; ++t) {
+      const double ang = -2.0 * M_PI * (double)t / (double)n;
+      a[t].x = std::cos(ang);
+      a[t].y = std::sin(ang);
+      b[t].x = (float)a[t].x;
+      b[t].y = (float)a[t].y;
+    }
+    TBD_HIP(hipMalloc(t64, sizeof(double2) * n));
+    TBD_HIP(hipMalloc(t32, sizeof(float2) * n));
+    TBD_HIP(hipMemcpy(*t64, a.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    TBD_HIP(hipMemcpy(*t32, b.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  };
+  mk_tw(d.nx, &twx64, &twx32);
+  mk_tw(d.ny, &twy64, &twy32);
+}
+
+void FftPlan::destroy() {
+  if (twx64) (void)hipFree(twx64);
+  if (twy64) (void)hipFree(twy64);
+  if (twx32) (void)hipFree(twx32);
+  if (twy32) (void)hipFree(twy32);
+  twx64 = twy64 = nullptr;
+  twx32 = twy32 = nullptr;
+}
+
+template <typename T> __host__ __device__ inline const C2<T>* twx(const FftPlan& p);
+template <> __host__ __device__ inline const double2* twx<double>(const FftPlan& p) { return p.twx64; }
+template <> __host__ __device__ inline const float2* twx<float>(const FftPlan& p) { return p.twx32; }
+template <typename T> __host__ __device__ inline const C2<T>* twy(const FftPlan& p);
+template <> __host__ __device__ inline const double2* twy<double>(const FftPlan& p) { return p.twy64; }
+template <> __host__ __device__ inline const float2* twy<float>(const FftPlan& p) { return p.twy32; }
+
+// ---------------------------------------------------------------------------
+// Block-cooperative Stockham transform of nseq sequences (length L.n, leading
+// dimension ldn) held in LDS.  tw: LDS copy of exp(-2 pi i t/n).  sign -1
+// forward, +1 inverse (unnormalised).
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int MAXB>
+__device__ __forceinline__ void lds_fft(C2<T>* buf, int ldn, int nseq, const FftLen& L,
+                                        const C2<T>* tw, int sign) {
+  const int N = L.n;
+  const T sg = (T)sign;
+  int ns = 1;
+  for (int s = 0; s < L.nstage; ++s) {
+    const int R = L.radix[s];
+    const int m = N / R;
+    const int total = nseq * m;
+    const int step = N / (ns * R);
+    C2<T> v[MAXB][4];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int q = threadIdx.x + b * NT;
+      if (q < total) {
+        const int sq = q / m;
+        const int j = q - sq * m;
+        const int k = j % ns;
+        const C2<T>* src = buf + sq * ldn;
+        C2<T> x0 = src[j];
+        C2<T> x1 = src[j + m];
+        C2<T> t1 = tw[k * step];
+        if (sign > 0) t1.y = -t1.y;  // tw holds exp(-...): conjugate for the inverse
+        x1 = cmul<T>(x1, t1);
+        if (R == 2) {
+          v[b][0] = cadd<T>(x0, x1);
+          v[b][1] = csub<T>(x0, x1);
+        } else if (R == 4) {
+          C2<T> x2 = src[j + 2 * m];
+          C2<T> x3 = src[j + 3 * m];
+          C2<T> t2 = tw[2 * k * step];
+          C2<T> t3 = tw[3 * k * step];
+          if (sign > 0) { t2.y = -t2.y; t3.y = -t3.y; }
+          x2 = cmul<T>(x2, t2);
+          x3 = cmul<T>(x3, t3);
+          const C2<T> a0 = cadd<T>(x0, x2), a1 = csub<T>(x0, x2);
+          const C2<T> a2 = cadd<T>(x1, x3), a3 = csub<T>(x1, x3);
+          // y1 = a1 + sign*i*a3 ; y3 = a1 - sign*i*a3
+          const C2<T> ia3 = mk<T>(-sg * a3.y, sg * a3.x);
+          v[b][0] = cadd<T>(a0, a2);
+          v[b][1] = cadd<T>(a1, ia3);
+          v[b][2] = csub<T>(a0, a2);
+          v[b][3] = csub<T>(a1, ia3);
+        } else {  // R == 3
+          C2<T> x2 = src[j + 2 * m];
+          C2<T> t2 = tw[2 * k * step];
+          if (sign > 0) t2.y = -t2.y;
+          x2 = cmul<T>(x2, t2);
+          const T h = (T)0.86602540378443864676372317075294;  // sqrt(3)/2
+          const C2<T> sm = cadd<T>(x1, x2), df = csub<T>(x1, x2);
+          const C2<T> c0 = mk<T>(x0.x - (T)0.5 * sm.x, x0.y - (T)0.5 * sm.y);
+          const C2<T> id = mk<T>(-sg * h * df.y, sg * h * df.x);
+          v[b][0] = cadd<T>(x0, sm);
+          v[b][1] = cadd<T>(c0, id);
+          v[b][2] = csub<T>(c0, id);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int q = threadIdx.x + b * NT;
+      if (q < total) {
+        const int sq = q / m;
+        const int j = q - sq * m;
+        const int k = j % ns;
+        C2<T>* dst = buf + sq * ldn + (j / ns) * ns * R + k;
+        dst[0] = v[b][0];
+        dst[ns] = v[b][1];
+        if (R >= 3) dst[2 * ns] = v[b][2];
+        if (R == 4) dst[3 * ns] = v[b][3];
+      }
+    }
+    __syncthreads();
+    ns *= R;
+  }
+}
+
+template <typename T, int NT>
+__device__ __forceinline__ void load_tw(C2<T>* dst, const C2<T>* src, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+}
+
+// Number of LDS complex slots per sequence (odd padding spreads banks).
+__host__ __device__ inline int lds_ld(int n) { return n + 1; }
+
+// ---------------------------------------------------------------------------
+// x pass, forward: rows (y0..y0+2*nseq) of plane pl -> T[pl][kx][y]
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int MAXB>
+__global__ __launch_bounds__(NT) void k_xpass_fwd(const T* __restrict__ phys, long long pstride,
+                                                  C2<T>* __restrict__ tmp, FftLen L, const C2<T>* twg,
+                                                  int ny, int mx, int nplanes, int nseq) {
+  extern __shared__ char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int N = L.n;
+  const int ldn = lds_ld(N);
+  C2<T>* tw = buf + nseq * ldn;
+  const int pl = blockIdx.y;                 // d*nplanes + zl
+  const int d = pl / nplanes, zl = pl - d * nplanes;
+  const int y0 = blockIdx.x * 2 * nseq;
+  const T* base = phys + (long long)d * pstride + (long long)zl * ny * N;
+  load_tw<T, NT>(tw, twg, N);
+  for (int idx = threadIdx.x; idx < nseq * N; idx += NT) {
+    const int s = idx / N, x = idx - s * N;
+    const int y = y0 + 2 * s;
+    const T a = (y < ny) ? base[(long long)y * N + x] : (T)0;
+    const T b = (y + 1 < ny) ? base[(long long)(y + 1) * N + x] : (T)0;
+    buf[s * ldn + x] = mk<T>(a, b);
+  }
+  __syncthreads();
+  lds_fft<T, NT, MAXB>(buf, ldn, nseq, L, tw, -1);
+  const int nr = 2 * nseq;
+  C2<T>* out = tmp + (long long)pl * mx * ny;
+  for (int idx = threadIdx.x; idx < mx * nr; idx += NT) {
+    const int k = idx / nr, r = idx - k * nr;
+    const int y = y0 + r;
+    if (y >= ny) continue;
+    const int s = r >> 1;
+    const C2<T> zk = buf[s * ldn + k];
+    const C2<T> zn = buf[s * ldn + (k == 0 ? 0 : N - k)];
+    C2<T> val;
+    if ((r & 1) == 0) val = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+    else val = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
+    out[(long long)k * ny + y] = val;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass, backward: T[pl][kx][y] (kx < mx) -> rows, halfcomplex c2r
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int MAXB>
+__global__ __launch_bounds__(NT) void k_xpass_bwd(const C2<T>* __restrict__ tmp, T* __restrict__ phys,
+                                                  long long pstride, FftLen L, const C2<T>* twg,
+                                                  int ny, int mx, int nplanes, int nseq) {
+  extern __shared__ char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int N = L.n;
+  const int ldn = lds_ld(N);
+  C2<T>* tw = buf + nseq * ldn;
+  const int pl = blockIdx.y;
+  const int d = pl / nplanes, zl = pl - d * nplanes;
+  const int y0 = blockIdx.x * 2 * nseq;
+  load_tw<T, NT>(tw, twg, N);
+  for (int idx = threadIdx.x; idx < nseq * N; idx += NT) {
+    const int s = idx / N, x = idx - s * N;
+    buf[s * ldn + x] = mk<T>((T)0, (T)0);
+  }
+  __syncthreads();
+  const C2<T>* in = tmp + (long long)pl * mx * ny;
+  for (int idx = threadIdx.x; idx < mx * nseq; idx += NT) {
+    const int k = idx / nseq, s = idx - k * nseq;
+    const int y = y0 + 2 * s;
+    C2<T> A = (y < ny) ? in[(long long)k * ny + y] : mk<T>((T)0, (T)0);
+    C2<T> B = (y + 1 < ny) ? in[(long long)k * ny + y + 1] : mk<T>((T)0, (T)0);
+    const bool selfconj = (k == 0) || (2 * k == N);
+    if (selfconj) { A.y = (T)0; B.y = (T)0; }
+    buf[s * ldn + k] = mk<T>(A.x - B.y, A.y + B.x);
+    if (!selfconj) buf[s * ldn + N - k] = mk<T>(A.x + B.y, B.x - A.y);
+  }
+  __syncthreads();
+  lds_fft<T, NT, MAXB>(buf, ldn, nseq, L, tw, +1);
+  T* base = phys + (long long)d * pstride + (long long)zl * ny * N;
+  for (int idx = threadIdx.x; idx < nseq * N; idx += NT) {
+    const int s = idx / N, x = idx - s * N;
+    const int y = y0 + 2 * s;
+    const C2<T> z = buf[s * ldn + x];
+    if (y < ny) base[(long long)y * N + x] = z.x;
+    if (y + 1 < ny) base[(long long)(y + 1) * N + x] = z.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y pass, forward: T[d][z][kx][y] -> spec (z fastest), keep nyc, scale
+// grid: (z tiles, mx, nv)
+// ---------------------------------------------------------------------------
+template <typename T, typename SpecT, int NT, int MAXB>
+__global__ __launch_bounds__(NT) void k_ypass_fwd(const C2<T>* __restrict__ tmp, SpecT* __restrict__ spec,
+                                                  long long sstride, FftLen L, const C2<T>* twg, int nz,
+                                                  int mx, int nyc, int nplanes, int zoff, int zt, T scale) {
+  extern __shared__ char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int N = L.n;
+  const int ldn = lds_ld(N);
+  C2<T>* tw = buf + zt * ldn;
+  const int z0 = blockIdx.x * zt;
+  const int k = blockIdx.y;
+  const int d = blockIdx.z;
+  const int nseq = min(zt, nplanes - z0);
+  load_tw<T, NT>(tw, twg, N);
+  for (int idx = threadIdx.x; idx < nseq * N; idx += NT) {
+    const int j = idx / N, y = idx - j * N;
+    buf[j * ldn + y] = tmp[(((long long)d * nplanes + z0 + j) * mx + k) * N + y];
+  }
+  __syncthreads();
+  lds_fft<T, NT, MAXB>(buf, ldn, nseq, L, tw, -1);
+  const int half = nyc / 2;
+  SpecT* out = spec + (long long)d * sstride + (long long)k * nyc * nz + zoff + z0;
+  for (int idx = threadIdx.x; idx < nyc * zt; idx += NT) {
+    const int iy = idx / zt, j = idx - iy * zt;
+    if (j >= nseq) continue;
+    const int yf = iy < half ? iy : iy + N - nyc;
+    const C2<T> v = buf[j * ldn + yf];
+    SpecT o;
+    o.x = v.x * scale;
+    o.y = v.y * scale;
+    out[(long long)iy * nz + j] = o;
+  }
+}
+
+// y pass, backward: spec (z fastest) -> T[d][z][kx][y], zero padded
+template <typename T, typename SpecT, int NT, int MAXB>
+__global__ __launch_bounds__(NT) void k_ypass_bwd(const SpecT* __restrict__ spec, long long sstride,
+                                                  C2<T>* __restrict__ tmp, FftLen L, const C2<T>* twg,
+                                                  int nz, int mx, int nyc, int nplanes, int zoff, int zt) {
+  extern __shared__ char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int N = L.n;
+  const int ldn = lds_ld(N);
+  C2<T>* tw = buf + zt * ldn;
+  const int z0 = blockIdx.x * zt;
+  const int k = blockIdx.y;
+  const int d = blockIdx.z;
+  const int nseq = min(zt, nplanes - z0);
+  load_tw<T, NT>(tw, twg, N);
+  for (int idx = threadIdx.x; idx < nseq * N; idx += NT) {
+    const int j = idx / N, y = idx - j * N;
+    buf[j * ldn + y] = mk<T>((T)0, (T)0);
+  }
+  __syncthreads();
+  const int half = nyc / 2;
+  const SpecT* in = spec + (long long)d * sstride + (long long)k * nyc * nz + zoff + z0;
+  for (int idx = threadIdx.x; idx < nyc * zt; idx += NT) {
+    const int iy = idx / zt, j = idx - iy * zt;
+    if (j >= nseq) continue;
+    const int yf = iy < half ? iy : iy + N - nyc;
+    const SpecT v = in[(long long)iy * nz + j];
+    buf[j * ldn + yf] = mk<T>((T)v.x, (T)v.y);
+  }
+  __syncthreads();
+  lds_fft<T, NT, MAXB>(buf, ldn, nseq, L, tw, +1);
+  for (int idx = threadIdx.x; idx < nseq * N; idx += NT) {
+    const int j = idx / N, y = idx - j * N;
+    tmp[(((long long)d * nplanes + z0 + j) * mx + k) * N + y] = buf[j * ldn + y];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch configuration
+// ---------------------------------------------------------------------------
+static constexpr int kNT = 256;
+static constexpr int kMaxB = 8;
+static constexpr size_t kLdsBudget = 64 * 1024;
+
+// sequences per block such that registers (MAXB butterflies of radix >= 2)
+// and the LDS budget hold them
+template <typename T>
+static int seq_per_block(int n) {
+  int byb = (2 * kMaxB * kNT) / n;              // nseq*n/2 <= MAXB*NT
+  const size_t per = sizeof(C2<T>) * (size_t)lds_ld(n);
+  int bylds = (int)((kLdsBudget - sizeof(C2<T>) * n) / per);
+  int s = std::min(byb, bylds);
+  if (s > 32) s = 32;
+  if (s < 1) s = 1;
+  return s;
+}
+
+template <typename T>
+static size_t lds_bytes(int n, int nseq) {
+  return sizeof(C2<T>) * ((size_t)nseq * lds_ld(n) + n);
+}
+
+template <typename T>
+void fft_xpass_forward(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
+                       hipStream_t st) {
+  const int n = p.d.nx;
+  const int nseq = std::max(1, std::min(seq_per_block<T>(n), (p.d.ny + 1) / 2));
+  dim3 grid(ceil_div(p.d.ny, 2 * nseq), nv * nplanes);
+  hipLaunchKernelGGL((k_xpass_fwd<T, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, nseq), st, phys,
+                     pstride, tmp, p.lx, twx<T>(p), p.d.ny, p.d.mx, nplanes, nseq);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T>
+void fft_xpass_backward(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
+                        hipStream_t st) {
+  const int n = p.d.nx;
+  const int nseq = std::max(1, std::min(seq_per_block<T>(n), (p.d.ny + 1) / 2));
+  dim3 grid(ceil_div(p.d.ny, 2 * nseq), nv * nplanes);
+  hipLaunchKernelGGL((k_xpass_bwd<T, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, nseq), st, tmp, phys,
+                     pstride, p.lx, twx<T>(p), p.d.ny, p.d.mx, nplanes, nseq);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T, typename SpecT>
+void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff,
+                       int nplanes, hipStream_t st, int scol) {
+  const int n = p.d.ny;
+  const int zt = std::max(1, std::min(seq_per_block<T>(n), nplanes));
+  dim3 grid(ceil_div(nplanes, zt), p.d.mx, nv);
+  const T scale = (T)(1.0 / ((double)p.d.nx * (double)p.d.ny));
+  hipLaunchKernelGGL((k_ypass_fwd<T, SpecT, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, zt), st, tmp, spec,
+                     sstride, p.ly, twy<T>(p), scol > 0 ? scol : p.d.nz, p.d.mx, p.d.nyc, nplanes, zoff, zt, scale);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T, typename SpecT>
+void fft_ypass_backward(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff,
+                        int nplanes, hipStream_t st, int scol) {
+  const int n = p.d.ny;
+  const int zt = std::max(1, std::min(seq_per_block<T>(n), nplanes));
+  dim3 grid(ceil_div(nplanes, zt), p.d.mx, nv);
+  hipLaunchKernelGGL((k_ypass_bwd<T, SpecT, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, zt), st, spec, sstride,
+                     tmp, p.ly, twy<T>(p), scol > 0 ? scol : p.d.nz, p.d.mx, p.d.nyc, nplanes, zoff, zt);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T, typename SpecT>
+void fft_forward(const FftPlan& p, const T* phys, long long pstride, SpecT* spec, long long sstride, int nv,
+                 int zoff, int nplanes, C2<T>* tmp, hipStream_t st, int scol) {
+  fft_xpass_forward<T>(p, phys, pstride, tmp, nv, nplanes, st);
+  fft_ypass_forward<T, SpecT>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol);
+}
+
+template <typename T, typename SpecT>
+void fft_backward(const FftPlan& p, const SpecT* spec, long long sstride, T* phys, long long pstride, int nv,
+                  int zoff, int nplanes, C2<T>* tmp, hipStream_t st, int scol) {
+  fft_ypass_backward<T, SpecT>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol);
+  fft_xpass_backward<T>(p, tmp, phys, pstride, nv, nplanes, st);
+}
+
+#define TBD_FFT_INST(T, S)                                                                              \
+  template void fft_forward<T, S>(const FftPlan&, const T*, long long, S*, long long, int, int, int,  \
+                                  C2<T>*, hipStream_t, int);                                          \
+  template void fft_backward<T, S>(const FftPlan&, const S*, long long, T*, long long, int, int, int, \
+                                   C2<T>*, hipStream_t, int);                                         \
+  template void fft_ypass_forward<T, S>(const FftPlan&, const C2<T>*, S*, long long, int, int, int,   \
+                                        hipStream_t, int);                                            \
+  template void fft_ypass_backward<T, S>(const FftPlan&, const S*, long long, C2<T>*, int, int, int,  \
+                                         hipStream_t, int);
+
+TBD_FFT_INST(double, double2)
+TBD_FFT_INST(float, float2)
+TBD_FFT_INST(float, double2)
+template void fft_xpass_forward<double>(const FftPlan&, const double*, long long, double2*, int, int, hipStream_t);
+template void fft_xpass_forward<float>(const FftPlan&, const float*, long long, float2*, int, int, hipStream_t);
+template void fft_xpass_backward<double>(const FftPlan&, const double2*, double*, long long, int, int, hipStream_t);
+template void fft_xpass_backward<float>(const FftPlan&, const float2*, float*, long long, int, int, hipStream_t);
+
+}  // namespace tbd

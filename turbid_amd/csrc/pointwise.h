@@ -1,0 +1,80 @@
+// Elementwise and reduction kernels of the substage.
+#pragma once
+
+#include "common.h"
+
+namespace tbd {
+
+// convective products adv_i = u.grad(u_i) (phase_carrier_flat.cc:792-803):
+// uvw[3] and g[9] (du/dx,du/dy,du/dz, dv/.., dw/..) physical, stride R
+template <typename T>
+void products_conv(const T* uvw, const T* g, T* adv, long long R, hipStream_t st);
+// divergence-form products (phase_carrier_flat.cc:725-744): the six distinct
+// u_i u_j in order uu, uv, uw, vv, vw, ww
+template <typename T>
+void products_div(const T* uvw, T* prod6, long long R, hipStream_t st);
+// scalar settling-flux products with the advecting velocity u + shift
+// (phase_scalar_flat.cc:256-279, 566-571, 587-590)
+// conv: adv = (U dc0 + V dc1) + W dc2 ; div: out_i = c * U_i
+template <typename T>
+void scalar_products_conv(const T* uvw, T su, T sv, T sw, const T* dc, T* adv, long long R, hipStream_t st);
+template <typename T, typename TC>
+void scalar_products_div(const T* uvw, T su, T sv, T sw, const TC* c, T* out3, long long R, hipStream_t st);
+
+// H = c1 H - dt adv, + buoyancy (K c sin, -K c cos) + dt*force on the mean
+// column (phase_carrier_flat.cc:294-504).  nf fields of stride S.
+struct HUpd {
+  double c1 = 0, dt = 0;
+  int buoy = 0;          // 1 if the scalar drives the carrier
+  double K = 0;          // dt*Cbar/Fr^2*(sg-1)
+  double sn = 0, cs = 0; int slope_dir = 0;
+  double f1 = 0, f2 = 0; // mean-mode force (u, v)
+  int nz = 0;
+};
+template <typename TA>
+void h_update(double2* h, const TA* adv, const double2* c, const HUpd& u, int nf, long long S, hipStream_t st);
+
+// Carrier boundary rows (boundary_carrier_flat.cc:185-246) and the mean-mode
+// top values (phase_carrier_flat.cc:542-547) written into rhs[3].
+struct BcArgs {
+  int nz = 0;
+  long long P = 0, S = 0;
+  double k = 0, s = 0;
+  const double2* m1b = nullptr; const double2* m1t = nullptr;
+  const double2* m2b = nullptr; const double2* m2t = nullptr;
+  const double2* n1t = nullptr; const double2* n2t = nullptr;   // open-top tangential rows
+  int open_top = 0;
+  double top0 = 0, top1 = 0, top2 = 0;
+};
+void bc_carrier(double2* rhs, const BcArgs& a, hipStream_t st);
+
+// Scalar rows: zero flux rows, then the erosion/deposition bottom row
+// (boundary_scalar_flat.cc:151-152, phase_scalar_flat.cc:349-374).
+void bc_scalar(double2* rhs, int nz, long long P, const double2* ehat, const double2* c, double ws, double cbar,
+               double re, double sc, int erosion, hipStream_t st);
+// erosion map on one plane: val = sqrt(a^2+b^2)/Re; val < tc -> 0 else M(val/tc - 1)
+template <typename T>
+void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re, double tc, double M,
+                 hipStream_t st);
+
+// Bartnicki positivity filter, 5 iterations per z-plane (phase_scalar_flat.cc:427-475)
+void bartnicki_filter(double* c, int nx, int ny, int nz, hipStream_t st);
+
+// Diagnostics (phase_carrier_flat.cc:639-671): feat[0..8] max |u|,|v|,|w| over
+// all / bottom / top planes, feat[19] max(|u|/dx + |v|/dy + |w|/dz(z)).
+// feat must be zeroed (non-negative max via integer atomics).
+template <typename T>
+void diagnostics(const T* uvw, long long R, int nx, int ny, int nz, double glx, double gly, const double* glz,
+                 double* feat, hipStream_t st);
+
+// feat[21] = trapz(gridz, Re c(:,0)) / 2 (phase_scalar_flat.cc:485-486)
+void trapz_mean(const double2* c, const double* gridz, int nz, double* out, hipStream_t st);
+
+// small copies: dst[i] = Re(src[idx[i]]) for the log features
+void gather_real(const double2* src, const long long* idx, int n, double* dst, hipStream_t st);
+void fill_zero(void* p, size_t bytes, hipStream_t st);
+// *out = (a * Re src[0]) * b
+void scaled_real(const double2* src, double a, double b, double* out, hipStream_t st);
+void copy_d2(double2* dst, const double2* src, long long n, hipStream_t st);
+
+}  // namespace tbd

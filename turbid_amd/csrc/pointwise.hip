@@ -1,0 +1,340 @@
+// Elementwise and reduction kernels of the substage (see pointwise.h).
+#include "pointwise.h"
+
+namespace tbd {
+
+static constexpr int kNT = 256;
+
+static inline dim3 grid1d(long long n, int per = 1) {
+  long long b = (n + (long long)kNT * per - 1) / ((long long)kNT * per);
+  if (b > 65536LL * 8) b = 65536LL * 8;
+  return dim3((unsigned)b);
+}
+
+#define GRID_STRIDE(i, n) \
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+template <typename T>
+__global__ void k_products_conv(const T* __restrict__ U, const T* __restrict__ g, T* __restrict__ adv, long long R) {
+  GRID_STRIDE(e, R) {
+    const T a = U[e], b = U[R + e], w = U[2 * R + e];
+    adv[e] = a * g[e] + b * g[R + e] + w * g[2 * R + e];
+    adv[R + e] = a * g[3 * R + e] + b * g[4 * R + e] + w * g[5 * R + e];
+    adv[2 * R + e] = a * g[6 * R + e] + b * g[7 * R + e] + w * g[8 * R + e];
+  }
+}
+template <typename T>
+void products_conv(const T* uvw, const T* g, T* adv, long long R, hipStream_t st) {
+  hipLaunchKernelGGL(k_products_conv<T>, grid1d(R), dim3(kNT), 0, st, uvw, g, adv, R);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_products_div(const T* __restrict__ U, T* __restrict__ o, long long R) {
+  GRID_STRIDE(e, R) {
+    const T a = U[e], b = U[R + e], w = U[2 * R + e];
+    o[e] = a * a;
+    o[R + e] = a * b;
+    o[2 * R + e] = a * w;
+    o[3 * R + e] = b * b;
+    o[4 * R + e] = b * w;
+    o[5 * R + e] = w * w;
+  }
+}
+template <typename T>
+void products_div(const T* uvw, T* prod6, long long R, hipStream_t st) {
+  hipLaunchKernelGGL(k_products_div<T>, grid1d(R), dim3(kNT), 0, st, uvw, prod6, R);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_scalar_conv(const T* __restrict__ U, T su, T sv, T sw, const T* __restrict__ dc, T* __restrict__ adv,
+                              long long R) {
+  GRID_STRIDE(e, R) {
+    adv[e] = (U[e] + su) * dc[e] + (U[R + e] + sv) * dc[R + e] + (U[2 * R + e] + sw) * dc[2 * R + e];
+  }
+}
+template <typename T>
+void scalar_products_conv(const T* uvw, T su, T sv, T sw, const T* dc, T* adv, long long R, hipStream_t st) {
+  hipLaunchKernelGGL(k_scalar_conv<T>, grid1d(R), dim3(kNT), 0, st, uvw, su, sv, sw, dc, adv, R);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T, typename TC>
+__global__ void k_scalar_div(const T* __restrict__ U, T su, T sv, T sw, const TC* __restrict__ c, T* __restrict__ o,
+                             long long R) {
+  GRID_STRIDE(e, R) {
+    const T cv = (T)c[e];
+    o[e] = cv * (U[e] + su);
+    o[R + e] = cv * (U[R + e] + sv);
+    o[2 * R + e] = cv * (U[2 * R + e] + sw);
+  }
+}
+template <typename T, typename TC>
+void scalar_products_div(const T* uvw, T su, T sv, T sw, const TC* c, T* out3, long long R, hipStream_t st) {
+  hipLaunchKernelGGL((k_scalar_div<T, TC>), grid1d(R), dim3(kNT), 0, st, uvw, su, sv, sw, c, out3, R);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename TA>
+__global__ void k_h_update(double2* __restrict__ h, const TA* __restrict__ adv, const double2* __restrict__ c,
+                           HUpd u, int nf, long long S) {
+  GRID_STRIDE(i, nf * S) {
+    const int f = (int)(i / S);
+    const long long e = i - (long long)f * S;
+    double2 hv = h[i];
+    const TA a = adv[i];
+    hv.x = u.c1 * hv.x - u.dt * (double)a.x;
+    hv.y = u.c1 * hv.y - u.dt * (double)a.y;
+    if (u.buoy) {
+      const double2 cv = c[e];
+      if (f == 2) {
+        hv.x = hv.x - u.K * cv.x * u.cs;
+        hv.y = hv.y - u.K * cv.y * u.cs;
+      } else if ((f == 0 && u.slope_dir == 1) || (f == 1 && u.slope_dir == 2)) {
+        hv.x = hv.x + u.K * cv.x * u.sn;
+        hv.y = hv.y + u.K * cv.y * u.sn;
+      }
+    }
+    if (e < u.nz) {                 // mean column (kx = ky = 0), all z
+      if (f == 0) hv.x += u.dt * u.f1;
+      else if (f == 1) hv.x += u.dt * u.f2;
+    }
+    h[i] = hv;
+  }
+}
+template <typename TA>
+void h_update(double2* h, const TA* adv, const double2* c, const HUpd& u, int nf, long long S, hipStream_t st) {
+  hipLaunchKernelGGL(k_h_update<TA>, grid1d(nf * S), dim3(kNT), 0, st, h, adv, c, u, nf, S);
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_bc_carrier(double2* __restrict__ rhs, BcArgs a) {
+  GRID_STRIDE(j, 3 * a.P) {
+    const int f = (int)(j / a.P);
+    const long long col = j - (long long)f * a.P;
+    const double s = a.s, k = a.k;
+    const double2 m1t = a.m1t[j], m2t = a.m2t[j], m1b = a.m1b[j], m2b = a.m2b[j];
+    double2 top = make_double2(k * ((1 + s) * m1t.x + (0 - s) * m2t.x), k * ((1 + s) * m1t.y + (0 - s) * m2t.y));
+    const double2 bot = make_double2(k * ((1 + s) * m1b.x + (0 - s) * m2b.x), k * ((1 + s) * m1b.y + (0 - s) * m2b.y));
+    if (a.open_top && f < 2) {
+      const double2 n1 = a.n1t[j], n2 = a.n2t[j];
+      top = make_double2(k * ((1 + s) * n1.x + (0 - s) * n2.x), k * ((1 + s) * n1.y + (0 - s) * n2.y));
+    }
+    if (col == 0) top = make_double2(f == 0 ? a.top0 : (f == 1 ? a.top1 : a.top2), 0.0);
+    double2* r = rhs + (long long)f * a.S + col * a.nz;
+    r[0] = bot;
+    r[a.nz - 1] = top;
+  }
+}
+void bc_carrier(double2* rhs, const BcArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_bc_carrier, grid1d(3 * a.P), dim3(kNT), 0, st, rhs, a);
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_bc_scalar(double2* __restrict__ rhs, int nz, long long P, const double2* __restrict__ ehat,
+                            const double2* __restrict__ c, double ws, double cbar, double re, double sc, int erosion) {
+  GRID_STRIDE(col, P) {
+    double2* r = rhs + col * nz;
+    double2 b = make_double2(0.0, 0.0);
+    if (erosion) {
+      const double2 e = ehat[col], cv = c[col * nz];
+      double2 v = make_double2(e.x - ws * cv.x * cbar, e.y - ws * cv.y * cbar);
+      const double scale = 10E+6;   // BOUNDARY_COEF_SCALE (common.h:55)
+      v = make_double2(-1.0 * v.x * re * sc / cbar * scale, -1.0 * v.y * re * sc / cbar * scale);
+      b = v;
+    }
+    r[0] = b;
+    r[nz - 1] = make_double2(0.0, 0.0);
+  }
+}
+void bc_scalar(double2* rhs, int nz, long long P, const double2* ehat, const double2* c, double ws, double cbar,
+               double re, double sc, int erosion, hipStream_t st) {
+  hipLaunchKernelGGL(k_bc_scalar, grid1d(P), dim3(kNT), 0, st, rhs, nz, P, ehat, c, ws, cbar, re, sc, erosion);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_erosion(const T* __restrict__ a, const T* __restrict__ b, double* __restrict__ e, long long n,
+                          double re, double tc, double M) {
+  GRID_STRIDE(i, n) {
+    const double x = (double)a[i], y = (double)b[i];
+    double val = sqrt(x * x + y * y) / re;
+    if (val < tc) val = 0;
+    else val = M * (val / tc - 1);
+    e[i] = val;
+  }
+}
+template <typename T>
+void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re, double tc, double M,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_erosion<T>, grid1d(n), dim3(kNT), 0, st, dudz, dvdz, e, n, re, tc, M);
+  TBD_HIP(hipGetLastError());
+}
+
+// One workgroup per z-plane; the plane statistics are FP64 sums.  A plane
+// with no negative entries is left untouched after the first reduction (each
+// further pass would add 0 to the positive entries and zero the zeros).
+__global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long long plane) {
+  __shared__ double red[4][16];
+  double* pl = c + (long long)blockIdx.x * plane;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int it = 0; it < 5; ++it) {
+    double pos = 0, npos = 0, neg = 0, nneg = 0;
+    for (long long e = threadIdx.x; e < plane; e += blockDim.x) {
+      const double v = pl[e];
+      if (v > 0) { pos += v; npos += 1; }
+      else if (v < 0) { neg += v; nneg += 1; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      pos += __shfl_xor(pos, o);
+      npos += __shfl_xor(npos, o);
+      neg += __shfl_xor(neg, o);
+      nneg += __shfl_xor(nneg, o);
+    }
+    if (lane == 0) { red[0][wave] = pos; red[1][wave] = npos; red[2][wave] = neg; red[3][wave] = nneg; }
+    __syncthreads();
+    double P = 0, NP = 0, N = 0, NN = 0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; ++w) { P += red[0][w]; NP += red[1][w]; N += red[2][w]; NN += red[3][w]; }
+    __syncthreads();
+    if (NN == 0) break;                 // uniform across the block
+    if (P + 0.0 + N < 0) {
+      for (long long e = threadIdx.x; e < plane; e += blockDim.x) pl[e] = 0;
+    } else {
+      const double add = N / NP;
+      for (long long e = threadIdx.x; e < plane; e += blockDim.x) {
+        const double v = pl[e];
+        pl[e] = v > 0 ? v + add : 0.0;
+      }
+    }
+    __syncthreads();
+  }
+}
+void bartnicki_filter(double* c, int nx, int ny, int nz, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter, dim3(nz), dim3(1024), 0, st, c, (long long)nx * ny);
+  TBD_HIP(hipGetLastError());
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__global__ void k_diag(const T* __restrict__ U, long long R, int nx, int ny, int nz, double glx, double gly,
+                       const double* __restrict__ glz, double* feat) {
+  const long long plane = (long long)nx * ny;
+  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  GRID_STRIDE(e, R) {
+    const double a = fabs((double)U[e]), b = fabs((double)U[R + e]), w = fabs((double)U[2 * R + e]);
+    const int z = (int)(e / plane);
+    m[0] = fmax(m[0], a);
+    m[1] = fmax(m[1], b);
+    m[2] = fmax(m[2], w);
+    if (z == 0) { m[3] = fmax(m[3], a); m[4] = fmax(m[4], b); m[5] = fmax(m[5], w); }
+    if (z == nz - 1) { m[6] = fmax(m[6], a); m[7] = fmax(m[7], b); m[8] = fmax(m[8], w); }
+    m[9] = fmax(m[9], a / glx + b / gly + w / glz[z]);
+  }
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    double v = m[i];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    m[i] = v;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    for (int i = 0; i < 9; ++i) atomic_max_nonneg(feat + i, m[i]);
+    atomic_max_nonneg(feat + 19, m[9]);
+  }
+}
+template <typename T>
+void diagnostics(const T* uvw, long long R, int nx, int ny, int nz, double glx, double gly, const double* glz,
+                 double* feat, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag<T>, grid1d(R, 8), dim3(kNT), 0, st, uvw, R, nx, ny, nz, glx, gly, glz, feat);
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_trapz(const double2* __restrict__ c, const double* __restrict__ z, int nz, double* out) {
+  __shared__ double part[64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i + 1 < nz; i += blockDim.x) s += (z[i + 1] - z[i]) * (c[i].x + c[i + 1].x) / 2;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+    *out = t / 2;
+  }
+}
+void trapz_mean(const double2* c, const double* gridz, int nz, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_trapz, dim3(1), dim3(256), 0, st, c, gridz, nz, out);
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_gather_real(const double2* __restrict__ src, const long long* __restrict__ idx, int n,
+                              double* __restrict__ dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]].x;
+}
+void gather_real(const double2* src, const long long* idx, int n, double* dst, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_real, dim3(1), dim3(64), 0, st, src, idx, n, dst);
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_zero(unsigned long long* p, long long n) {
+  GRID_STRIDE(i, n) p[i] = 0ULL;
+}
+__global__ void k_zero_bytes(unsigned char* p, long long n) {
+  GRID_STRIDE(i, n) p[i] = 0;
+}
+void fill_zero(void* p, size_t bytes, hipStream_t st) {
+  if (bytes % 8 == 0) {
+    hipLaunchKernelGGL(k_zero, grid1d(bytes / 8), dim3(kNT), 0, st, (unsigned long long*)p, (long long)(bytes / 8));
+  } else {
+    hipLaunchKernelGGL(k_zero_bytes, grid1d(bytes), dim3(kNT), 0, st, (unsigned char*)p, (long long)bytes);
+  }
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_scaled_real(const double2* __restrict__ src, double a, double b, double* out) {
+  if (threadIdx.x == 0) *out = (a * src[0].x) * b;
+}
+void scaled_real(const double2* src, double a, double b, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_scaled_real, dim3(1), dim3(64), 0, st, src, a, b, out);
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_copy_d2(double2* __restrict__ d, const double2* __restrict__ s, long long n) {
+  GRID_STRIDE(i, n) d[i] = s[i];
+}
+void copy_d2(double2* dst, const double2* src, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_d2, grid1d(n), dim3(kNT), 0, st, dst, src, n);
+  TBD_HIP(hipGetLastError());
+}
+
+template void products_conv<double>(const double*, const double*, double*, long long, hipStream_t);
+template void products_conv<float>(const float*, const float*, float*, long long, hipStream_t);
+template void products_div<double>(const double*, double*, long long, hipStream_t);
+template void products_div<float>(const float*, float*, long long, hipStream_t);
+template void scalar_products_conv<double>(const double*, double, double, double, const double*, double*, long long,
+                                           hipStream_t);
+template void scalar_products_conv<float>(const float*, float, float, float, const float*, float*, long long,
+                                          hipStream_t);
+template void scalar_products_div<double, double>(const double*, double, double, double, const double*, double*,
+                                                  long long, hipStream_t);
+template void scalar_products_div<float, float>(const float*, float, float, float, const float*, float*, long long,
+                                                hipStream_t);
+template void scalar_products_div<float, double>(const float*, float, float, float, const double*, float*, long long,
+                                                 hipStream_t);
+template void h_update<double2>(double2*, const double2*, const double2*, const HUpd&, int, long long, hipStream_t);
+template void h_update<float2>(double2*, const float2*, const double2*, const HUpd&, int, long long, hipStream_t);
+template void erosion_map<double>(const double*, const double*, double*, long long, double, double, double,
+                                  hipStream_t);
+template void erosion_map<float>(const float*, const float*, double*, long long, double, double, double, hipStream_t);
+template void diagnostics<double>(const double*, long long, int, int, int, double, double, const double*, double*,
+                                  hipStream_t);
+template void diagnostics<float>(const float*, long long, int, int, int, double, double, const double*, double*,
+                                 hipStream_t);
+
+}  // namespace tbd

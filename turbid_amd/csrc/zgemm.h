@@ -1,0 +1,85 @@
+// Wall-normal contractions Y = M X over every spectral column (reference
+// RealCplxCube::Multiply, src/cube_real_cplx.cc:461-513) on MFMA, with the
+// pointwise work around them fused into the epilogue.  M is a real nz*nz
+// operator (D1, D2, V~, V~^-1) stored row-major with a padded leading
+// dimension; X, Y are complex spectral fields (z fastest, one column per
+// (kx,ky)).
+#pragma once
+
+#include "common.h"
+
+namespace tbd {
+
+enum ZMode {
+  Z_PLAIN = 0,    // y_f = M x_f
+  Z_GRAD = 1,     // y_{3f+0,1,2} = (i kx x_f, i ky x_f, M x_f)          (SpaceStepper::Gradient)
+  Z_RHS = 2,      // y_f = (s2 h_f + s1 x_f) - ((M x_f - kx^2 x_f) - ky^2 x_f)   (Laplacian + rhs)
+  Z_DIV = 3,      // y = q ((M x + i kx e0) + i ky e1)                     (Divergence, pressure rhs)
+  Z_CORRECT = 4,  // y_f = e_f - cf g_f, g = (i kx x, i ky x, M x); mean w := 0; wall rows saved
+  Z_GRADTMP = 5,  // y_f = (i kx e0_f + i ky e1_f) + M x_f                  (GradientTmp summed)
+};
+
+struct ZArgs {
+  int mode = Z_PLAIN;
+  int nz = 0;            // rows of M and of each column
+  int lda = 0;           // leading dimension of the row-major operator
+  int P = 0;             // columns per field
+  int nyc = 0;           // spectral y size (column -> (ix, iy))
+  const void* op = nullptr;       // operator (T*), shared by all fields of the launch
+  const void* x = nullptr;  long long xs = 0;   // field f at x + f*xs (complex elements)
+  void* y = nullptr;        long long ys = 0;
+  const void* e0 = nullptr; const void* e1 = nullptr; const void* e2 = nullptr;
+  long long es = 0;                             // stride between fields of e0/e1
+  const void* h = nullptr;  long long hs = 0;
+  const double* kx = nullptr;    // per spectral x index
+  const double* ky = nullptr;    // per spectral y index
+  double s1 = 0, s2 = 0, q = 0, cf = 0;
+  // Z_CORRECT wall rows: [3P] bottom/top of (i kx p, i ky p, dp/dz), and
+  // [2P] bottom/top of (i kx dp/dz, i ky dp/dz) when open_top
+  double2* wall_bot = nullptr; double2* wall_top = nullptr;
+  double2* wall2_bot = nullptr; double2* wall2_top = nullptr;
+  int open_top = 0;
+};
+
+// T: arithmetic/operator type; TX: input complex type; TY: output complex type.
+// nfield fields are processed in one launch (blockIdx.y).
+template <typename T, typename TX, typename TY>
+void zgemm(const ZArgs& a, int nfield, hipStream_t st);
+
+// Fused Helmholtz solve (reference FlatCarrierPhase::Helmholtz,
+// phase_carrier_flat.cc:861-994, FlatScalarPhase::Helmholtz,
+// phase_scalar_flat.cc:635-667) for up to three fields with their own
+// factorisation.  FP64.
+struct HField {
+  const double* vinv = nullptr;   // row-major, lda
+  const double* vnorm = nullptr;
+  const double* lambda = nullptr; // nz
+  const double* fix = nullptr;    // 4*nz
+  double sigma = 0;               // stage coefficient (0 for pressure)
+  int nullrow = -1;               // gated row on the mean column (sigma == 0 only)
+  const double2* x = nullptr;     // right-hand side
+  double2* y = nullptr;           // solution (may alias x)
+};
+
+struct HArgs {
+  int nz = 0, lda = 0, P = 0, nyc = 0;
+  int mode = 0;                   // 0: forward fix; 1: boundary rows zeroed (pressure)
+  const double* kx = nullptr;
+  const double* ky = nullptr;
+  HField f[3];
+};
+
+void helmholtz(const HArgs& a, int nfield, hipStream_t st);
+
+// Rows 0 and nz-1 of M x for every column (thin contraction: wall values of
+// dU/dz, dV/dz used by the erosion map and the log features).
+void zrows_wall(const double* op_rowmajor, int lda, int nz, const double2* x, long long P,
+                double2* bottom, double2* top, hipStream_t st);
+
+// out[2f] / out[2f+1] = Re (M x_f)(row 0 / row nz-1) on the mean column of
+// field f = 0..2 (fields stride S): the wall mean-mode gradients of the log
+// (phase_carrier_flat.cc:686-691).
+void wall_mean_rows(const double* op_rowmajor, int lda, int nz, const double2* x, long long S, double* out,
+                    hipStream_t st);
+
+}  // namespace tbd

@@ -1,0 +1,290 @@
+"""ctypes binding of libturbid_amd.so (the C ABI in include/turbid_amd.h).
+
+The library is the only compute path: if it is missing, or no GPU is
+visible, every entry point raises.  Nothing here falls back to a CPU
+implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libturbid_amd.so")
+
+CONFIG_FIELDS = [
+    ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+    ("nphase", ctypes.c_int),
+    ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+    ("slope", ctypes.c_double), ("slope_direction", ctypes.c_int),
+    ("mapping", ctypes.c_int), ("mapping_coefficient", ctypes.c_double),
+    ("dt", ctypes.c_double), ("output_on_step", ctypes.c_int),
+    ("force_type", ctypes.c_int), ("force_direction", ctypes.c_int * 2),
+    ("force_amplitude", ctypes.c_double * 4),
+    ("reynolds", ctypes.c_double), ("froude", ctypes.c_double),
+    ("averaged_concentration", ctypes.c_double), ("specific_gravity", ctypes.c_double),
+    ("affected_by_scalar_phase", ctypes.c_int),
+    ("vt_u1", ctypes.c_double * 5), ("vt_u2", ctypes.c_double * 5),
+    ("vt_u3", ctypes.c_double * 5), ("vt_p", ctypes.c_double * 5),
+    ("schmidt", ctypes.c_double), ("settling_velocity", ctypes.c_double),
+    ("erosion_rate", ctypes.c_double), ("critical_shear_stress", ctypes.c_double),
+    ("scalar_bc", ctypes.c_int), ("precision", ctypes.c_int),
+]
+
+# symbols declared in include/turbid_amd.h
+EXPORTS = [
+    "tbd_create", "tbd_destroy", "tbd_last_error", "tbd_upload_state", "tbd_download_state",
+    "tbd_download_physical_velocity", "tbd_substage", "tbd_step", "tbd_set_dt", "tbd_features",
+    "tbd_sync", "tbd_profile_enable", "tbd_profile_read", "tbd_profile_reset", "tbd_get_operators",
+    "tbd_get_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
+]
+
+KCLASS = {"fft": 0, "zgemm": 1, "helmholtz": 2, "other": 3}
+
+
+class TbdConfig(ctypes.Structure):
+    _fields_ = CONFIG_FIELDS
+
+
+def make_config(cfg):
+    s = TbdConfig()
+    for name, ctype in TbdConfig._fields_:
+        if name not in cfg:
+            continue
+        v = cfg[name]
+        if hasattr(ctype, "_length_"):
+            arr = getattr(s, name)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(s, name, v)
+    return s
+
+
+class TurbidError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TurbidError("libturbid_amd.so not built: run `python -m turbid_amd.build` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.tbd_create.argtypes = [ctypes.POINTER(TbdConfig), ctypes.POINTER(vp)]
+        L.tbd_destroy.argtypes = [vp]
+        L.tbd_destroy.restype = None
+        L.tbd_last_error.restype = ctypes.c_char_p
+        L.tbd_upload_state.argtypes = [vp, dp, dp, dp]
+        L.tbd_download_state.argtypes = [vp, dp, dp, dp]
+        L.tbd_download_physical_velocity.argtypes = [vp, dp]
+        L.tbd_substage.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.tbd_step.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+        L.tbd_set_dt.argtypes = [vp, ctypes.c_double]
+        L.tbd_features.argtypes = [vp, dp]
+        L.tbd_sync.argtypes = [vp]
+        L.tbd_profile_enable.argtypes = [vp, ctypes.c_int]
+        L.tbd_profile_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.tbd_profile_reset.argtypes = [vp]
+        L.tbd_get_operators.argtypes = [vp, dp, dp, dp]
+        L.tbd_get_factorisation.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
+        L.tbd_dft_forward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
+        L.tbd_dft_backward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
+        L.tbd_zcontract.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp, dp, dp]
+        L.tbd_helmholtz.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc):
+    if rc != 0:
+        raise TurbidError(lib().tbd_last_error().decode())
+
+
+def dims(nx, ny, nz):
+    nxc, nyc = nx // 3 * 2, ny // 3 * 2
+    return nxc, nyc, nxc // 2 + 1
+
+
+def dft_forward(phys, fp32=False):
+    """phys (nv, nz, ny, nx) -> spectral (nv, mx, nyc, nz) complex128."""
+    nv, nz, ny, nx = phys.shape
+    _, nyc, mx = dims(nx, ny, nz)
+    src = np.ascontiguousarray(phys, dtype=np.float64)
+    out = np.zeros((nv, mx, nyc, nz), dtype=np.complex128)
+    _check(lib().tbd_dft_forward(nx, ny, nz, nv, int(fp32), _p(src), _p(out.view(np.float64))))
+    return out
+
+
+def dft_backward(spec, nx, ny, fp32=False):
+    nv, mx, nyc, nz = spec.shape
+    src = np.ascontiguousarray(spec, dtype=np.complex128)
+    out = np.zeros((nv, nz, ny, nx))
+    _check(lib().tbd_dft_backward(nx, ny, nz, nv, int(fp32), _p(src.view(np.float64)), _p(out)))
+    return out
+
+
+def zcontract(m, x, fp32=False):
+    """y[:, c] = m @ x[:, c]; m (nz, nz) real, x (ncol, nz) complex (column-contiguous)."""
+    nz = m.shape[0]
+    mm = np.asfortranarray(m, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    ncol = xx.shape[0]
+    y = np.zeros_like(xx)
+    _check(lib().tbd_zcontract(nz, ncol, int(fp32), _p(mm), _p(xx.view(np.float64)), _p(y.view(np.float64))))
+    return y
+
+
+class Solver:
+    """One DNS context on the GPU (carrier + optional sediment phase)."""
+
+    def __init__(self, cfg):
+        self.cfg = dict(cfg)
+        s = make_config(cfg)
+        h = ctypes.c_void_p()
+        _check(lib().tbd_create(ctypes.byref(s), ctypes.byref(h)))
+        self._h = h
+        self.nx, self.ny, self.nz = cfg["nx"], cfg["ny"], cfg["nz"]
+        self.nxc, self.nyc, self.mx = dims(self.nx, self.ny, self.nz)
+        self.P = self.nyc * self.mx
+        self.S = self.nz * self.P
+        self.nphase = cfg.get("nphase", 1)
+
+    def close(self):
+        if self._h:
+            lib().tbd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, uvw, p, c=None):
+        u = np.ascontiguousarray(uvw, dtype=np.complex128).reshape(-1)
+        pp = np.ascontiguousarray(p, dtype=np.complex128).reshape(-1)
+        cc = None if c is None else np.ascontiguousarray(c, dtype=np.complex128).reshape(-1)
+        _check(lib().tbd_upload_state(self._h, _p(u.view(np.float64)), _p(pp.view(np.float64)),
+                                      None if cc is None else _p(cc.view(np.float64))))
+
+    def download(self):
+        u = np.zeros(3 * self.S, dtype=np.complex128)
+        p = np.zeros(self.S, dtype=np.complex128)
+        c = np.zeros(self.S, dtype=np.complex128) if self.nphase > 1 else None
+        _check(lib().tbd_download_state(self._h, _p(u.view(np.float64)), _p(p.view(np.float64)),
+                                        None if c is None else _p(c.view(np.float64))))
+        return u, p, c
+
+    def physical_velocity(self):
+        out = np.zeros(3 * self.nx * self.ny * self.nz)
+        _check(lib().tbd_download_physical_velocity(self._h, _p(out)))
+        return out.reshape(3, self.nz, self.ny, self.nx)
+
+    def substage(self, stage, time, alternation):
+        _check(lib().tbd_substage(self._h, stage, time, int(alternation)))
+
+    def step(self, time, alternation):
+        _check(lib().tbd_step(self._h, time, int(alternation)))
+
+    def set_dt(self, dt):
+        _check(lib().tbd_set_dt(self._h, dt))
+
+    def features(self):
+        f = np.zeros(50)
+        _check(lib().tbd_features(self._h, _p(f)))
+        return f
+
+    def sync(self):
+        _check(lib().tbd_sync(self._h))
+
+    def profile(self, enable=True):
+        _check(lib().tbd_profile_enable(self._h, int(enable)))
+
+    def profile_reset(self):
+        _check(lib().tbd_profile_reset(self._h))
+
+    def profile_read(self, kclass):
+        n = ctypes.c_longlong()
+        ms = ctypes.c_double()
+        _check(lib().tbd_profile_read(self._h, KCLASS.get(kclass, kclass), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def operators(self):
+        n = self.nz
+        g = np.zeros(n)
+        d1 = np.zeros((n, n), order="F")
+        d2 = np.zeros((n, n), order="F")
+        _check(lib().tbd_get_operators(self._h, _p(g), _p(d1), _p(d2)))
+        return g, d1, d2
+
+    def factorisation(self, which):
+        n = self.nz
+        V = np.zeros((n, n), order="F")
+        Vi = np.zeros((n, n), order="F")
+        lam = np.zeros(n)
+        fix = np.zeros(4 * n)
+        _check(lib().tbd_get_factorisation(self._h, which, _p(V), _p(Vi), _p(lam), _p(fix)))
+        return V, Vi, lam, fix.reshape(4, n)
+
+    def helmholtz(self, which, stage, mode, x):
+        xx = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        assert xx.size == self.S
+        y = np.zeros_like(xx)
+        _check(lib().tbd_helmholtz(self._h, which, stage, mode, _p(xx.view(np.float64)), _p(y.view(np.float64))))
+        return y
+
+
+class Stepper:
+    """TimeStepper mirror in step mode (reference src/stepper_time.cc:72-113,
+    215-246): step counter, time += dt, CFL guard on the previous step's
+    feature(19), alternation flag toggled after every step
+    (src/stepper_time.h:75-84)."""
+
+    def __init__(self, dt, max_step=10**9, max_cfl=1e30, start_step=0):
+        self.dt = dt
+        self.step = max(0, start_step)
+        self.time = self.step * dt
+        self.max_step = max_step
+        self.max_cfl = max_cfl
+        self.forward = self.step <= max_step
+        self.alternation = False
+        self.cfl = 0.0
+
+    def update(self, maxulen):
+        if not self.forward:
+            return
+        self.step += 1
+        self.time = self.time + self.dt
+        self.forward = self.step < self.max_step
+        self.cfl = maxulen * self.dt
+        if self.cfl > self.max_cfl:
+            self.forward = False
+
+    def toggle(self):
+        self.alternation = not self.alternation
+
+
+def run_steps(solver, stepper, nsteps):
+    """FlatChannelSolver::Execute's loop body, nsteps times (src/solver_channel_flat.cc:199-254)."""
+    feat = np.zeros(50)
+    for _ in range(nsteps):
+        if not stepper.forward:
+            break
+        stepper.update(feat[19])
+        solver.step(stepper.time, stepper.alternation)
+        stepper.toggle()
+        feat = solver.features()
+    return feat
