@@ -92,9 +92,13 @@ int tbd_features(tbd_ctx* ctx, double* out50);
 int tbd_sync(tbd_ctx* ctx);
 
 /* Kernel timing with HIP events on the context's stream.
- * class: 0 plane transforms, 1 z-contractions, 2 Helmholtz solves, 3 other. */
+ * class: 0 FP64 plane transforms, 1 FP64 z-contractions, 2 Helmholtz solves
+ * (FP64), 3 other, 4 FP32 z-contractions, 5 FP32 plane transforms.
+ * units: algorithmic work of the timed launches -- field-transforms of one
+ * Nx*Ny*Nz component (class 0) or full nz*nz contractions over the P
+ * spectral columns of one component (classes 1 and 2). */
 int tbd_profile_enable(tbd_ctx* ctx, int enable);
-int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms);
+int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* units);
 int tbd_profile_reset(tbd_ctx* ctx);
 
 /* Operators built by the context (pinned against closed forms in tests):

@@ -128,13 +128,14 @@ int tbd_profile_enable(tbd_ctx* ctx, int enable) {
   });
 }
 
-int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms) {
+int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* units) {
   return guard([&] {
-    TBD_CHECK(ctx && kclass >= 0 && kclass < 4, "tbd_profile_read > bad argument");
+    TBD_CHECK(ctx && kclass >= 0 && kclass < 6, "tbd_profile_read > bad argument");
     ctx->c->sync();
     ctx->c->prof.collect();
     if (launches) *launches = ctx->c->prof.launches[kclass];
     if (total_ms) *total_ms = ctx->c->prof.ms[kclass];
+    if (units) *units = ctx->c->prof.units[kclass];
   });
 }
 

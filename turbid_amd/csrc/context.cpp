@@ -44,6 +44,7 @@ void Profiler::collect() {
     TBD_HIP(hipEventElapsedTime(&t, r.a, r.b));
     launches[r.kclass] += 1;
     ms[r.kclass] += t;
+    units[r.kclass] += r.units;
     pool.push_back(r.a);
     pool.push_back(r.b);
   }
@@ -51,7 +52,7 @@ void Profiler::collect() {
 }
 void Profiler::reset() {
   collect();
-  for (int i = 0; i < 4; ++i) { launches[i] = 0; ms[i] = 0; }
+  for (int i = 0; i < 6; ++i) { launches[i] = 0; ms[i] = 0; units[i] = 0; }
 }
 Profiler::~Profiler() {
   for (auto& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -59,18 +60,21 @@ Profiler::~Profiler() {
 }
 
 template <class F>
-void Context::launch(int kclass, F&& f) {
+void Context::launch(int kclass, double units, F&& f) {
   if (!prof.on) {
     f();
     return;
   }
   Profiler::Rec r;
+  r.units = units;
   prof.begin(kclass, st_, r);
   f();
   prof.end(r, st_);
 }
 
-enum { KC_FFT = 0, KC_ZGEMM = 1, KC_HELM = 2, KC_OTHER = 3 };
+enum { KC_FFT = 0, KC_ZGEMM = 1, KC_HELM = 2, KC_OTHER = 3, KC_ZGEMM32 = 4, KC_FFT32 = 5 };
+template <typename T> static int kc_z() { return std::is_same<T, float>::value ? KC_ZGEMM32 : KC_ZGEMM; }
+template <typename T> static int kc_f() { return std::is_same<T, float>::value ? KC_FFT32 : KC_FFT; }
 
 void* Context::alloc(size_t bytes) {
   void* p = nullptr;
@@ -322,16 +326,16 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   const T* d1op = std::is_same<T, float>::value ? (const T*)d1f_ : (const T*)d1_;
 
   // wall gradients of the mean mode for the log (dudxyz(2) rows 0, N-1 col 0)
-  launch(KC_OTHER, [&] { wall_mean_rows(d1_, lda_, d.nz, uvw_, S, wallmean_, st_); });
+  launch(KC_OTHER, 0.0, [&] { wall_mean_rows(d1_, lda_, d.nz, uvw_, S, wallmean_, st_); });
   if (!alternation) {
     // convective form (phase_carrier_flat.cc:784-806)
     ZArgs a;
     a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
     a.x = uvw_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
-    launch(KC_ZGEMM, [&] { zgemm<T, double2, CT>(a, 3, st_); });
-    launch(KC_FFT, [&] { fft_backward<T, CT>(fft_, grad, S, gphys, R, 9, 0, d.nz, tmp, st_); });
-    launch(KC_OTHER, [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
-    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, aphys, R, adv, S, 3, 0, d.nz, tmp, st_); });
+    launch(kc_z<T>(), 3.0, [&] { zgemm<T, double2, CT>(a, 3, st_); });
+    launch(kc_f<T>(), 9.0, [&] { fft_backward<T, CT>(fft_, grad, S, gphys, R, 9, 0, d.nz, tmp, st_); });
+    launch(KC_OTHER, 0.0, [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
+    launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, aphys, R, adv, S, 3, 0, d.nz, tmp, st_); });
     if (cfg.nphase > 1 && cfg.scalar_bc == 1) {
       const size_t pl = (size_t)d.nx * d.ny * sizeof(T);
       TBD_HIP(hipMemcpyAsync(dudz_bot_, gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
@@ -339,8 +343,8 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     }
   } else {
     // divergence form (phase_carrier_flat.cc:725-779): six distinct products
-    launch(KC_OTHER, [&] { products_div<T>(uvwnl, gphys, R, st_); });
-    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, gphys, R, prod, S, 6, 0, d.nz, tmp, st_); });
+    launch(KC_OTHER, 0.0, [&] { products_div<T>(uvwnl, gphys, R, st_); });
+    launch(kc_f<T>(), 6.0, [&] { fft_forward<T, CT>(fft_, gphys, R, prod, S, 6, 0, d.nz, tmp, st_); });
     // adv_i = (i kx F_i0 + i ky F_i1) + D1 F_i2 ; rows (uu,uv,uw) (uv,vv,vw) (uw,vw,ww)
     const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     for (int f = 0; f < 3; ++f) {
@@ -348,17 +352,17 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       a.mode = Z_GRADTMP; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
       a.x = prod + rows[f][2] * S; a.e0 = prod + rows[f][0] * S; a.e1 = prod + rows[f][1] * S;
       a.y = adv + f * S; a.kx = kx_; a.ky = ky_;
-      launch(KC_ZGEMM, [&] { zgemm<T, CT, CT>(a, 1, st_); });
+      launch(kc_z<T>(), 1.0, [&] { zgemm<T, CT, CT>(a, 1, st_); });
     }
     // Gradient(u), Gradient(v) + DftBackward for the erosion map: only the
     // bottom plane of du/dz, dv/dz is read (phase_scalar_flat.cc:349-351)
     if (cfg.nphase > 1 && cfg.scalar_bc == 1) {
-      launch(KC_OTHER, [&] {
+      launch(KC_OTHER, 0.0, [&] {
         zrows_wall(d1_, lda_, d.nz, uvw_, d.P, wrows_, wrows_ + d.P, st_);
         zrows_wall(d1_, lda_, d.nz, uvw_ + S, d.P, wrows_ + 2 * d.P, wrows_ + 3 * d.P, st_);
       });
       const long long plane = (long long)d.nx * d.ny;
-      launch(KC_FFT, [&] {
+      launch(kc_f<T>(), 2.0 / d.nz, [&] {
         fft_backward<T, double2>(fft_, wrows_, 2 * d.P, (T*)dudz_bot_, plane, 1, 0, 1, tmp, st_, 1);
         fft_backward<T, double2>(fft_, wrows_ + 2 * d.P, 2 * d.P, (T*)dvdz_bot_, plane, 1, 0, 1, tmp, st_, 1);
       });
@@ -371,7 +375,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   u.K = dt_ * cfg.averaged_concentration / std::pow(cfg.froude, 2) * (cfg.specific_gravity - 1);
   u.sn = std::sin(cfg.slope); u.cs = std::cos(cfg.slope); u.slope_dir = cfg.slope_direction;
   forces(time, u.f1, u.f2);
-  launch(KC_OTHER, [&] { h_update<CT>(huvw_, adv, c_, u, 3, S, st_); });
+  launch(KC_OTHER, 0.0, [&] { h_update<CT>(huvw_, adv, c_, u, 3, S, st_); });
   // lap = D2 u - kx^2 u - ky^2 u ; rhs = sigma (c2 H + u) - lap (:517-536)
   {
     const double redtc3 = -cfg.reynolds / dt_ / c3;
@@ -379,7 +383,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     a.mode = Z_RHS; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d2_;
     a.x = uvw_; a.xs = S; a.y = rhs_; a.ys = S; a.h = huvw_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
-    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 3, st_); });
+    launch(KC_ZGEMM, 3.0, [&] { zgemm<double, double2, double2>(a, 3, st_); });
   }
   // boundary rows from the extrapolated wall pressure gradient (boundary_carrier_flat.cc:185-246)
   {
@@ -399,7 +403,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     b.n1t = wall2_[wslot_][1]; b.n2t = wall2_[1 - wslot_][1];
     b.open_top = opentop_ ? 1 : 0;
     b.top0 = cfg.vt_u1[4]; b.top1 = cfg.vt_u2[4]; b.top2 = cfg.vt_u3[4];
-    launch(KC_OTHER, [&] { bc_carrier(rhs_, b, st_); });
+    launch(KC_OTHER, 0.0, [&] { bc_carrier(rhs_, b, st_); });
   }
   // intermediate velocity: three Helmholtz solves (:554, :867-931)
   {
@@ -412,7 +416,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       h.f[f].x = rhs_ + f * S;
       h.f[f].y = suvw_ + f * S;
     }
-    launch(KC_HELM, [&] { helmholtz(h, 3, st_); });
+    launch(KC_HELM, 6.0, [&] { helmholtz(h, 3, st_); });
   }
 }
 
@@ -429,7 +433,7 @@ void Context::carrier_correct(int stage) {
     a.mode = Z_DIV; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1_;
     a.x = suvw_ + 2 * S; a.e0 = suvw_; a.e1 = suvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
     a.q = 0.5 / dt_ / c3;
-    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // pressure solve (:594, :955-987)
   {
@@ -440,7 +444,7 @@ void Context::carrier_correct(int stage) {
     h.f[0].nullrow = fac[3].nullrow;
     h.f[0].x = rhsp_;
     h.f[0].y = p_;
-    launch(KC_HELM, [&] { helmholtz(h, 1, st_); });
+    launch(KC_HELM, 2.0, [&] { helmholtz(h, 1, st_); });
   }
   // grad p, boundary Update (m2 <- m1, m1 <- walls), u = s - 2 dt c3 grad p,
   // mean w := 0 (:597-620, boundary_carrier_flat.cc:131-170)
@@ -454,13 +458,13 @@ void Context::carrier_correct(int stage) {
     a.wall_bot = wall_[wslot_][0]; a.wall_top = wall_[wslot_][1];
     a.wall2_bot = wall2_[wslot_][0]; a.wall2_top = wall2_[wslot_][1];
     a.open_top = opentop_ ? 1 : 0;
-    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // uvwnonlinear = DftBackward(uvw) (:625-626)
-  launch(KC_FFT, [&] { fft_backward<T, double2>(fft_, uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_, st_); });
+  launch(kc_f<T>(), 3.0, [&] { fft_backward<T, double2>(fft_, uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_, st_); });
   if (stage == 2) {   // diagnostics (:633-703)
     fill_zero(fl_, sizeof(double) * 20, st_);
-    launch(KC_OTHER, [&] {
+    launch(KC_OTHER, 0.0, [&] {
       diagnostics<T>((const T*)uvwnl_, R, d.nx, d.ny, d.nz, glx_, gly_, glz_, fl_, st_);
     });
     TBD_HIP(hipMemcpyAsync(fg_, fl_, sizeof(double) * 50, hipMemcpyDeviceToDevice, st_));
@@ -494,27 +498,27 @@ void Context::scalar_correct(int stage, int alternation) {
   const T sv = (T)(cfg.slope_direction == 2 ? ws * std::sin(cfg.slope) : 0.0);
   const T sw = (T)(-ws * std::cos(cfg.slope));
   if (alternation) {   // divergence form (:563-580)
-    launch(KC_FFT, [&] { fft_backward<T, double2>(fft_, c_, S, aphys, R, 1, 0, d.nz, tmp, st_); });
-    launch(KC_OTHER, [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
-    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp, st_); });
+    launch(kc_f<T>(), 1.0, [&] { fft_backward<T, double2>(fft_, c_, S, aphys, R, 1, 0, d.nz, tmp, st_); });
+    launch(KC_OTHER, 0.0, [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
+    launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp, st_); });
     ZArgs a;
     a.mode = Z_GRADTMP; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
     a.x = prod + 2 * S; a.e0 = prod; a.e1 = prod + S; a.y = advc; a.kx = kx_; a.ky = ky_;
-    launch(KC_ZGEMM, [&] { zgemm<T, CT, CT>(a, 1, st_); });
+    launch(kc_z<T>(), 1.0, [&] { zgemm<T, CT, CT>(a, 1, st_); });
   } else {             // convective form (:584-592)
     ZArgs a;
     a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
     a.x = c_; a.xs = S; a.y = grad + 6 * S; a.ys = S; a.kx = kx_; a.ky = ky_;
-    launch(KC_ZGEMM, [&] { zgemm<T, double2, CT>(a, 1, st_); });
-    launch(KC_FFT, [&] { fft_backward<T, CT>(fft_, grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp, st_); });
-    launch(KC_OTHER, [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
-    launch(KC_FFT, [&] { fft_forward<T, CT>(fft_, aphys, R, advc, S, 1, 0, d.nz, tmp, st_); });
+    launch(kc_z<T>(), 1.0, [&] { zgemm<T, double2, CT>(a, 1, st_); });
+    launch(kc_f<T>(), 3.0, [&] { fft_backward<T, CT>(fft_, grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp, st_); });
+    launch(KC_OTHER, 0.0, [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
+    launch(kc_f<T>(), 1.0, [&] { fft_forward<T, CT>(fft_, aphys, R, advc, S, 1, 0, d.nz, tmp, st_); });
   }
   // H = c1 H - dt adv (:313-315)
   {
     HUpd u;
     u.c1 = c1; u.dt = dt_; u.nz = 0;
-    launch(KC_OTHER, [&] { h_update<CT>(hc_, advc, c_, u, 1, S, st_); });
+    launch(KC_OTHER, 0.0, [&] { h_update<CT>(hc_, advc, c_, u, 1, S, st_); });
   }
   // lap, rhs (:319-329)
   {
@@ -523,28 +527,28 @@ void Context::scalar_correct(int stage, int alternation) {
     a.mode = Z_RHS; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d2_;
     a.x = c_; a.xs = S; a.y = rhsc_; a.ys = S; a.h = hc_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
-    launch(KC_ZGEMM, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // boundary rows + erosion/deposition (boundary_scalar_flat.cc:151-152, :340-384)
   const int erosion = cfg.scalar_bc == 1 ? 1 : 0;
   if (erosion) {
     const long long plane = (long long)d.nx * d.ny;
-    launch(KC_OTHER, [&] {
+    launch(KC_OTHER, 0.0, [&] {
       erosion_map<T>((const T*)dudz_bot_, (const T*)dvdz_bot_, ephys_, plane, cfg.reynolds,
                      cfg.critical_shear_stress, cfg.erosion_rate, st_);
     });
-    launch(KC_FFT, [&] {
+    launch(KC_FFT, 1.0 / d.nz, [&] {
       fft_forward<double, double2>(fft_, ephys_, plane, ehat_, d.P, 1, 0, 1, (double2*)tmp_, st_, 1);
     });
   }
   const double wsc = cfg.settling_velocity * std::cos(cfg.slope);
-  launch(KC_OTHER, [&] {
+  launch(KC_OTHER, 0.0, [&] {
     bc_scalar(rhsc_, d.nz, d.P, ehat_, c_, wsc, cfg.averaged_concentration, cfg.reynolds, cfg.schmidt, erosion,
               st_);
   });
   if (erosion) {   // feature 28: mean erosion flux, 29: mean deposition flux (:377-380)
     TBD_HIP(hipMemcpyAsync(fl_ + 28, ehat_, sizeof(double), hipMemcpyDeviceToDevice, st_));
-    launch(KC_OTHER, [&] { scaled_real(c_, wsc, cfg.averaged_concentration, fl_ + 29, st_); });
+    launch(KC_OTHER, 0.0, [&] { scaled_real(c_, wsc, cfg.averaged_concentration, fl_ + 29, st_); });
   }
   // Helmholtz (:399, :640-661)
   {
@@ -555,13 +559,13 @@ void Context::scalar_correct(int stage, int alternation) {
     h.f[0].nullrow = -1;
     h.f[0].x = rhsc_;
     h.f[0].y = c_;
-    launch(KC_HELM, [&] { helmholtz(h, 1, st_); });
+    launch(KC_HELM, 2.0, [&] { helmholtz(h, 1, st_); });
   }
   // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
-  launch(KC_FFT, [&] { fft_backward<double, double2>(fft_, c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_, st_); });
-  launch(KC_OTHER, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, st_); });
-  launch(KC_FFT, [&] { fft_forward<double, double2>(fft_, cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_, st_); });
-  launch(KC_OTHER, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });
+  launch(KC_FFT, 1.0, [&] { fft_backward<double, double2>(fft_, c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_, st_); });
+  launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, st_); });
+  launch(KC_FFT, 1.0, [&] { fft_forward<double, double2>(fft_, cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_, st_); });
+  launch(KC_OTHER, 0.0, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });
 }
 
 void Context::substage(int stage, double time, int alternation) {

@@ -14,11 +14,12 @@ namespace tbd {
 
 struct Profiler {
   bool on = false;
-  struct Rec { int kclass; hipEvent_t a, b; };
+  struct Rec { int kclass; double units; hipEvent_t a, b; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
-  long long launches[4] = {0, 0, 0, 0};
-  double ms[4] = {0, 0, 0, 0};
+  long long launches[6] = {0, 0, 0, 0, 0, 0};
+  double ms[6] = {0, 0, 0, 0, 0, 0};
+  double units[6] = {0, 0, 0, 0, 0, 0};   // contractions / field-transforms
   hipEvent_t get();
   void begin(int kclass, hipStream_t st, Rec& r);
   void end(Rec& r, hipStream_t st);
@@ -52,7 +53,7 @@ class Context {
   template <typename T> void carrier_correct(int stage);
   template <typename T> void scalar_correct(int stage, int alternation);
   void forces(double time, double& f1, double& f2) const;
-  template <class F> void launch(int kclass, F&& f);
+  template <class F> void launch(int kclass, double units, F&& f);
   void* alloc(size_t bytes);
 
   hipStream_t st_ = nullptr;

@@ -39,7 +39,7 @@ EXPORTS = [
     "tbd_get_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
 ]
 
-KCLASS = {"fft": 0, "zgemm": 1, "helmholtz": 2, "other": 3}
+KCLASS = {"fft": 0, "zgemm": 1, "helmholtz": 2, "other": 3, "zgemm32": 4, "fft32": 5}
 
 
 class TbdConfig(ctypes.Structure):
@@ -90,7 +90,7 @@ def lib():
         L.tbd_features.argtypes = [vp, dp]
         L.tbd_sync.argtypes = [vp]
         L.tbd_profile_enable.argtypes = [vp, ctypes.c_int]
-        L.tbd_profile_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), dp]
+        L.tbd_profile_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), dp, dp]
         L.tbd_profile_reset.argtypes = [vp]
         L.tbd_get_operators.argtypes = [vp, dp, dp, dp]
         L.tbd_get_factorisation.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
@@ -217,10 +217,13 @@ class Solver:
         _check(lib().tbd_profile_reset(self._h))
 
     def profile_read(self, kclass):
+        """(launches, total_ms, units) of a kernel class since the last reset."""
         n = ctypes.c_longlong()
         ms = ctypes.c_double()
-        _check(lib().tbd_profile_read(self._h, KCLASS.get(kclass, kclass), ctypes.byref(n), ctypes.byref(ms)))
-        return n.value, ms.value
+        un = ctypes.c_double()
+        _check(lib().tbd_profile_read(self._h, KCLASS.get(kclass, kclass), ctypes.byref(n), ctypes.byref(ms),
+                                      ctypes.byref(un)))
+        return n.value, ms.value, un.value
 
     def operators(self):
         n = self.nz
