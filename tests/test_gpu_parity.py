@@ -145,3 +145,15 @@ def test_steps_mixed():
     ug, pg, cg = g.download()
     assert rel_l2(ug, uo) < 1e-5
     assert rel_l2(cg, co) < 1e-5
+
+
+def test_steps_unfused_path(monkeypatch):
+    # the unfused transform path (separate x passes + product kernels) stays
+    # interchangeable with the fused one
+    monkeypatch.setenv("TBD_UNFUSED", "1")
+    cfg = small_config()
+    o, g = _run_pair(cfg, 2)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    assert rel_l2(ug, uo) < 1e-9
+    assert rel_l2(cg, co) < 1e-9

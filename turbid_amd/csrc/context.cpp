@@ -4,9 +4,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <type_traits>
 
+#include "nonlinear.h"
 #include "pointwise.h"
 #include "zgemm.h"
 
@@ -102,6 +105,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   TBD_CHECK(d.nyc >= 2 && d.mx >= 1, "Context > grid too small for the 3/2 rule");
   TBD_HIP(hipStreamCreate(&st_));
   fft_.create(d);
+  if (const char* e = std::getenv("TBD_UNFUSED")) fused_ = std::string(e) != "1";
   dt_ = olddt_ = cfg.dt;
   const double c1[3] = {0.0 / 1.0, -5.0 / 9.0, -153.0 / 128.0};    // stepper_time_rk3.cc:29-31
   const double c2[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
@@ -333,18 +337,34 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
     a.x = uvw_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
     launch(kc_z<T>(), 3.0, [&] { zgemm<T, double2, CT>(a, 3, st_); });
-    launch(kc_f<T>(), 9.0, [&] { fft_backward<T, CT>(fft_, grad, S, gphys, R, 9, 0, d.nz, tmp, st_); });
-    launch(KC_OTHER, 0.0, [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
-    launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, aphys, R, adv, S, 3, 0, d.nz, tmp, st_); });
-    if (cfg.nphase > 1 && cfg.scalar_bc == 1) {
-      const size_t pl = (size_t)d.nx * d.ny * sizeof(T);
-      TBD_HIP(hipMemcpyAsync(dudz_bot_, gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
-      TBD_HIP(hipMemcpyAsync(dvdz_bot_, gphys + 5 * R, pl, hipMemcpyDeviceToDevice, st_));
+    const bool erosion_planes = cfg.nphase > 1 && cfg.scalar_bc == 1;
+    if (fused_) {
+      // y^-1 of the nine gradients; x^-1, u.grad u_i, x in one kernel; y
+      launch(kc_f<T>(), 9.0, [&] { fft_ypass_backward<T, CT>(fft_, grad, S, tmp, 9, 0, d.nz, st_); });
+      launch(kc_f<T>(), 0.0, [&] {
+        xfused_conv<T>(fft_, tmp, uvwnl, R, (T)0, (T)0, (T)0, 3, tmp, erosion_planes ? (T*)dudz_bot_ : nullptr,
+                       erosion_planes ? (T*)dvdz_bot_ : nullptr, st_);
+      });
+      launch(kc_f<T>(), 3.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, adv, S, 3, 0, d.nz, st_); });
+    } else {
+      launch(kc_f<T>(), 9.0, [&] { fft_backward<T, CT>(fft_, grad, S, gphys, R, 9, 0, d.nz, tmp, st_); });
+      launch(KC_OTHER, 0.0, [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
+      launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, aphys, R, adv, S, 3, 0, d.nz, tmp, st_); });
+      if (erosion_planes) {
+        const size_t pl = (size_t)d.nx * d.ny * sizeof(T);
+        TBD_HIP(hipMemcpyAsync(dudz_bot_, gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
+        TBD_HIP(hipMemcpyAsync(dvdz_bot_, gphys + 5 * R, pl, hipMemcpyDeviceToDevice, st_));
+      }
     }
   } else {
     // divergence form (phase_carrier_flat.cc:725-779): six distinct products
-    launch(KC_OTHER, 0.0, [&] { products_div<T>(uvwnl, gphys, R, st_); });
-    launch(kc_f<T>(), 6.0, [&] { fft_forward<T, CT>(fft_, gphys, R, prod, S, 6, 0, d.nz, tmp, st_); });
+    if (fused_) {
+      launch(kc_f<T>(), 0.0, [&] { xfused_div6<T>(fft_, uvwnl, R, tmp, st_); });
+      launch(kc_f<T>(), 6.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, prod, S, 6, 0, d.nz, st_); });
+    } else {
+      launch(KC_OTHER, 0.0, [&] { products_div<T>(uvwnl, gphys, R, st_); });
+      launch(kc_f<T>(), 6.0, [&] { fft_forward<T, CT>(fft_, gphys, R, prod, S, 6, 0, d.nz, tmp, st_); });
+    }
     // adv_i = (i kx F_i0 + i ky F_i1) + D1 F_i2 ; rows (uu,uv,uw) (uv,vv,vw) (uw,vw,ww)
     const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     for (int f = 0; f < 3; ++f) {
@@ -498,9 +518,15 @@ void Context::scalar_correct(int stage, int alternation) {
   const T sv = (T)(cfg.slope_direction == 2 ? ws * std::sin(cfg.slope) : 0.0);
   const T sw = (T)(-ws * std::cos(cfg.slope));
   if (alternation) {   // divergence form (:563-580)
-    launch(kc_f<T>(), 1.0, [&] { fft_backward<T, double2>(fft_, c_, S, aphys, R, 1, 0, d.nz, tmp, st_); });
-    launch(KC_OTHER, 0.0, [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
-    launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp, st_); });
+    if (fused_) {
+      launch(kc_f<T>(), 1.0, [&] { fft_ypass_backward<T, double2>(fft_, c_, S, tmp, 1, 0, d.nz, st_); });
+      launch(kc_f<T>(), 0.0, [&] { xfused_cdiv<T>(fft_, tmp, uvwnl, R, su, sv, sw, tmp, st_); });
+      launch(kc_f<T>(), 3.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, prod, S, 3, 0, d.nz, st_); });
+    } else {
+      launch(kc_f<T>(), 1.0, [&] { fft_backward<T, double2>(fft_, c_, S, aphys, R, 1, 0, d.nz, tmp, st_); });
+      launch(KC_OTHER, 0.0, [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
+      launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp, st_); });
+    }
     ZArgs a;
     a.mode = Z_GRADTMP; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
     a.x = prod + 2 * S; a.e0 = prod; a.e1 = prod + S; a.y = advc; a.kx = kx_; a.ky = ky_;
@@ -510,9 +536,19 @@ void Context::scalar_correct(int stage, int alternation) {
     a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
     a.x = c_; a.xs = S; a.y = grad + 6 * S; a.ys = S; a.kx = kx_; a.ky = ky_;
     launch(kc_z<T>(), 1.0, [&] { zgemm<T, double2, CT>(a, 1, st_); });
-    launch(kc_f<T>(), 3.0, [&] { fft_backward<T, CT>(fft_, grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp, st_); });
-    launch(KC_OTHER, 0.0, [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
-    launch(kc_f<T>(), 1.0, [&] { fft_forward<T, CT>(fft_, aphys, R, advc, S, 1, 0, d.nz, tmp, st_); });
+    if (fused_) {
+      launch(kc_f<T>(), 3.0, [&] { fft_ypass_backward<T, CT>(fft_, grad + 6 * S, S, tmp, 3, 0, d.nz, st_); });
+      launch(kc_f<T>(), 0.0, [&] {
+        xfused_conv<T>(fft_, tmp, uvwnl, R, su, sv, sw, 1, tmp, nullptr, nullptr, st_);
+      });
+      launch(kc_f<T>(), 1.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, advc, S, 1, 0, d.nz, st_); });
+    } else {
+      launch(kc_f<T>(), 3.0, [&] {
+        fft_backward<T, CT>(fft_, grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp, st_);
+      });
+      launch(KC_OTHER, 0.0, [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
+      launch(kc_f<T>(), 1.0, [&] { fft_forward<T, CT>(fft_, aphys, R, advc, S, 1, 0, d.nz, tmp, st_); });
+    }
   }
   // H = c1 H - dt adv (:313-315)
   {

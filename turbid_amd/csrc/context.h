@@ -60,6 +60,7 @@ class Context {
   FftPlan fft_;
   int lda_ = 0;
   bool opentop_ = false;
+  bool fused_ = true;       // x transforms fused around the nonlinear products
   double dt_ = 0, olddt_ = 0;
   double coef_[3][3];
 

@@ -1,0 +1,285 @@
+// Fused x transforms + nonlinear products (see nonlinear.h).
+#include "nonlinear.h"
+
+#include "fft_device.h"
+
+namespace tbd {
+
+static constexpr int kNT = 256;
+static constexpr int kPPT = 8;     // packed row-pair positions per thread
+static constexpr int kFftB = 4;    // butterflies per thread per stage
+
+struct XfArgs {
+  FftLen L;
+  int ny = 0, mx = 0, nz = 0, nseq = 0, nout = 0;
+  long long R = 0;
+  const void* tw = nullptr;
+  const void* gin = nullptr;
+  const void* U = nullptr;
+  void* gout = nullptr;
+  void* bot0 = nullptr;
+  void* bot1 = nullptr;
+  double su = 0, sv = 0, sw = 0;
+};
+
+// inverse x transform input: zero-padded, Hermitian-completed (halfcomplex)
+template <typename T>
+__device__ __forceinline__ void load_c2r(C2<T>* buf, int ldn, const C2<T>* in, int N, int ny, int mx, int y0,
+                                         int nseq) {
+  for (int idx = threadIdx.x; idx < nseq * N; idx += kNT) {
+    const int s = idx / N, x = idx - s * N;
+    buf[s * ldn + x] = mk<T>((T)0, (T)0);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < mx * nseq; idx += kNT) {
+    const int k = idx / nseq, s = idx - k * nseq;
+    const int y = y0 + 2 * s;
+    C2<T> A = (y < ny) ? in[(long long)k * ny + y] : mk<T>((T)0, (T)0);
+    C2<T> B = (y + 1 < ny) ? in[(long long)k * ny + y + 1] : mk<T>((T)0, (T)0);
+    const bool selfconj = (k == 0) || (2 * k == N);
+    if (selfconj) { A.y = (T)0; B.y = (T)0; }
+    buf[s * ldn + k] = mk<T>(A.x - B.y, A.y + B.x);
+    if (!selfconj) buf[s * ldn + N - k] = mk<T>(A.x + B.y, B.x - A.y);
+  }
+  __syncthreads();
+}
+
+// forward x transform output: truncate to kx < mx, unpack the two rows
+template <typename T>
+__device__ __forceinline__ void store_r2c(const C2<T>* buf, int ldn, C2<T>* out, int N, int ny, int mx, int y0,
+                                          int nseq) {
+  const int nr = 2 * nseq;
+  for (int idx = threadIdx.x; idx < mx * nr; idx += kNT) {
+    const int k = idx / nr, r = idx - k * nr;
+    const int y = y0 + r;
+    if (y >= ny) continue;
+    const int s = r >> 1;
+    const C2<T> zk = buf[s * ldn + k];
+    const C2<T> zn = buf[s * ldn + (k == 0 ? 0 : N - k)];
+    C2<T> val;
+    if ((r & 1) == 0) val = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+    else val = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
+    out[(long long)k * ny + y] = val;
+  }
+}
+
+// packed physical rows (2s, 2s+1) of a field at this thread's positions
+template <typename T>
+__device__ __forceinline__ void load_rows(C2<T>* v, const T* base, int N, int ny, int y0, int nseq, T shift) {
+#pragma unroll
+  for (int b = 0; b < kPPT; ++b) {
+    const int q = threadIdx.x + b * kNT;
+    if (q < nseq * N) {
+      const int s = q / N, x = q - s * N;
+      const int y = y0 + 2 * s;
+      const T a = (y < ny) ? base[(long long)y * N + x] + shift : (T)0;
+      const T c = (y + 1 < ny) ? base[(long long)(y + 1) * N + x] + shift : (T)0;
+      v[b] = mk<T>(a, c);
+    }
+  }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kNT) void k_xfused(XfArgs a) {
+  extern __shared__ char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int N = a.L.n;
+  const int ldn = lds_ld(N);
+  C2<T>* tw = buf + a.nseq * ldn;
+  const int z = blockIdx.y;
+  const int nseq = a.nseq;
+  const int y0 = blockIdx.x * 2 * nseq;
+  const long long plane = (long long)a.mx * a.ny;            // x-pencil plane
+  const long long pstride = (long long)a.nz * plane;          // x-pencil component
+  const C2<T>* gin = (const C2<T>*)a.gin;
+  C2<T>* gout = (C2<T>*)a.gout;
+  const T* U = (const T*)a.U;
+  const T shifts[3] = {(T)a.su, (T)a.sv, (T)a.sw};
+  const T* Ub = U + (long long)z * a.ny * N;
+  load_tw<T, kNT>(tw, (const C2<T>*)a.tw, N);
+  __syncthreads();
+
+  if (MODE == 0) {
+    // adv_i = sum_j U_j g_ij
+    C2<T> acc[3][kPPT];
+    C2<T> uj[kPPT];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int b = 0; b < kPPT; ++b) acc[i][b] = mk<T>((T)0, (T)0);
+    for (int j = 0; j < 3; ++j) {
+      load_rows<T>(uj, Ub + j * a.R, N, a.ny, y0, nseq, shifts[j]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (i < a.nout) {   // uniform across the block
+          const int gcomp = (a.nout == 3) ? 3 * i + j : j;
+          load_c2r<T>(buf, ldn, gin + gcomp * pstride + z * plane, N, a.ny, a.mx, y0, nseq);
+          lds_fft<T, kNT, kFftB>(buf, ldn, nseq, a.L, tw, +1);
+          if (a.bot0 && z == 0 && j == 2 && i < 2) {   // physical bottom plane of du/dz, dv/dz
+            T* bot = (T*)(i == 0 ? a.bot0 : a.bot1);
+            for (int idx = threadIdx.x; idx < nseq * N; idx += kNT) {
+              const int s = idx / N, x = idx - s * N;
+              const int y = y0 + 2 * s;
+              const C2<T> v = buf[s * ldn + x];
+              if (y < a.ny) bot[(long long)y * N + x] = v.x;
+              if (y + 1 < a.ny) bot[(long long)(y + 1) * N + x] = v.y;
+            }
+          }
+#pragma unroll
+          for (int b = 0; b < kPPT; ++b) {
+            const int q = threadIdx.x + b * kNT;
+            if (q < nseq * N) {
+              const int s = q / N, x = q - s * N;
+              const C2<T> g = buf[s * ldn + x];
+              acc[i][b].x = acc[i][b].x + uj[b].x * g.x;
+              acc[i][b].y = acc[i][b].y + uj[b].y * g.y;
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (i < a.nout) {
+#pragma unroll
+        for (int b = 0; b < kPPT; ++b) {
+          const int q = threadIdx.x + b * kNT;
+          if (q < nseq * N) {
+            const int s = q / N, x = q - s * N;
+            buf[s * ldn + x] = acc[i][b];
+          }
+        }
+        __syncthreads();
+        lds_fft<T, kNT, kFftB>(buf, ldn, nseq, a.L, tw, -1);
+        store_r2c<T>(buf, ldn, gout + i * pstride + z * plane, N, a.ny, a.mx, y0, nseq);
+        __syncthreads();
+      }
+    }
+  } else if (MODE == 1) {
+    // uu, uv, uw, vv, vw, ww
+    C2<T> u[3][kPPT];
+    for (int j = 0; j < 3; ++j) load_rows<T>(u[j], Ub + j * a.R, N, a.ny, y0, nseq, (T)0);
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2};
+    constexpr int pb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int o = 0; o < 6; ++o) {
+#pragma unroll
+      for (int b = 0; b < kPPT; ++b) {
+        const int q = threadIdx.x + b * kNT;
+        if (q < nseq * N) {
+          const int s = q / N, x = q - s * N;
+          const C2<T> ua = u[pa[o]][b], ub = u[pb[o]][b];
+          buf[s * ldn + x] = mk<T>(ua.x * ub.x, ua.y * ub.y);
+        }
+      }
+      __syncthreads();
+      lds_fft<T, kNT, kFftB>(buf, ldn, nseq, a.L, tw, -1);
+      store_r2c<T>(buf, ldn, gout + o * pstride + z * plane, N, a.ny, a.mx, y0, nseq);
+      __syncthreads();
+    }
+  } else {
+    // c (U_j + shift_j), j = 0..2
+    C2<T> cv[kPPT];
+    C2<T> uj[kPPT];
+    load_c2r<T>(buf, ldn, gin + z * plane, N, a.ny, a.mx, y0, nseq);
+    lds_fft<T, kNT, kFftB>(buf, ldn, nseq, a.L, tw, +1);
+#pragma unroll
+    for (int b = 0; b < kPPT; ++b) {
+      const int q = threadIdx.x + b * kNT;
+      if (q < nseq * N) {
+        const int s = q / N, x = q - s * N;
+        cv[b] = buf[s * ldn + x];
+      }
+    }
+    __syncthreads();
+    for (int j = 0; j < 3; ++j) {
+      load_rows<T>(uj, Ub + j * a.R, N, a.ny, y0, nseq, shifts[j]);
+#pragma unroll
+      for (int b = 0; b < kPPT; ++b) {
+        const int q = threadIdx.x + b * kNT;
+        if (q < nseq * N) {
+          const int s = q / N, x = q - s * N;
+          buf[s * ldn + x] = mk<T>(cv[b].x * uj[b].x, cv[b].y * uj[b].y);
+        }
+      }
+      __syncthreads();
+      lds_fft<T, kNT, kFftB>(buf, ldn, nseq, a.L, tw, -1);
+      store_r2c<T>(buf, ldn, gout + j * pstride + z * plane, N, a.ny, a.mx, y0, nseq);
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T> static const void* twx_of(const FftPlan& p);
+template <> const void* twx_of<double>(const FftPlan& p) { return p.twx64; }
+template <> const void* twx_of<float>(const FftPlan& p) { return p.twx32; }
+
+template <typename T>
+static XfArgs make_args(const FftPlan& p, long long R) {
+  XfArgs a;
+  a.L = p.lx;
+  a.ny = p.d.ny;
+  a.mx = p.d.mx;
+  a.nz = p.d.nz;
+  a.R = R;
+  a.tw = twx_of<T>(p);
+  const int n = p.d.nx;
+  int nseq = (kPPT * kNT) / n;           // positions per thread <= kPPT
+  nseq = std::min(nseq, 2 * kFftB * kNT / n);
+  nseq = std::min(nseq, (p.d.ny + 1) / 2);
+  a.nseq = std::max(1, std::min(nseq, 16));
+  TBD_CHECK(a.nseq * n <= kPPT * kNT && a.nseq * n / 2 <= kFftB * kNT, "xfused > row length too large");
+  return a;
+}
+
+template <typename T, int MODE>
+static void launch_xf(const FftPlan& p, const XfArgs& a, hipStream_t st) {
+  const int n = p.d.nx;
+  const size_t lds = sizeof(C2<T>) * ((size_t)a.nseq * lds_ld(n) + n);
+  dim3 grid(ceil_div(p.d.ny, 2 * a.nseq), p.d.nz);
+  hipLaunchKernelGGL((k_xfused<T, MODE>), grid, dim3(kNT), lds, st, a);
+  TBD_HIP(hipGetLastError());
+}
+
+template <typename T>
+void xfused_conv(const FftPlan& p, const C2<T>* g, const T* U, long long R, T su, T sv, T sw, int nout, C2<T>* adv,
+                 T* bot0, T* bot1, hipStream_t st) {
+  TBD_CHECK(nout == 1 || nout == 3, "xfused_conv > nout must be 1 or 3");
+  XfArgs a = make_args<T>(p, R);
+  a.gin = g; a.U = U; a.gout = adv; a.nout = nout;
+  a.su = su; a.sv = sv; a.sw = sw;
+  a.bot0 = (nout == 3) ? bot0 : nullptr;
+  a.bot1 = (nout == 3) ? bot1 : nullptr;
+  if (a.bot0 && !a.bot1) a.bot0 = nullptr;
+  launch_xf<T, 0>(p, a, st);
+}
+
+template <typename T>
+void xfused_div6(const FftPlan& p, const T* U, long long R, C2<T>* prod, hipStream_t st) {
+  XfArgs a = make_args<T>(p, R);
+  a.U = U; a.gout = prod; a.nout = 6;
+  launch_xf<T, 1>(p, a, st);
+}
+
+template <typename T>
+void xfused_cdiv(const FftPlan& p, const C2<T>* c, const T* U, long long R, T su, T sv, T sw, C2<T>* prod,
+                 hipStream_t st) {
+  XfArgs a = make_args<T>(p, R);
+  a.gin = c; a.U = U; a.gout = prod; a.nout = 3;
+  a.su = su; a.sv = sv; a.sw = sw;
+  launch_xf<T, 2>(p, a, st);
+}
+
+template void xfused_conv<double>(const FftPlan&, const double2*, const double*, long long, double, double, double,
+                                  int, double2*, double*, double*, hipStream_t);
+template void xfused_conv<float>(const FftPlan&, const float2*, const float*, long long, float, float, float, int,
+                                 float2*, float*, float*, hipStream_t);
+template void xfused_div6<double>(const FftPlan&, const double*, long long, double2*, hipStream_t);
+template void xfused_div6<float>(const FftPlan&, const float*, long long, float2*, hipStream_t);
+template void xfused_cdiv<double>(const FftPlan&, const double2*, const double*, long long, double, double, double,
+                                  double2*, hipStream_t);
+template void xfused_cdiv<float>(const FftPlan&, const float2*, const float*, long long, float, float, float, float2*,
+                                 hipStream_t);
+
+}  // namespace tbd
