@@ -212,6 +212,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   fl_ = (double*)alloc(sizeof(double) * 64);
   fg_ = (double*)alloc(sizeof(double) * 64);
   wallmean_ = (double*)alloc(sizeof(double) * 8);
+  diagpart_ = (double*)alloc(sizeof(double) * 10 * kDiagMaxBlocks);
   TBD_HIP(hipStreamSynchronize(st_));
 }
 
@@ -485,7 +486,7 @@ void Context::carrier_correct(int stage) {
   if (stage == 2) {   // diagnostics (:633-703)
     fill_zero(fl_, sizeof(double) * 20, st_);
     launch(KC_OTHER, 0.0, [&] {
-      diagnostics<T>((const T*)uvwnl_, R, d.nx, d.ny, d.nz, glx_, gly_, glz_, fl_, st_);
+      diagnostics<T>((const T*)uvwnl_, R, d.nx, d.ny, d.nz, glx_, gly_, glz_, fl_, diagpart_, st_);
     });
     TBD_HIP(hipMemcpyAsync(fg_, fl_, sizeof(double) * 50, hipMemcpyDeviceToDevice, st_));
     // 10..15: dU/dz, dV/dz, dW/dz on the mean column at bottom, then top

@@ -107,6 +107,7 @@ class Context {
   double* fl_ = nullptr;       // featurelocal_ (64)
   double* fg_ = nullptr;       // featureglobal_ (64)
   double* wallmean_ = nullptr; // 6: D1 u/v/w rows 0 and nz-1 on the mean column
+  double* diagpart_ = nullptr; // per-block maxima of the diagnostics
 
   std::vector<void*> allocs_;
 };

@@ -62,10 +62,11 @@ void bartnicki_filter(double* c, int nx, int ny, int nz, hipStream_t st);
 
 // Diagnostics (phase_carrier_flat.cc:639-671): feat[0..8] max |u|,|v|,|w| over
 // all / bottom / top planes, feat[19] max(|u|/dx + |v|/dy + |w|/dz(z)).
-// feat must be zeroed (non-negative max via integer atomics).
+// part: kDiagMaxBlocks * 10 doubles of scratch for the per-block maxima.
+static constexpr int kDiagMaxBlocks = 2048;
 template <typename T>
 void diagnostics(const T* uvw, long long R, int nx, int ny, int nz, double glx, double gly, const double* glz,
-                 double* feat, hipStream_t st);
+                 double* feat, double* part, hipStream_t st);
 
 // feat[21] = trapz(gridz, Re c(:,0)) / 2 (phase_scalar_flat.cc:485-486)
 void trapz_mean(const double2* c, const double* gridz, int nz, double* out, hipStream_t st);

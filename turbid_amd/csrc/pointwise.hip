@@ -217,40 +217,76 @@ void bartnicki_filter(double* c, int nx, int ny, int nz, hipStream_t st) {
   TBD_HIP(hipGetLastError());
 }
 
-__device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
-  atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+// Max-norm and CFL diagnostics: per-block partial maxima (z planes strided
+// over blockIdx.y, plane points over blockIdx.x), then one block reduces the
+// partials.  No atomics, so the result does not depend on scheduling.
+static constexpr int kDiagVals = 10;
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
 }
-
 template <typename T>
-__global__ void k_diag(const T* __restrict__ U, long long R, int nx, int ny, int nz, double glx, double gly,
-                       const double* __restrict__ glz, double* feat) {
-  const long long plane = (long long)nx * ny;
-  double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  GRID_STRIDE(e, R) {
-    const double a = fabs((double)U[e]), b = fabs((double)U[R + e]), w = fabs((double)U[2 * R + e]);
-    const int z = (int)(e / plane);
-    m[0] = fmax(m[0], a);
-    m[1] = fmax(m[1], b);
-    m[2] = fmax(m[2], w);
-    if (z == 0) { m[3] = fmax(m[3], a); m[4] = fmax(m[4], b); m[5] = fmax(m[5], w); }
-    if (z == nz - 1) { m[6] = fmax(m[6], a); m[7] = fmax(m[7], b); m[8] = fmax(m[8], w); }
-    m[9] = fmax(m[9], a / glx + b / gly + w / glz[z]);
+__global__ __launch_bounds__(kNT) void k_diag_part(const T* __restrict__ U, long long R, int plane, int nz, double glx,
+                                                    double gly, const double* __restrict__ glz,
+                                                    double* __restrict__ part) {
+  double m[kDiagVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int z = blockIdx.y; z < nz; z += gridDim.y) {
+    const double gz = glz[z];
+    const T* u = U + (long long)z * plane;
+    double a0 = 0, a1 = 0, a2 = 0, cf = 0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < plane; e += gridDim.x * blockDim.x) {
+      const double a = fabs((double)u[e]), b = fabs((double)u[R + e]), w = fabs((double)u[2 * R + e]);
+      a0 = fmax(a0, a);
+      a1 = fmax(a1, b);
+      a2 = fmax(a2, w);
+      cf = fmax(cf, a / glx + b / gly + w / gz);
+    }
+    m[0] = fmax(m[0], a0);
+    m[1] = fmax(m[1], a1);
+    m[2] = fmax(m[2], a2);
+    m[9] = fmax(m[9], cf);
+    if (z == 0) { m[3] = fmax(m[3], a0); m[4] = fmax(m[4], a1); m[5] = fmax(m[5], a2); }
+    if (z == nz - 1) { m[6] = fmax(m[6], a0); m[7] = fmax(m[7], a1); m[8] = fmax(m[8], a2); }
   }
+  __shared__ double red[kDiagVals][kNT / 64];
 #pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    double v = m[i];
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    m[i] = v;
+  for (int i = 0; i < kDiagVals; ++i) {
+    const double v = wave_max(m[i]);
+    if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = v;
   }
-  if ((threadIdx.x & 63) == 0) {
-    for (int i = 0; i < 9; ++i) atomic_max_nonneg(feat + i, m[i]);
-    atomic_max_nonneg(feat + 19, m[9]);
+  __syncthreads();
+  if (threadIdx.x < kDiagVals) {
+    double v = 0;
+    for (int w = 0; w < kNT / 64; ++w) v = fmax(v, red[threadIdx.x][w]);
+    part[(long long)(blockIdx.y * gridDim.x + blockIdx.x) * kDiagVals + threadIdx.x] = v;
+  }
+}
+__global__ __launch_bounds__(kNT) void k_diag_final(const double* __restrict__ part, int nblk, double* feat) {
+  __shared__ double red[kDiagVals][kNT / 64];
+  for (int i = 0; i < kDiagVals; ++i) {
+    double v = 0;
+    for (int b = threadIdx.x; b < nblk; b += kNT) v = fmax(v, part[(long long)b * kDiagVals + i]);
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kDiagVals) {
+    double v = 0;
+    for (int w = 0; w < kNT / 64; ++w) v = fmax(v, red[threadIdx.x][w]);
+    feat[threadIdx.x < 9 ? threadIdx.x : 19] = v;
   }
 }
 template <typename T>
 void diagnostics(const T* uvw, long long R, int nx, int ny, int nz, double glx, double gly, const double* glz,
-                 double* feat, hipStream_t st) {
-  hipLaunchKernelGGL(k_diag<T>, grid1d(R, 8), dim3(kNT), 0, st, uvw, R, nx, ny, nz, glx, gly, glz, feat);
+                 double* feat, double* part, hipStream_t st) {
+  const int plane = nx * ny;
+  int nbx = (plane + kNT * 8 - 1) / (kNT * 8);
+  nbx = nbx < 1 ? 1 : (nbx > 32 ? 32 : nbx);
+  int nby = kDiagMaxBlocks / nbx;
+  nby = nby > nz ? nz : (nby < 1 ? 1 : nby);
+  hipLaunchKernelGGL(k_diag_part<T>, dim3(nbx, nby), dim3(kNT), 0, st, uvw, R, plane, nz, glx, gly, glz, part);
+  TBD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kNT), 0, st, (const double*)part, nbx * nby, feat);
   TBD_HIP(hipGetLastError());
 }
 
@@ -332,9 +368,9 @@ template void h_update<float2>(double2*, const float2*, const double2*, const HU
 template void erosion_map<double>(const double*, const double*, double*, long long, double, double, double,
                                   hipStream_t);
 template void erosion_map<float>(const float*, const float*, double*, long long, double, double, double, hipStream_t);
-template void diagnostics<double>(const double*, long long, int, int, int, double, double, const double*, double*,
+template void diagnostics<double>(const double*, long long, int, int, int, double, double, const double*, double*, double*,
                                   hipStream_t);
-template void diagnostics<float>(const float*, long long, int, int, int, double, double, const double*, double*,
+template void diagnostics<float>(const float*, long long, int, int, int, double, double, const double*, double*, double*,
                                  hipStream_t);
 
 }  // namespace tbd
