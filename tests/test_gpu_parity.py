@@ -147,11 +147,60 @@ def test_steps_mixed():
     assert rel_l2(cg, co) < 1e-5
 
 
-def test_steps_unfused_path(monkeypatch):
-    # the unfused transform path (separate x passes + product kernels) stays
-    # interchangeable with the fused one
-    monkeypatch.setenv("TBD_UNFUSED", "1")
+def test_steps_fused_path(monkeypatch):
+    # the fused transform path (x transforms and products in one kernel) stays
+    # interchangeable with the default one
+    monkeypatch.setenv("TBD_FUSED", "1")
     cfg = small_config()
+    o, g = _run_pair(cfg, 2)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    assert rel_l2(ug, uo) < 1e-9
+    assert rel_l2(cg, co) < 1e-9
+
+
+@pytest.mark.parametrize("shape", [(54, 36, 3), (256, 256, 2), (24, 48, 5)])
+@pytest.mark.parametrize("stockham", ["0", "1"])
+def test_dft_both_kernel_families(shape, stockham, monkeypatch):
+    # two-level register kernels (lengths whose factors divide 48) and the LDS
+    # Stockham kernels (every other 2^a 3^b length, or forced) agree with the oracle
+    monkeypatch.setenv("TBD_FFT_STOCKHAM", stockham)
+    nx, ny, nz = shape
+    u = RNG.standard_normal((2, nz, ny, nx))
+    ref = O.dft_forward(u)
+    assert rel_l2(RT.dft_forward(u), ref) < 1e-13
+    assert rel_l2(RT.dft_forward(u, fp32=True), ref) < 2e-6
+    assert rel_l2(RT.dft_backward(ref, nx, ny), O.dft_backward(ref, nx, ny)) < 1e-13
+    assert rel_l2(RT.dft_backward(ref, nx, ny, fp32=True), O.dft_backward(ref, nx, ny)) < 2e-6
+
+
+def test_steps_stockham_path(monkeypatch):
+    monkeypatch.setenv("TBD_FFT_STOCKHAM", "1")
+    cfg = small_config()
+    o, g = _run_pair(cfg, 2)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    assert rel_l2(ug, uo) < 1e-9
+    assert rel_l2(cg, co) < 1e-9
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_steps_plain_operators(precision, monkeypatch):
+    # D1/D2 applied as full nz x nz operators instead of their even/odd split
+    monkeypatch.setenv("TBD_NO_PARITY", "1")
+    cfg = small_config(precision=precision)
+    o, g = _run_pair(cfg, 2)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    tol = 1e-9 if precision == 0 else 1e-5
+    assert rel_l2(ug, uo) < tol
+    assert rel_l2(cg, co) < tol
+
+
+@pytest.mark.parametrize("nz", [65, 97])
+def test_steps_parity_sizes(nz):
+    # even/odd split on the CGL sizes of the ensemble members
+    cfg = small_config(nx=16, ny=16, nz=nz)
     o, g = _run_pair(cfg, 2)
     uo, po, co = o.get_state()
     ug, pg, cg = g.download()

@@ -105,7 +105,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   TBD_CHECK(d.nyc >= 2 && d.mx >= 1, "Context > grid too small for the 3/2 rule");
   TBD_HIP(hipStreamCreate(&st_));
   fft_.create(d);
-  if (const char* e = std::getenv("TBD_UNFUSED")) fused_ = std::string(e) != "1";
+  if (const char* e = std::getenv("TBD_FUSED")) fused_ = std::string(e) == "1";
   dt_ = olddt_ = cfg.dt;
   const double c1[3] = {0.0 / 1.0, -5.0 / 9.0, -153.0 / 128.0};    // stepper_time_rk3.cc:29-31
   const double c2[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
@@ -141,6 +141,29 @@ Context::Context(const tbd_config& c) : cfg(c) {
     std::vector<float> f(r.begin(), r.end());
     d1f_ = (float*)alloc(sizeof(float) * f.size());
     TBD_HIP(hipMemcpy(d1f_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  }
+  // D1 is centro-antisymmetric and D2 centro-symmetric on the unmapped CGL
+  // grid: apply them through their even/odd split (half the MFMA work)
+  {
+    const std::vector<double> r1 = rowmajor(ch.d1, nz, lda_), r2 = rowmajor(ch.d2, nz, lda_);
+    auto centro = [&](const std::vector<double>& A, double sgn) {
+      double mx = 0, dv = 0;
+      for (int i = 0; i < nz; ++i)
+        for (int j = 0; j < nz; ++j) {
+          mx = std::max(mx, std::fabs(A[(size_t)i * lda_ + j]));
+          dv = std::max(dv, std::fabs(A[(size_t)i * lda_ + j] - sgn * A[(size_t)(nz - 1 - i) * lda_ + nz - 1 - j]));
+        }
+      return dv <= 1e-12 * mx;
+    };
+    const char* np = std::getenv("TBD_NO_PARITY");
+    parity_ = nz % 2 == 1 && !(np && np[0] == '1') && centro(r1, -1.0) && centro(r2, 1.0);
+    if (parity_) {
+      d1p_ = up(pack_parity<double>(r1.data(), nz, lda_, &ldp_));
+      d2p_ = up(pack_parity<double>(r2.data(), nz, lda_, &ldp_));
+      const std::vector<float> f = pack_parity<float>(r1.data(), nz, lda_, &ldp_);
+      d1pf_ = (float*)alloc(sizeof(float) * f.size());
+      TBD_HIP(hipMemcpy(d1pf_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+    }
   }
   for (int k = 0; k < (cfg.nphase > 1 ? 5 : 4); ++k) {
     vinv_[k] = up(rowmajor(fac[k].vinv, nz, lda_));
@@ -313,6 +336,26 @@ void Context::forces(double time, double& f1, double& f2) const {
   }
 }
 
+// wall-normal operators of a contraction, in the arithmetic type T
+template <typename T>
+void Context::set_d1(ZArgs& a) const {
+  const bool f = std::is_same<T, float>::value;
+  if (parity_) {
+    a.op = f ? (const void*)d1pf_ : (const void*)d1p_;
+    a.lda = ldp_;
+    a.parity = -1;
+  } else {
+    a.op = f ? (const void*)d1f_ : (const void*)d1_;
+    a.lda = lda_;
+    a.parity = 0;
+  }
+}
+void Context::set_d2(ZArgs& a) const {
+  a.op = parity_ ? d2p_ : d2_;
+  a.lda = parity_ ? ldp_ : lda_;
+  a.parity = parity_ ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // FlatCarrierPhase::Predict (phase_carrier_flat.cc:272-559)
 // ---------------------------------------------------------------------------
@@ -328,25 +371,27 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   T* gphys = (T*)gphys_;
   T* aphys = (T*)aphys_;
   CT* tmp = (CT*)tmp_;
-  const T* d1op = std::is_same<T, float>::value ? (const T*)d1f_ : (const T*)d1_;
 
   // wall gradients of the mean mode for the log (dudxyz(2) rows 0, N-1 col 0)
   launch(KC_OTHER, 0.0, [&] { wall_mean_rows(d1_, lda_, d.nz, uvw_, S, wallmean_, st_); });
   if (!alternation) {
     // convective form (phase_carrier_flat.cc:784-806)
     ZArgs a;
-    a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+    a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = uvw_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
     launch(kc_z<T>(), 3.0, [&] { zgemm<T, double2, CT>(a, 3, st_); });
     const bool erosion_planes = cfg.nphase > 1 && cfg.scalar_bc == 1;
     if (fused_) {
       // y^-1 of the nine gradients; x^-1, u.grad u_i, x in one kernel; y
       launch(kc_f<T>(), 9.0, [&] { fft_ypass_backward<T, CT>(fft_, grad, S, tmp, 9, 0, d.nz, st_); });
+      // outputs go to a separate x-pencil buffer: the three outputs of a row
+      // tile are written by different workgroups while the inputs are read
+      CT* xout = (CT*)gphys_;
       launch(kc_f<T>(), 0.0, [&] {
-        xfused_conv<T>(fft_, tmp, uvwnl, R, (T)0, (T)0, (T)0, 3, tmp, erosion_planes ? (T*)dudz_bot_ : nullptr,
+        xfused_conv<T>(fft_, tmp, uvwnl, R, (T)0, (T)0, (T)0, 3, xout, erosion_planes ? (T*)dudz_bot_ : nullptr,
                        erosion_planes ? (T*)dvdz_bot_ : nullptr, st_);
       });
-      launch(kc_f<T>(), 3.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, adv, S, 3, 0, d.nz, st_); });
+      launch(kc_f<T>(), 3.0, [&] { fft_ypass_forward<T, CT>(fft_, xout, adv, S, 3, 0, d.nz, st_); });
     } else {
       launch(kc_f<T>(), 9.0, [&] { fft_backward<T, CT>(fft_, grad, S, gphys, R, 9, 0, d.nz, tmp, st_); });
       launch(KC_OTHER, 0.0, [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
@@ -370,7 +415,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     for (int f = 0; f < 3; ++f) {
       ZArgs a;
-      a.mode = Z_GRADTMP; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+      a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
       a.x = prod + rows[f][2] * S; a.e0 = prod + rows[f][0] * S; a.e1 = prod + rows[f][1] * S;
       a.y = adv + f * S; a.kx = kx_; a.ky = ky_;
       launch(kc_z<T>(), 1.0, [&] { zgemm<T, CT, CT>(a, 1, st_); });
@@ -401,7 +446,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   {
     const double redtc3 = -cfg.reynolds / dt_ / c3;
     ZArgs a;
-    a.mode = Z_RHS; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d2_;
+    a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
     a.x = uvw_; a.xs = S; a.y = rhs_; a.ys = S; a.h = huvw_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
     launch(KC_ZGEMM, 3.0, [&] { zgemm<double, double2, double2>(a, 3, st_); });
@@ -451,7 +496,7 @@ void Context::carrier_correct(int stage) {
   // div s, rhs_p = div / (2 dt c3) (:586-589)
   {
     ZArgs a;
-    a.mode = Z_DIV; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1_;
+    a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
     a.x = suvw_ + 2 * S; a.e0 = suvw_; a.e1 = suvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
     a.q = 0.5 / dt_ / c3;
     launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });
@@ -473,7 +518,7 @@ void Context::carrier_correct(int stage) {
   wslot_ = 1 - wslot_;
   {
     ZArgs a;
-    a.mode = Z_CORRECT; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1_;
+    a.mode = Z_CORRECT; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
     a.x = p_; a.e0 = suvw_; a.es = S; a.y = uvw_; a.ys = S; a.kx = kx_; a.ky = ky_;
     a.cf = 2 * dt_ * c3;
     a.wall_bot = wall_[wslot_][0]; a.wall_top = wall_[wslot_][1];
@@ -511,7 +556,6 @@ void Context::scalar_correct(int stage, int alternation) {
   CT* prod = (CT*)prod_;
   CT* advc = (CT*)advc_;
   CT* tmp = (CT*)tmp_;
-  const T* d1op = std::is_same<T, float>::value ? (const T*)d1f_ : (const T*)d1_;
   // advecting velocity = carrier velocity + settling shift on the mean mode
   // (:256-279); its physical form is the carrier's plus constants
   const double ws = cfg.settling_velocity;
@@ -529,12 +573,12 @@ void Context::scalar_correct(int stage, int alternation) {
       launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp, st_); });
     }
     ZArgs a;
-    a.mode = Z_GRADTMP; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+    a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = prod + 2 * S; a.e0 = prod; a.e1 = prod + S; a.y = advc; a.kx = kx_; a.ky = ky_;
     launch(kc_z<T>(), 1.0, [&] { zgemm<T, CT, CT>(a, 1, st_); });
   } else {             // convective form (:584-592)
     ZArgs a;
-    a.mode = Z_GRAD; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d1op;
+    a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = c_; a.xs = S; a.y = grad + 6 * S; a.ys = S; a.kx = kx_; a.ky = ky_;
     launch(kc_z<T>(), 1.0, [&] { zgemm<T, double2, CT>(a, 1, st_); });
     if (fused_) {
@@ -561,7 +605,7 @@ void Context::scalar_correct(int stage, int alternation) {
   {
     const double redtc3 = -cfg.reynolds * cfg.schmidt / dt_ / c3;
     ZArgs a;
-    a.mode = Z_RHS; a.nz = d.nz; a.lda = lda_; a.P = d.P; a.nyc = d.nyc; a.op = d2_;
+    a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
     a.x = c_; a.xs = S; a.y = rhsc_; a.ys = S; a.h = hc_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
     launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });

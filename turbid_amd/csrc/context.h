@@ -9,6 +9,7 @@
 #include "common.h"
 #include "fft.h"
 #include "setup.h"
+#include "zgemm.h"
 
 namespace tbd {
 
@@ -49,6 +50,8 @@ class Context {
   Profiler prof;
 
  private:
+  template <typename T> void set_d1(ZArgs& a) const;
+  void set_d2(ZArgs& a) const;
   template <typename T> void carrier_predict(int stage, double time, int alternation);
   template <typename T> void carrier_correct(int stage);
   template <typename T> void scalar_correct(int stage, int alternation);
@@ -60,7 +63,7 @@ class Context {
   FftPlan fft_;
   int lda_ = 0;
   bool opentop_ = false;
-  bool fused_ = true;       // x transforms fused around the nonlinear products
+  bool fused_ = false;      // x transforms fused around the nonlinear products (TBD_FUSED=1)
   double dt_ = 0, olddt_ = 0;
   double coef_[3][3];
 
@@ -68,6 +71,11 @@ class Context {
   double* d1_ = nullptr;
   double* d2_ = nullptr;
   float* d1f_ = nullptr;
+  bool parity_ = false;     // D1, D2 applied through their even/odd split
+  int ldp_ = 0;             // leading dimension of the split operators
+  double* d1p_ = nullptr;
+  double* d2p_ = nullptr;
+  float* d1pf_ = nullptr;
   double* vinv_[5] = {nullptr};
   double* vnorm_[5] = {nullptr};
   double* lam_[5] = {nullptr};

@@ -30,6 +30,9 @@ struct FftPlan {
   double2* twy64 = nullptr;
   float2* twx32 = nullptr;
   float2* twy32 = nullptr;
+  // two-level register kernels where the length allows; false selects the
+  // LDS Stockham kernels for every length (environment TBD_FFT_STOCKHAM=1)
+  bool two_level = true;
   void create(const Dims& dims);
   void destroy();
 };

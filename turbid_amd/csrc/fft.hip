@@ -9,8 +9,10 @@
 // global loads/stores of the passes.
 #include "fft.h"
 #include "fft_device.h"
+#include "regfft.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 namespace tbd {
@@ -28,6 +30,8 @@ FftLen make_fftlen(int n) {
 
 void FftPlan::create(const Dims& dims) {
   d = dims;
+  const char* stockham = std::getenv("TBD_FFT_STOCKHAM");
+  two_level = !(stockham && stockham[0] == '1');
   lx = make_fftlen(d.nx);
   ly = make_fftlen(d.ny);
   auto mk_tw = [](int n, double2** t64, float2** t32) {
@@ -227,6 +231,315 @@ __global__ __launch_bounds__(NT) void k_ypass_bwd(const SpecT* __restrict__ spec
 }
 
 // ---------------------------------------------------------------------------
+// Two-level register transforms (regfft.h) for lengths N = N1*N2 with both
+// factors dividing 48.  A workgroup of NT threads handles S sequences; every
+// level-A task holds one DFT of size N1 in registers, every level-B task one
+// DFT of size N2, and the data crosses LDS once between them.  Global loads
+// and stores are arranged so that consecutive lanes touch consecutive
+// addresses: y for the x-pencil side, z for the spectral side, x for rows.
+// ---------------------------------------------------------------------------
+template <int N1, int N2, int NT>
+__host__ __device__ constexpr int seqs2() {
+  return (NT / (N1 > N2 ? N1 : N2)) > 0 ? NT / (N1 > N2 ? N1 : N2) : 1;
+}
+template <typename T, int N1, int N2, int NT>
+__host__ __device__ constexpr int lds2_elems(bool with_rows) {
+  return (seqs2<N1, N2, NT>() * xseq<N1, N2>() > (with_rows ? seqs2<N1, N2, NT>() * (N1 * N2 + 1) : 0))
+             ? seqs2<N1, N2, NT>() * xseq<N1, N2>()
+             : seqs2<N1, N2, NT>() * (N1 * N2 + 1);
+}
+
+// y pass, forward: T[d][z][kx][y] -> spectral (z fastest), keep nyc, scale
+template <typename T, typename SpecT, int N1, int N2, int NT>
+__global__ __launch_bounds__(NT) void k_ypf2(const C2<T>* __restrict__ tmp, SpecT* __restrict__ spec,
+                                             long long sstride, const C2<T>* __restrict__ tw, int scol, int mx,
+                                             int nyc, int nplanes, int zoff, T scale) {
+  constexpr int N = N1 * N2;
+  constexpr int S = seqs2<N1, N2, NT>();
+  __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(false)];
+  C2<T>* E = reinterpret_cast<C2<T>*>(raw);
+  const int z0 = blockIdx.x * S, k = blockIdx.y, d = blockIdx.z;
+  const int ns = min(S, nplanes - z0);
+  const int t = threadIdx.x;
+  if (t < S * N2) {
+    const int n2 = t % N2, s = t / N2;
+    if (s < ns) {
+      const C2<T>* src = tmp + (((long long)d * nplanes + z0 + s) * mx + k) * N + n2;
+      C2<T> v[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[N2 * n1];
+      level_a_store<T, N1, N2, -1>(v, E, s, n2, tw);
+    }
+  }
+  __syncthreads();
+  if (t < S * N1) {
+    const int s = t % S, k1 = t / S;
+    if (s < ns) {
+      C2<T> w[N2];
+      level_b_load<T, N1, N2, -1>(w, E, s, k1);
+      const int half = nyc / 2;
+      SpecT* out = spec + (long long)d * sstride + (long long)k * nyc * scol + zoff + z0 + s;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int y = k1 + N1 * k2;
+        const int iy = y < half ? y : (y >= N - half ? y - N + nyc : -1);
+        if (iy >= 0) {
+          SpecT o;
+          o.x = w[k2].x * scale;
+          o.y = w[k2].y * scale;
+          out[(long long)iy * scol] = o;
+        }
+      }
+    }
+  }
+}
+
+// y pass, backward: spectral (z fastest) -> T[d][z][kx][y], zero padded
+template <typename T, typename SpecT, int N1, int N2, int NT>
+__global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, long long sstride,
+                                             C2<T>* __restrict__ tmp, const C2<T>* __restrict__ tw, int scol, int mx,
+                                             int nyc, int nplanes, int zoff) {
+  constexpr int N = N1 * N2;
+  constexpr int S = seqs2<N1, N2, NT>();
+  __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(false)];
+  C2<T>* E = reinterpret_cast<C2<T>*>(raw);
+  const int z0 = blockIdx.x * S, k = blockIdx.y, d = blockIdx.z;
+  const int ns = min(S, nplanes - z0);
+  const int t = threadIdx.x;
+  if (t < S * N2) {
+    const int s = t % S, n2 = t / S;
+    if (s < ns) {
+      const int half = nyc / 2;
+      const SpecT* in = spec + (long long)d * sstride + (long long)k * nyc * scol + zoff + z0 + s;
+      C2<T> v[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) {
+        const int y = N2 * n1 + n2;
+        const int iy = y < half ? y : (y >= N - half ? y - N + nyc : -1);
+        if (iy >= 0) {
+          const SpecT u = in[(long long)iy * scol];
+          v[n1] = mk<T>((T)u.x, (T)u.y);
+        } else {
+          v[n1] = mk<T>((T)0, (T)0);
+        }
+      }
+      level_a_store<T, N1, N2, +1>(v, E, s, n2, tw);
+    }
+  }
+  __syncthreads();
+  if (t < S * N1) {
+    const int k1 = t % N1, s = t / N1;
+    if (s < ns) {
+      C2<T> w[N2];
+      level_b_load<T, N1, N2, +1>(w, E, s, k1);
+      C2<T>* out = tmp + (((long long)d * nplanes + z0 + s) * mx + k) * N + k1;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) out[N1 * k2] = w[k2];
+    }
+  }
+}
+
+// x pass, forward: rows (y0 .. y0+2S) of plane pl -> T[pl][kx][y]
+template <typename T, int N1, int N2, int NT>
+__global__ __launch_bounds__(NT) void k_xpf2(const T* __restrict__ phys, long long pstride, C2<T>* __restrict__ tmp,
+                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes) {
+  constexpr int N = N1 * N2;
+  constexpr int S = seqs2<N1, N2, NT>();
+  __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(true)];
+  C2<T>* E = reinterpret_cast<C2<T>*>(raw);
+  C2<T>* Z = E;   // natural-order spectra after level B (aliases E)
+  const int pl = blockIdx.y;
+  const int d = pl / nplanes, zl = pl - d * nplanes;
+  const int y0 = blockIdx.x * 2 * S;
+  const T* base = phys + (long long)d * pstride + (long long)zl * ny * N;
+  const int t = threadIdx.x;
+  if (t < S * N2) {
+    const int n2 = t % N2, s = t / N2;
+    const int ya = y0 + 2 * s;
+    const T* ra = base + (long long)ya * N + n2;
+    C2<T> v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const T a = (ya < ny) ? ra[N2 * n1] : (T)0;
+      const T b = (ya + 1 < ny) ? ra[N + N2 * n1] : (T)0;
+      v[n1] = mk<T>(a, b);
+    }
+    level_a_store<T, N1, N2, -1>(v, E, s, n2, tw);
+  }
+  __syncthreads();
+  C2<T> w[N2];
+  const int k1 = t % N1, sb = t / N1;
+  if (t < S * N1) level_b_load<T, N1, N2, -1>(w, E, sb, k1);
+  __syncthreads();
+  if (t < S * N1) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) Z[sb * (N + 1) + k1 + N1 * k2] = w[k2];
+  }
+  __syncthreads();
+  const int nr = 2 * S;
+  C2<T>* out = tmp + (long long)pl * mx * ny;
+  for (int idx = t; idx < mx * nr; idx += NT) {
+    const int kk = idx / nr, r = idx - kk * nr;
+    const int y = y0 + r;
+    if (y >= ny) continue;
+    const int s = r >> 1;
+    const C2<T> zk = Z[s * (N + 1) + kk];
+    const C2<T> zn = Z[s * (N + 1) + (kk == 0 ? 0 : N - kk)];
+    C2<T> val;
+    if ((r & 1) == 0) val = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+    else val = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
+    out[(long long)kk * ny + y] = val;
+  }
+}
+
+// x pass, backward: T[pl][kx][y] (kx < mx) -> rows, halfcomplex c2r
+template <typename T, int N1, int N2, int NT>
+__global__ __launch_bounds__(NT) void k_xpb2(const C2<T>* __restrict__ tmp, T* __restrict__ phys, long long pstride,
+                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes) {
+  constexpr int N = N1 * N2;
+  constexpr int S = seqs2<N1, N2, NT>();
+  __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(true)];
+  C2<T>* E = reinterpret_cast<C2<T>*>(raw);
+  C2<T>* Z = E;   // packed two-row spectra (aliases E)
+  const int pl = blockIdx.y;
+  const int d = pl / nplanes, zl = pl - d * nplanes;
+  const int y0 = blockIdx.x * 2 * S;
+  const int t = threadIdx.x;
+  const C2<T>* in = tmp + (long long)pl * mx * ny;
+  for (int idx = t; idx < S * N; idx += NT) {
+    const int s = idx % S, kk = idx / S;
+    const int ya = y0 + 2 * s;
+    const bool direct = kk < mx;
+    const int kx = direct ? kk : N - kk;
+    C2<T> z = mk<T>((T)0, (T)0);
+    if (direct || kx < mx) {
+      C2<T> A = (ya < ny) ? in[(long long)kx * ny + ya] : mk<T>((T)0, (T)0);
+      C2<T> B = (ya + 1 < ny) ? in[(long long)kx * ny + ya + 1] : mk<T>((T)0, (T)0);
+      if (kx == 0 || 2 * kx == N) { A.y = (T)0; B.y = (T)0; }
+      z = direct ? mk<T>(A.x - B.y, A.y + B.x) : mk<T>(A.x + B.y, B.x - A.y);
+    }
+    Z[s * (N + 1) + kk] = z;
+  }
+  __syncthreads();
+  C2<T> v[N1];
+  const int n2 = t % N2, sa = t / N2;
+  if (t < S * N2) {
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = Z[sa * (N + 1) + N2 * n1 + n2];
+  }
+  __syncthreads();
+  if (t < S * N2) level_a_store<T, N1, N2, +1>(v, E, sa, n2, tw);
+  __syncthreads();
+  if (t < S * N1) {
+    const int k1 = t % N1, s = t / N1;
+    C2<T> w[N2];
+    level_b_load<T, N1, N2, +1>(w, E, s, k1);
+    const int ya = y0 + 2 * s;
+    T* ra = phys + (long long)d * pstride + ((long long)zl * ny + ya) * N + k1;
+    if (ya < ny) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) ra[N1 * k2] = w[k2].x;
+    }
+    if (ya + 1 < ny) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) ra[N + N1 * k2] = w[k2].y;
+    }
+  }
+}
+
+template <typename T> struct Nt2;
+template <> struct Nt2<double> { static constexpr int v = 128; };
+template <> struct Nt2<float> { static constexpr int v = 256; };
+
+// Supported two-level lengths; F(N1, N2) is expanded per length.
+#define TBD_FOR_LENGTHS(F)                                                                              \
+  F(384, 24, 16) F(256, 16, 16) F(192, 16, 12) F(128, 16, 8) F(96, 12, 8) F(64, 8, 8) F(48, 8, 6)    \
+  F(32, 8, 4) F(24, 6, 4) F(16, 4, 4) F(12, 4, 3) F(8, 4, 2) F(6, 3, 2) F(4, 4, 1) F(3, 3, 1) F(2, 2, 1)
+
+template <typename T, typename SpecT>
+static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
+                 hipStream_t st, int scol) {
+  constexpr int NT = Nt2<T>::v;
+  const T scale = (T)(1.0 / ((double)p.d.nx * (double)p.d.ny));
+  const int sc = scol > 0 ? scol : p.d.nz;
+  switch (p.d.ny) {
+#define TBD_CASE(NN, A, B)                                                                                  \
+  case NN: {                                                                                                \
+    constexpr int S = seqs2<A, B, NT>();                                                                    \
+    dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);                                                            \
+    hipLaunchKernelGGL((k_ypf2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, tmp, spec, sstride, twy<T>(p), sc, \
+                       p.d.mx, p.d.nyc, nplanes, zoff, scale);                                              \
+    return true;                                                                                            \
+  }
+    TBD_FOR_LENGTHS(TBD_CASE)
+#undef TBD_CASE
+    default:
+      return false;
+  }
+}
+
+template <typename T, typename SpecT>
+static bool ypb2(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
+                 hipStream_t st, int scol) {
+  constexpr int NT = Nt2<T>::v;
+  const int sc = scol > 0 ? scol : p.d.nz;
+  switch (p.d.ny) {
+#define TBD_CASE(NN, A, B)                                                                                  \
+  case NN: {                                                                                                \
+    constexpr int S = seqs2<A, B, NT>();                                                                    \
+    dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);                                                            \
+    hipLaunchKernelGGL((k_ypb2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, spec, sstride, tmp, twy<T>(p), sc, \
+                       p.d.mx, p.d.nyc, nplanes, zoff);                                                     \
+    return true;                                                                                            \
+  }
+    TBD_FOR_LENGTHS(TBD_CASE)
+#undef TBD_CASE
+    default:
+      return false;
+  }
+}
+
+template <typename T>
+static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
+                 hipStream_t st) {
+  constexpr int NT = Nt2<T>::v;
+  switch (p.d.nx) {
+#define TBD_CASE(NN, A, B)                                                                              \
+  case NN: {                                                                                            \
+    constexpr int S = seqs2<A, B, NT>();                                                                \
+    dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);                                                   \
+    hipLaunchKernelGGL((k_xpf2<T, A, B, NT>), grid, dim3(NT), 0, st, phys, pstride, tmp, twx<T>(p), p.d.ny, \
+                       p.d.mx, nplanes);                                                                \
+    return true;                                                                                        \
+  }
+    TBD_FOR_LENGTHS(TBD_CASE)
+#undef TBD_CASE
+    default:
+      return false;
+  }
+}
+
+template <typename T>
+static bool xpb2(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
+                 hipStream_t st) {
+  constexpr int NT = Nt2<T>::v;
+  switch (p.d.nx) {
+#define TBD_CASE(NN, A, B)                                                                              \
+  case NN: {                                                                                            \
+    constexpr int S = seqs2<A, B, NT>();                                                                \
+    dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);                                                   \
+    hipLaunchKernelGGL((k_xpb2<T, A, B, NT>), grid, dim3(NT), 0, st, tmp, phys, pstride, twx<T>(p), p.d.ny, \
+                       p.d.mx, nplanes);                                                                \
+    return true;                                                                                        \
+  }
+    TBD_FOR_LENGTHS(TBD_CASE)
+#undef TBD_CASE
+    default:
+      return false;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launch configuration
 // ---------------------------------------------------------------------------
 static constexpr int kNT = 256;
@@ -254,6 +567,10 @@ static size_t lds_bytes(int n, int nseq) {
 template <typename T>
 void fft_xpass_forward(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
                        hipStream_t st) {
+  if (p.two_level && xpf2<T>(p, phys, pstride, tmp, nv, nplanes, st)) {
+    TBD_HIP(hipGetLastError());
+    return;
+  }
   const int n = p.d.nx;
   const int nseq = std::max(1, std::min(seq_per_block<T>(n), (p.d.ny + 1) / 2));
   dim3 grid(ceil_div(p.d.ny, 2 * nseq), nv * nplanes);
@@ -265,6 +582,10 @@ void fft_xpass_forward(const FftPlan& p, const T* phys, long long pstride, C2<T>
 template <typename T>
 void fft_xpass_backward(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
                         hipStream_t st) {
+  if (p.two_level && xpb2<T>(p, tmp, phys, pstride, nv, nplanes, st)) {
+    TBD_HIP(hipGetLastError());
+    return;
+  }
   const int n = p.d.nx;
   const int nseq = std::max(1, std::min(seq_per_block<T>(n), (p.d.ny + 1) / 2));
   dim3 grid(ceil_div(p.d.ny, 2 * nseq), nv * nplanes);
@@ -276,6 +597,10 @@ void fft_xpass_backward(const FftPlan& p, const C2<T>* tmp, T* phys, long long p
 template <typename T, typename SpecT>
 void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff,
                        int nplanes, hipStream_t st, int scol) {
+  if (p.two_level && ypf2<T, SpecT>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol)) {
+    TBD_HIP(hipGetLastError());
+    return;
+  }
   const int n = p.d.ny;
   const int zt = std::max(1, std::min(seq_per_block<T>(n), nplanes));
   dim3 grid(ceil_div(nplanes, zt), p.d.mx, nv);
@@ -288,6 +613,10 @@ void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long lon
 template <typename T, typename SpecT>
 void fft_ypass_backward(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff,
                         int nplanes, hipStream_t st, int scol) {
+  if (p.two_level && ypb2<T, SpecT>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol)) {
+    TBD_HIP(hipGetLastError());
+    return;
+  }
   const int n = p.d.ny;
   const int zt = std::max(1, std::min(seq_per_block<T>(n), nplanes));
   dim3 grid(ceil_div(nplanes, zt), p.d.mx, nv);

@@ -18,6 +18,8 @@ namespace tbd {
 // adv_i = sum_j (U_j + shift_j) g_{i,j}; nout = 3 (carrier: g has 9 components
 // ordered du/dx, du/dy, du/dz, dv/dx, ..., dw/dz) or 1 (scalar: g has 3).
 // bot0/bot1 (optional, nout = 3): physical bottom plane of du/dz and dv/dz.
+// With nout = 3, adv must not overlap g (the outputs of a row tile are written
+// by different workgroups); with nout = 1 it may be g itself.
 template <typename T>
 void xfused_conv(const FftPlan& p, const C2<T>* g, const T* U, long long R, T su, T sv, T sw, int nout,
                  C2<T>* adv, T* bot0, T* bot1, hipStream_t st);

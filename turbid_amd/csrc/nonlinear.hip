@@ -2,6 +2,7 @@
 #include "nonlinear.h"
 
 #include "fft_device.h"
+#include "regfft.h"
 
 namespace tbd {
 
@@ -211,6 +212,218 @@ __global__ __launch_bounds__(kNT) void k_xfused(XfArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-level register version (lengths N = N1*N2 whose factors divide 48).
+// A workgroup takes S packed row pairs of one z-plane.  The inverse x
+// transform leaves task (k1 = t % N1, s = t / N1) holding the physical points
+// x = k1 + N1*k2, k2 < N2; the forward transform uses the swapped split
+// (N2 first), whose level-A input is exactly that set, so products are
+// formed in registers and only the transforms' own exchanges touch LDS.
+// ---------------------------------------------------------------------------
+template <int N1, int N2, int NT>
+__host__ __device__ constexpr int xf_seqs() {
+  return (NT / (N1 > N2 ? N1 : N2)) > 0 ? NT / (N1 > N2 ? N1 : N2) : 1;
+}
+template <int N1, int N2, int NT>
+__host__ __device__ constexpr int xf_lds() {
+  constexpr int S = xf_seqs<N1, N2, NT>();
+  constexpr int a = S * xseq<N1, N2>(), b = S * xseq<N2, N1>(), c = S * (N1 * N2 + 1);
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+// inverse x transform of one x-pencil plane (kx < mx, y fastest) for the
+// rows y0 .. y0+2S: task (k1, s) receives w[k2] = row pair at x = k1 + N1*k2
+template <typename T, int N1, int N2, int NT>
+__device__ __forceinline__ void xback_regs(const C2<T>* __restrict__ in, int ny, int mx, int y0, C2<T>* L,
+                                           const C2<T>* __restrict__ tw, C2<T>* w) {
+  constexpr int N = N1 * N2;
+  constexpr int S = xf_seqs<N1, N2, NT>();
+  const int t = threadIdx.x;
+  __syncthreads();
+  for (int idx = t; idx < S * N; idx += NT) {
+    const int s = idx % S, kk = idx / S;
+    const int ya = y0 + 2 * s;
+    const bool direct = kk < mx;
+    const int kx = direct ? kk : N - kk;
+    C2<T> z = mk<T>((T)0, (T)0);
+    if (direct || kx < mx) {
+      C2<T> A = (ya < ny) ? in[(long long)kx * ny + ya] : mk<T>((T)0, (T)0);
+      C2<T> B = (ya + 1 < ny) ? in[(long long)kx * ny + ya + 1] : mk<T>((T)0, (T)0);
+      if (kx == 0 || 2 * kx == N) { A.y = (T)0; B.y = (T)0; }
+      z = direct ? mk<T>(A.x - B.y, A.y + B.x) : mk<T>(A.x + B.y, B.x - A.y);
+    }
+    L[s * (N + 1) + kk] = z;
+  }
+  __syncthreads();
+  C2<T> v[N1];
+  const int n2 = t % N2, sa = t / N2;
+  if (t < S * N2) {
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = L[sa * (N + 1) + N2 * n1 + n2];
+  }
+  __syncthreads();
+  if (t < S * N2) level_a_store<T, N1, N2, +1>(v, L, sa, n2, tw);
+  __syncthreads();
+  if (t < S * N1) level_b_load<T, N1, N2, +1>(w, L, t / N1, t % N1);
+}
+
+// forward x transform of the row pairs held as in xback_regs, truncated to
+// kx < mx and unpacked into the x-pencil plane out
+template <typename T, int N1, int N2, int NT>
+__device__ __forceinline__ void xfwd_regs(C2<T>* w, C2<T>* __restrict__ out, int ny, int mx, int y0, C2<T>* L,
+                                          const C2<T>* __restrict__ tw) {
+  constexpr int N = N1 * N2;
+  constexpr int S = xf_seqs<N1, N2, NT>();
+  const int t = threadIdx.x;
+  __syncthreads();
+  if (t < S * N1) level_a_store<T, N2, N1, -1>(w, L, t / N1, t % N1, tw);
+  __syncthreads();
+  C2<T> q[N1];
+  const int k1b = t % N2, sb = t / N2;
+  if (t < S * N2) level_b_load<T, N2, N1, -1>(q, L, sb, k1b);
+  __syncthreads();
+  if (t < S * N2) {
+#pragma unroll
+    for (int k2 = 0; k2 < N1; ++k2) L[sb * (N + 1) + k1b + N2 * k2] = q[k2];
+  }
+  __syncthreads();
+  const int nr = 2 * S;
+  for (int idx = t; idx < mx * nr; idx += NT) {
+    const int kk = idx / nr, r = idx - kk * nr;
+    const int y = y0 + r;
+    if (y >= ny) continue;
+    const int s = r >> 1;
+    const C2<T> zk = L[s * (N + 1) + kk];
+    const C2<T> zn = L[s * (N + 1) + (kk == 0 ? 0 : N - kk)];
+    C2<T> val;
+    if ((r & 1) == 0) val = mk<T>((T)0.5 * (zk.x + zn.x), (T)0.5 * (zk.y - zn.y));
+    else val = mk<T>((T)0.5 * (zk.y + zn.y), (T)0.5 * (zn.x - zk.x));
+    out[(long long)kk * ny + y] = val;
+  }
+}
+
+// packed physical rows (ya, ya+1) of one component at x = k1 + N1*k2
+template <typename T, int N1, int N2>
+__device__ __forceinline__ void load_pair(C2<T>* u, const T* __restrict__ base, int N, int ny, int ya, int k1,
+                                          T shift) {
+  const T* ra = base + (long long)ya * N + k1;
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) {
+    const T a = (ya < ny) ? ra[N1 * k2] + shift : (T)0;
+    const T b = (ya + 1 < ny) ? ra[N + N1 * k2] + shift : (T)0;
+    u[k2] = mk<T>(a, b);
+  }
+}
+
+template <typename T, int N1, int N2, int NT, int MODE>
+__global__ __launch_bounds__(NT) void k_xfused2(XfArgs a) {
+  constexpr int N = N1 * N2;
+  constexpr int S = xf_seqs<N1, N2, NT>();
+  __shared__ __align__(16) char raw[sizeof(C2<T>) * xf_lds<N1, N2, NT>()];
+  C2<T>* L = reinterpret_cast<C2<T>*>(raw);
+  const int z = blockIdx.y;
+  const int y0 = blockIdx.x * 2 * S;
+  const long long plane = (long long)a.mx * a.ny;
+  const long long pstride = (long long)a.nz * plane;
+  const C2<T>* gin = (const C2<T>*)a.gin;
+  C2<T>* gout = (C2<T>*)a.gout;
+  const C2<T>* tw = (const C2<T>*)a.tw;
+  const T* Ub = (const T*)a.U + (long long)z * a.ny * N;
+  const T shifts[3] = {(T)a.su, (T)a.sv, (T)a.sw};
+  const int t = threadIdx.x;
+  const bool act = t < S * N1;
+  const int k1 = t % N1, s = t / N1;
+  const int ya = y0 + 2 * s;
+  if (MODE == 0) {
+    // adv_i = sum_j (U_j + shift_j) g_ij for the output i = blockIdx.z
+    const int i = blockIdx.z;
+    C2<T> acc[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) acc[k2] = mk<T>((T)0, (T)0);
+    for (int j = 0; j < 3; ++j) {
+      const int gcomp = (a.nout == 3) ? 3 * i + j : j;
+      C2<T> w[N2];
+      xback_regs<T, N1, N2, NT>(gin + gcomp * pstride + z * plane, a.ny, a.mx, y0, L, tw, w);
+      if (act) {
+        if (a.bot0 && z == 0 && j == 2 && i < 2) {   // physical bottom plane of du/dz, dv/dz
+          T* bot = (T*)(i == 0 ? a.bot0 : a.bot1) + (long long)ya * N + k1;
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) {
+            if (ya < a.ny) bot[N1 * k2] = w[k2].x;
+            if (ya + 1 < a.ny) bot[N + N1 * k2] = w[k2].y;
+          }
+        }
+        C2<T> u[N2];
+        load_pair<T, N1, N2>(u, Ub + j * a.R, N, a.ny, ya, k1, shifts[j]);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+          acc[k2].x = acc[k2].x + u[k2].x * w[k2].x;
+          acc[k2].y = acc[k2].y + u[k2].y * w[k2].y;
+        }
+      }
+    }
+    xfwd_regs<T, N1, N2, NT>(acc, gout + i * pstride + z * plane, a.ny, a.mx, y0, L, tw);
+  } else if (MODE == 1) {
+    // uu, uv, uw, vv, vw, ww
+    C2<T> u[3][N2];
+    if (act)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) load_pair<T, N1, N2>(u[j], Ub + j * a.R, N, a.ny, ya, k1, (T)0);
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2};
+    constexpr int pb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int o = 0; o < 6; ++o) {
+      C2<T> pr[N2];
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const C2<T> ua = u[pa[o]][k2], ub = u[pb[o]][k2];
+        pr[k2] = mk<T>(ua.x * ub.x, ua.y * ub.y);
+      }
+      xfwd_regs<T, N1, N2, NT>(pr, gout + o * pstride + z * plane, a.ny, a.mx, y0, L, tw);
+    }
+  } else {
+    // c (U_j + shift_j), j = 0..2
+    C2<T> cv[N2];
+    xback_regs<T, N1, N2, NT>(gin + z * plane, a.ny, a.mx, y0, L, tw, cv);
+    for (int j = 0; j < 3; ++j) {
+      C2<T> pr[N2];
+      if (act) {
+        C2<T> u[N2];
+        load_pair<T, N1, N2>(u, Ub + j * a.R, N, a.ny, ya, k1, shifts[j]);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) pr[k2] = mk<T>(cv[k2].x * u[k2].x, cv[k2].y * u[k2].y);
+      }
+      xfwd_regs<T, N1, N2, NT>(pr, gout + j * pstride + z * plane, a.ny, a.mx, y0, L, tw);
+    }
+  }
+}
+
+template <typename T> struct XfNt;
+template <> struct XfNt<double> { static constexpr int v = 128; };
+template <> struct XfNt<float> { static constexpr int v = 256; };
+
+#define TBD_XF_LENGTHS(F)                                                                               \
+  F(384, 24, 16) F(256, 16, 16) F(192, 16, 12) F(128, 16, 8) F(96, 12, 8) F(64, 8, 8) F(48, 8, 6)    \
+  F(32, 8, 4) F(24, 6, 4) F(16, 4, 4) F(12, 4, 3) F(8, 4, 2) F(6, 3, 2) F(4, 4, 1) F(3, 3, 1) F(2, 2, 1)
+
+template <typename T, int MODE>
+static bool launch_xf2(const FftPlan& p, const XfArgs& a, hipStream_t st) {
+  constexpr int NT = XfNt<T>::v;
+  switch (p.d.nx) {
+#define TBD_CASE(NN, A, B)                                                                    \
+  case NN: {                                                                                  \
+    constexpr int S = xf_seqs<A, B, NT>();                                                    \
+    dim3 grid(ceil_div(p.d.ny, 2 * S), p.d.nz, MODE == 0 ? a.nout : 1);                       \
+    hipLaunchKernelGGL((k_xfused2<T, A, B, NT, MODE>), grid, dim3(NT), 0, st, a);             \
+    return true;                                                                              \
+  }
+    TBD_XF_LENGTHS(TBD_CASE)
+#undef TBD_CASE
+    default:
+      return false;
+  }
+}
+
 template <typename T> static const void* twx_of(const FftPlan& p);
 template <> const void* twx_of<double>(const FftPlan& p) { return p.twx64; }
 template <> const void* twx_of<float>(const FftPlan& p) { return p.twx32; }
@@ -235,6 +448,10 @@ static XfArgs make_args(const FftPlan& p, long long R) {
 
 template <typename T, int MODE>
 static void launch_xf(const FftPlan& p, const XfArgs& a, hipStream_t st) {
+  if (p.two_level && launch_xf2<T, MODE>(p, a, st)) {
+    TBD_HIP(hipGetLastError());
+    return;
+  }
   const int n = p.d.nx;
   const size_t lds = sizeof(C2<T>) * ((size_t)a.nseq * lds_ld(n) + n);
   dim3 grid(ceil_div(p.d.ny, 2 * a.nseq), p.d.nz);

@@ -6,6 +6,8 @@
 // (kx,ky)).
 #pragma once
 
+#include <vector>
+
 #include "common.h"
 
 namespace tbd {
@@ -23,6 +25,10 @@ struct ZArgs {
   int mode = Z_PLAIN;
   int nz = 0;            // rows of M and of each column
   int lda = 0;           // leading dimension of the row-major operator
+  // 0: op is the nz x nz operator.  +1 / -1: op is a centro-symmetric /
+  // centro-antisymmetric operator A in its even/odd split (pack_parity),
+  // applied as y_i = s_i + d_i, y_{nz-1-i} = +-(s_i - d_i), i <= M = (nz-1)/2
+  int parity = 0;
   int P = 0;             // columns per field
   int nyc = 0;           // spectral y size (column -> (ix, iy))
   const void* op = nullptr;       // operator (T*), shared by all fields of the launch
@@ -40,6 +46,14 @@ struct ZArgs {
   double2* wall2_bot = nullptr; double2* wall2_top = nullptr;
   int open_top = 0;
 };
+
+// Even/odd split of a centro-(anti)symmetric nz x nz operator A (nz = 2M+1,
+// row-major with leading dimension lda): rows 0..M hold P_ij = A_ij +
+// A_i,nz-1-j (j < M) and P_iM = A_iM; rows M+1..2M+1 hold Q_ij = A_ij -
+// A_i,nz-1-j (j < M).  Leading dimension of the result: ldp = ceil4(M+1).
+// With e_j, o_j the even and odd halves of a column, s = P e and d = Q o.
+template <typename T>
+std::vector<T> pack_parity(const double* A, int nz, int lda, int* ldp);
 
 // T: arithmetic/operator type; TX: input complex type; TY: output complex type.
 // nfield fields are processed in one launch (blockIdx.y).

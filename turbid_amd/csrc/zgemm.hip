@@ -59,108 +59,195 @@ static constexpr int kNT = 512;           // threads per workgroup
 static constexpr int kWaves = kNT / 64;
 static constexpr int kMaxU = 2;           // units (and VALU tail outputs) per thread in buffered mode
 
+// A contraction segment: output rows [0, rows) of the operator block stored
+// at op + opofs*lda, contracted with panel entries [kofs, kofs + K) of every
+// column, results emitted at panel index rbase + row.  A plain operator is
+// one segment; an operator applied through its even/odd split is two.
+struct Seg {
+  int rows, K, kofs, opofs, rbase;
+  int MT, RM, KM, npair;   // 16-row tiles, rows covered by them, k covered by full blocks, tile pairs
+};
 struct Panel {
-  int nz, MT, KM, npair, ncg, ncol, ldx;
+  int nz;        // column length in HBM
+  int nseg;
+  Seg sg[2];
+  int ncg, ncol, ldx;
+  int nunit, ntail;
+  int parity;    // 0 plain, +1 centro-symmetric, -1 centro-antisymmetric operator
 };
 
-__host__ __device__ inline int panel_ld(int nz) { return nz | 1; }
+__host__ inline Seg make_seg(int rows, int K, int kofs, int opofs, int rbase) {
+  Seg g;
+  g.rows = rows; g.K = K; g.kofs = kofs; g.opofs = opofs; g.rbase = rbase;
+  g.MT = rows / 16;
+  g.RM = g.MT * 16;
+  g.KM = (K / 16) * 16;
+  g.npair = (g.MT + 1) / 2;
+  return g;
+}
 
-__host__ inline Panel make_panel(int nz, size_t elem, size_t lds_limit) {
+// parity != 0 needs nz odd: the operator is stored as the (M+1) x (M+1)
+// even block followed by the (M+1) x M odd block (M = (nz-1)/2)
+__host__ inline Panel make_panel(int nz, size_t elem, size_t lds_limit, int parity = 0) {
   Panel pn;
   pn.nz = nz;
-  pn.MT = nz / 16;
-  pn.KM = pn.MT * 16;
-  pn.npair = (pn.MT + 1) / 2;
-  pn.ldx = panel_ld(nz);
-  int ncg = pn.npair > 0 ? (kWaves + pn.npair - 1) / pn.npair : 4;
+  pn.parity = parity;
+  if (parity == 0) {
+    pn.nseg = 1;
+    pn.sg[0] = make_seg(nz, nz, 0, 0, 0);
+    pn.sg[1] = pn.sg[0];
+    pn.ldx = nz | 1;
+  } else {
+    const int M = (nz - 1) / 2;
+    pn.nseg = 2;
+    pn.sg[0] = make_seg(M + 1, M + 1, 0, 0, 0);
+    pn.sg[1] = make_seg(M + 1, M, M + 1, M + 1, M + 1);
+    pn.ldx = (nz + 1) | 1;    // indices 0 .. 2M+1
+  }
+  int pairs = 0;
+  for (int i = 0; i < pn.nseg; ++i) pairs += pn.sg[i].npair;
+  int ncg = pairs > 0 ? (kWaves + pairs - 1) / pairs : 4;
   if (ncg > 4) ncg = 4;
   while (ncg > 1 && (size_t)16 * ncg * pn.ldx * elem > lds_limit) --ncg;
   pn.ncg = ncg;
   pn.ncol = 16 * ncg;
+  pn.nunit = pairs * ncg;
+  pn.ntail = 0;
+  for (int i = 0; i < pn.nseg; ++i) pn.ntail += (pn.sg[i].rows - pn.sg[i].RM) * pn.ncol;
   return pn;
 }
 
-// One unit: rows mt0*16 .. mt0*16+31 (second tile if TWO) x the 16 panel
-// columns of group cg.  out[t][r] = row mt0*16 + 16t + AccT::row(g, r), column
-// cg*16 + lane%16.
+// One unit of segment sg: rows mt0*16 .. mt0*16+31 (second tile if TWO) x the
+// 16 panel columns of group cg.  out[t][r] = segment row mt0*16 + 16t +
+// AccT::row(g, r), column cg*16 + lane%16.
 template <typename T, bool TWO>
-__device__ __forceinline__ void unit_mfma(const T* __restrict__ op, int lda, const Panel& pn,
+__device__ __forceinline__ void unit_mfma(const T* __restrict__ op, int lda, const Seg& sg, int ldx,
                                           const C2<T>* __restrict__ xs, int mt0, int cg, C2<T> (&out)[2][4]) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int cl = lane & 15;
   typedef typename AccT<T>::type Acc;
   Acc r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
-  const T* A0 = op + (long long)(mt0 * 16 + cl) * lda + 4 * g;
+  const T* ob = op + (long long)sg.opofs * lda;
+  const T* A0 = ob + (long long)(mt0 * 16 + cl) * lda + 4 * g;
   const T* A1 = A0 + 16LL * lda;
-  const C2<T>* B = xs + (cg * 16 + cl) * pn.ldx + 4 * g;
-  V4<T> a0 = ld4(A0), a1 = a0;
-  if (TWO) a1 = ld4(A1);
-  for (int kb = 0; kb < pn.KM; kb += 16) {
+  const C2<T>* xc = xs + (cg * 16 + cl) * ldx + sg.kofs;
+  const C2<T>* B = xc + 4 * g;
+  // 16 k of MFMAs on operand set (p, q); B read from the panel
+  auto block = [&](const V4<T>& p, const V4<T>& q, int kb) {
     const C2<T> b0 = B[kb], b1 = B[kb + 1], b2 = B[kb + 2], b3 = B[kb + 3];
-    V4<T> n0 = a0, n1 = a1;
-    if (kb + 16 < pn.KM) {
-      n0 = ld4(A0 + kb + 16);
-      if (TWO) n1 = ld4(A1 + kb + 16);
+    r0 = mfma16(p.x, b0.x, r0);
+    i0 = mfma16(p.x, b0.y, i0);
+    if (TWO) { r1 = mfma16(q.x, b0.x, r1); i1 = mfma16(q.x, b0.y, i1); }
+    r0 = mfma16(p.y, b1.x, r0);
+    i0 = mfma16(p.y, b1.y, i0);
+    if (TWO) { r1 = mfma16(q.y, b1.x, r1); i1 = mfma16(q.y, b1.y, i1); }
+    r0 = mfma16(p.z, b2.x, r0);
+    i0 = mfma16(p.z, b2.y, i0);
+    if (TWO) { r1 = mfma16(q.z, b2.x, r1); i1 = mfma16(q.z, b2.y, i1); }
+    r0 = mfma16(p.w, b3.x, r0);
+    i0 = mfma16(p.w, b3.y, i0);
+    if (TWO) { r1 = mfma16(q.w, b3.x, r1); i1 = mfma16(q.w, b3.y, i1); }
+  };
+  // two operand register sets used alternately: the loads of one set are in
+  // flight while the other set's MFMAs issue (no register copies)
+  if (sg.KM > 0) {
+    V4<T> pa = ld4(A0), qa = pa, pb = pa, qb = pa;
+    if (TWO) qa = ld4(A1);
+    for (int kb = 0; kb < sg.KM; kb += 32) {
+      const bool more = kb + 16 < sg.KM;
+      if (more) {
+        pb = ld4(A0 + kb + 16);
+        if (TWO) qb = ld4(A1 + kb + 16);
+      }
+      block(pa, qa, kb);
+      if (!more) break;
+      if (kb + 32 < sg.KM) {
+        pa = ld4(A0 + kb + 32);
+        if (TWO) qa = ld4(A1 + kb + 32);
+      }
+      block(pb, qb, kb + 16);
     }
-    r0 = mfma16(a0.x, b0.x, r0);
-    i0 = mfma16(a0.x, b0.y, i0);
-    if (TWO) { r1 = mfma16(a1.x, b0.x, r1); i1 = mfma16(a1.x, b0.y, i1); }
-    r0 = mfma16(a0.y, b1.x, r0);
-    i0 = mfma16(a0.y, b1.y, i0);
-    if (TWO) { r1 = mfma16(a1.y, b1.x, r1); i1 = mfma16(a1.y, b1.y, i1); }
-    r0 = mfma16(a0.z, b2.x, r0);
-    i0 = mfma16(a0.z, b2.y, i0);
-    if (TWO) { r1 = mfma16(a1.z, b2.x, r1); i1 = mfma16(a1.z, b2.y, i1); }
-    r0 = mfma16(a0.w, b3.x, r0);
-    i0 = mfma16(a0.w, b3.y, i0);
-    if (TWO) { r1 = mfma16(a1.w, b3.x, r1); i1 = mfma16(a1.w, b3.y, i1); }
-    a0 = n0;
-    a1 = n1;
   }
-  const C2<T>* xc = xs + (cg * 16 + cl) * pn.ldx;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = mt0 * 16 + AccT<T>::row(g, r);
-    T vr = r0[r], vi = i0[r];
-    T wr = r1[r], wi = i1[r];
-    const T* o0 = op + (long long)row * lda;
-    const T* o1 = o0 + 16LL * lda;
-    for (int k = pn.KM; k < pn.nz; ++k) {
-      const C2<T> xv = xc[k];
-      const T m0 = o0[k];
-      vr += m0 * xv.x;
-      vi += m0 * xv.y;
+    out[0][r] = mk<T>(r0[r], i0[r]);
+    out[1][r] = mk<T>(r1[r], i1[r]);
+  }
+  // k beyond the last full block (one k for the CGL sizes)
+  for (int k = sg.KM; k < sg.K; ++k) {
+    const C2<T> xv = xc[k];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = mt0 * 16 + AccT<T>::row(g, r);
+      const T m0 = ob[(long long)row * lda + k];
+      out[0][r].x += m0 * xv.x;
+      out[0][r].y += m0 * xv.y;
       if (TWO) {
-        const T m1 = o1[k];
-        wr += m1 * xv.x;
-        wi += m1 * xv.y;
+        const T m1 = ob[(long long)(row + 16) * lda + k];
+        out[1][r].x += m1 * xv.x;
+        out[1][r].y += m1 * xv.y;
       }
     }
-    out[0][r] = mk<T>(vr, vi);
-    out[1][r] = mk<T>(wr, wi);
   }
 }
 
+// Rows beyond the last full tile of each segment on the vector ALUs: each
+// output is a dot product of length K split over a group of G lanes
+// (coalesced operand reads) and reduced with cross-lane shuffles.
+__host__ __device__ inline int tail_group(int ntail) {
+  int g = 64;
+  while (g > 1 && g * ntail > kNT) g >>= 1;
+  return g;
+}
+// tail output idx -> (segment, row within the segment, column)
+__device__ __forceinline__ void tail_index(const Panel& pn, int idx, int& si, int& row, int& c) {
+  const int n0 = (pn.sg[0].rows - pn.sg[0].RM) * pn.ncol;
+  si = idx < n0 ? 0 : 1;
+  const int li = si == 0 ? idx : idx - n0;
+  const int rr = li / pn.ncol;
+  c = li - rr * pn.ncol;
+  row = pn.sg[si].RM + rr;
+}
 template <typename T>
-__device__ __forceinline__ C2<T> tail_row(const T* __restrict__ op, int lda, const Panel& pn,
-                                          const C2<T>* __restrict__ xs, int row, int c) {
-  const T* orow = op + (long long)row * lda;
-  const C2<T>* xc = xs + c * pn.ldx;
+__device__ __forceinline__ C2<T> tail_dot(const T* __restrict__ op, int lda, const Panel& pn,
+                                          const C2<T>* __restrict__ xs, int idx, int G, int lig, bool active) {
   T vr = 0, vi = 0;
-  for (int k = 0; k < pn.nz; ++k) {
-    const T m = orow[k];
-    const C2<T> xv = xc[k];
-    vr += m * xv.x;
-    vi += m * xv.y;
+  if (active) {
+    int si, row, c;
+    tail_index(pn, idx, si, row, c);
+    const Seg& sg = pn.sg[si];
+    const T* orow = op + (long long)(sg.opofs + row) * lda;
+    const C2<T>* xc = xs + c * pn.ldx + sg.kofs;
+    for (int k = lig; k < sg.K; k += G) {
+      const T m = orow[k];
+      const C2<T> xv = xc[k];
+      vr += m * xv.x;
+      vi += m * xv.y;
+    }
+  }
+  for (int o = G >> 1; o > 0; o >>= 1) {
+    vr += __shfl_xor(vr, o, G);
+    vi += __shfl_xor(vi, o, G);
   }
   return mk<T>(vr, vi);
 }
 
-// Y(row, c) = sum_k op[row][k] X(k, c) for every row and panel column c;
-// emit(row, c, value).  BUFFERED > 0: every value is emitted after a barrier
-// that follows the last panel read (emit may overwrite the panel); each wave
-// holds at most BUFFERED units and each thread at most BUFFERED tail values.
+// unit u -> (segment, first tile, column group)
+__device__ __forceinline__ void unit_index(const Panel& pn, int u, int& si, int& mt0, int& cg) {
+  const int n0 = pn.sg[0].npair * pn.ncg;
+  si = u < n0 ? 0 : 1;
+  const int lu = si == 0 ? u : u - n0;
+  const int p = lu / pn.ncg;
+  cg = lu - p * pn.ncg;
+  mt0 = 2 * p;
+}
+
+// Y(row, c) = sum_k op[row][k] X(k, c) for every segment row and panel column
+// c; emit(panel index, c, value).  BUFFERED > 0: every value is emitted after
+// a barrier that follows the last panel read (emit may overwrite the panel);
+// each wave holds at most BUFFERED units and each thread at most BUFFERED
+// tail values.
 template <typename T, int BUFFERED, class Emit>
 __device__ __forceinline__ void gemm_panel(const T* __restrict__ op, int lda, const Panel& pn, const C2<T>* xs,
                                            Emit emit) {
@@ -168,25 +255,42 @@ __device__ __forceinline__ void gemm_panel(const T* __restrict__ op, int lda, co
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int cl = lane & 15;
-  const int nunit = pn.npair * pn.ncg;
-  const int ntail = (pn.nz - pn.KM) * pn.ncol;
+  const int nunit = pn.nunit;
+  const int ntail = pn.ntail;
+  const int G = tail_group(ntail);
+  const int per = kNT / G;                       // tail outputs per pass
+  const int lig = threadIdx.x & (G - 1);
+  auto emit_unit = [&](int si, int mt0, int cg, const C2<T> (&out)[2][4]) {
+    const Seg& sg = pn.sg[si];
+    const bool two = mt0 + 1 < sg.MT;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      emit(sg.rbase + mt0 * 16 + AccT<T>::row(g, r), cg * 16 + cl, out[0][r]);
+      if (two) emit(sg.rbase + mt0 * 16 + 16 + AccT<T>::row(g, r), cg * 16 + cl, out[1][r]);
+    }
+  };
+  auto run_unit = [&](int si, int mt0, int cg, C2<T> (&out)[2][4]) {
+    const Seg& sg = pn.sg[si];
+    if (mt0 + 1 < sg.MT) unit_mfma<T, true>(op, lda, sg, pn.ldx, xs, mt0, cg, out);
+    else unit_mfma<T, false>(op, lda, sg, pn.ldx, xs, mt0, cg, out);
+  };
+  auto emit_tail = [&](int idx, C2<T> v) {
+    int si, row, c;
+    tail_index(pn, idx, si, row, c);
+    emit(pn.sg[si].rbase + row, c, v);
+  };
   if constexpr (BUFFERED == 0) {
     for (int u = wave; u < nunit; u += kWaves) {
-      const int p = u / pn.ncg, cg = u - p * pn.ncg;
-      const int mt0 = 2 * p;
+      int si, mt0, cg;
+      unit_index(pn, u, si, mt0, cg);
       C2<T> out[2][4];
-      const bool two = mt0 + 1 < pn.MT;
-      if (two) unit_mfma<T, true>(op, lda, pn, xs, mt0, cg, out);
-      else unit_mfma<T, false>(op, lda, pn, xs, mt0, cg, out);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        emit(mt0 * 16 + AccT<T>::row(g, r), cg * 16 + cl, out[0][r]);
-        if (two) emit(mt0 * 16 + 16 + AccT<T>::row(g, r), cg * 16 + cl, out[1][r]);
-      }
+      run_unit(si, mt0, cg, out);
+      emit_unit(si, mt0, cg, out);
     }
-    for (int idx = threadIdx.x; idx < ntail; idx += kNT) {
-      const int rr = idx / pn.ncol, c = idx - rr * pn.ncol;
-      emit(pn.KM + rr, c, tail_row<T>(op, lda, pn, xs, pn.KM + rr, c));
+    for (int base = 0; base < ntail; base += per) {
+      const int idx = base + threadIdx.x / G;
+      const C2<T> v = tail_dot<T>(op, lda, pn, xs, idx, G, lig, idx < ntail);
+      if (lig == 0 && idx < ntail) emit_tail(idx, v);
     }
   } else {
     C2<T> buf[BUFFERED][2][4];
@@ -195,50 +299,39 @@ __device__ __forceinline__ void gemm_panel(const T* __restrict__ op, int lda, co
     for (int j = 0; j < BUFFERED; ++j) {
       const int u = wave + j * kWaves;
       if (u < nunit) {
-        const int p = u / pn.ncg, cg = u - p * pn.ncg;
-        const int mt0 = 2 * p;
-        if (mt0 + 1 < pn.MT) unit_mfma<T, true>(op, lda, pn, xs, mt0, cg, buf[j]);
-        else unit_mfma<T, false>(op, lda, pn, xs, mt0, cg, buf[j]);
+        int si, mt0, cg;
+        unit_index(pn, u, si, mt0, cg);
+        run_unit(si, mt0, cg, buf[j]);
       }
     }
 #pragma unroll
     for (int t = 0; t < BUFFERED; ++t) {
-      const int idx = threadIdx.x + t * kNT;
-      if (idx < ntail) {
-        const int rr = idx / pn.ncol, c = idx - rr * pn.ncol;
-        tl[t] = tail_row<T>(op, lda, pn, xs, pn.KM + rr, c);
-      }
+      const int idx = t * per + threadIdx.x / G;
+      if (t * per < ntail) tl[t] = tail_dot<T>(op, lda, pn, xs, idx, G, lig, idx < ntail);
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < BUFFERED; ++j) {
       const int u = wave + j * kWaves;
       if (u < nunit) {
-        const int p = u / pn.ncg, cg = u - p * pn.ncg;
-        const int mt0 = 2 * p;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          emit(mt0 * 16 + AccT<T>::row(g, r), cg * 16 + cl, buf[j][0][r]);
-          if (mt0 + 1 < pn.MT) emit(mt0 * 16 + 16 + AccT<T>::row(g, r), cg * 16 + cl, buf[j][1][r]);
-        }
+        int si, mt0, cg;
+        unit_index(pn, u, si, mt0, cg);
+        emit_unit(si, mt0, cg, buf[j]);
       }
     }
 #pragma unroll
     for (int t = 0; t < BUFFERED; ++t) {
-      const int idx = threadIdx.x + t * kNT;
-      if (idx < ntail) {
-        const int rr = idx / pn.ncol, c = idx - rr * pn.ncol;
-        emit(pn.KM + rr, c, tl[t]);
-      }
+      const int idx = t * per + threadIdx.x / G;
+      if (lig == 0 && idx < ntail) emit_tail(idx, tl[t]);
     }
   }
 }
 
 // units per wave / tail values per thread that the buffered mode must hold
 static inline int buffered_depth(const Panel& pn) {
-  const int nunit = pn.npair * pn.ncg;
-  const int u = (nunit + kWaves - 1) / kWaves;
-  const int t = ((pn.nz - pn.KM) * pn.ncol + kNT - 1) / kNT;
+  const int u = (pn.nunit + kWaves - 1) / kWaves;
+  const int per = kNT / tail_group(pn.ntail);
+  const int t = (pn.ntail + per - 1) / per;
   return u > t ? u : t;
 }
 
@@ -321,8 +414,11 @@ __device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, 
 
 static constexpr size_t kLdsLimit = 80 * 1024;   // two workgroups per CU
 
-template <typename T, typename TX, typename TY>
-__global__ __launch_bounds__(kNT, 4) void k_zgemm(ZArgs a, Panel pn) {
+// Panel load, buffered contraction written over the panel, then the epilogue
+// as one coalesced pass (row fastest) that re-reads the input column where the
+// mode needs it.
+template <typename T, typename TX, typename TY, int DEPTH>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_zgemm(ZArgs a, Panel pn) {
   extern __shared__ char smem_raw[];
   C2<T>* xs = reinterpret_cast<C2<T>*>(smem_raw);
   const int nz = a.nz;
@@ -331,24 +427,69 @@ __global__ __launch_bounds__(kNT, 4) void k_zgemm(ZArgs a, Panel pn) {
   const long long c0 = (long long)blockIdx.x * pn.ncol;
   const int ncol = (int)min((long long)pn.ncol, a.P - c0);
   const TX* x = (const TX*)a.x + f * a.xs + c0 * nz;
-  for (int idx = threadIdx.x; idx < pn.ncol * nz; idx += kNT) {
-    const int c = idx / nz, k = idx - c * nz;
-    xs[c * ldx + k] = (c < ncol) ? ld<T, TX>(x + (long long)c * nz + k) : mk<T>((T)0, (T)0);
+  if (pn.parity == 0) {
+    for (int idx = threadIdx.x; idx < pn.ncol * nz; idx += kNT) {
+      const int c = idx / nz, k = idx - c * nz;
+      xs[c * ldx + k] = (c < ncol) ? ld<T, TX>(x + (long long)c * nz + k) : mk<T>((T)0, (T)0);
+    }
+  } else {
+    // even part e_j = (x_j + x_{N-1-j})/2 at j <= M (e_M = x_M), odd part
+    // o_j = (x_j - x_{N-1-j})/2 at M+1+j, j < M
+    const int M = (nz - 1) / 2;
+    for (int idx = threadIdx.x; idx < pn.ncol * (M + 1); idx += kNT) {
+      const int c = idx / (M + 1), j = idx - c * (M + 1);
+      C2<T> e = mk<T>((T)0, (T)0), o = e;
+      if (c < ncol) {
+        const C2<T> lo = ld<T, TX>(x + (long long)c * nz + j);
+        const C2<T> hi = ld<T, TX>(x + (long long)c * nz + nz - 1 - j);
+        if (j < M) {
+          e = mk<T>((T)0.5 * (lo.x + hi.x), (T)0.5 * (lo.y + hi.y));
+          o = mk<T>((T)0.5 * (lo.x - hi.x), (T)0.5 * (lo.y - hi.y));
+        } else {
+          e = lo;
+        }
+      }
+      xs[c * ldx + j] = e;
+      if (j < M) xs[c * ldx + M + 1 + j] = o;
+    }
   }
   __syncthreads();
-  gemm_panel<T, 0>((const T*)a.op, a.lda, pn, xs, [&](int row, int c, C2<T> v) {
-    if (c < ncol) zepilogue<T, TX, TY>(a, f, c0 + c, row, v, xs[c * ldx + row]);
-  });
+  gemm_panel<T, DEPTH>((const T*)a.op, a.lda, pn, xs, [&](int row, int c, C2<T> v) { xs[c * ldx + row] = v; });
+  __syncthreads();
+  const bool needx = a.mode == Z_GRAD || a.mode == Z_RHS || a.mode == Z_CORRECT;
+  const int M = (nz - 1) / 2;
+  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
+    const int c = idx / nz, row = idx - c * nz;
+    const C2<T> xv = needx ? ld<T, TX>(x + idx) : mk<T>((T)0, (T)0);
+    C2<T> v;
+    if (pn.parity == 0) {
+      v = xs[c * ldx + row];
+    } else {
+      // y_i = s_i + d_i, y_{N-1-i} = parity (s_i - d_i), i <= M
+      const bool top = row > M;
+      const int i = top ? nz - 1 - row : row;
+      const C2<T> sv = xs[c * ldx + i], dv = xs[c * ldx + M + 1 + i];
+      v = top ? (pn.parity > 0 ? csub<T>(sv, dv) : csub<T>(dv, sv)) : cadd<T>(sv, dv);
+    }
+    zepilogue<T, TX, TY>(a, f, c0 + c, row, v, xv);
+  }
 }
 
 template <typename T, typename TX, typename TY>
 void zgemm(const ZArgs& a, int nfield, hipStream_t st) {
-  TBD_CHECK(a.nz >= 4 && a.lda >= a.nz && a.lda % 4 == 0 && a.P > 0, "zgemm > bad shape");
-  const Panel pn = make_panel(a.nz, sizeof(C2<T>), kLdsLimit);
+  TBD_CHECK(a.nz >= 4 && a.lda % 4 == 0 && a.P > 0, "zgemm > bad shape");
+  TBD_CHECK(a.parity == 0 ? a.lda >= a.nz : (a.nz % 2 == 1 && a.lda >= (a.nz + 1) / 2),
+            "zgemm > operator layout does not match the column length");
+  const Panel pn = make_panel(a.nz, sizeof(C2<T>), kLdsLimit, a.parity);
   const size_t lds = sizeof(C2<T>) * pn.ncol * pn.ldx;
   TBD_CHECK(lds <= 160 * 1024, "zgemm > nz too large for one LDS panel");
+  const int depth = buffered_depth(pn);
+  TBD_CHECK(depth <= kMaxU, "zgemm > nz too large for the register-buffered contraction");
   dim3 grid(ceil_div(a.P, pn.ncol), nfield);
-  hipLaunchKernelGGL((k_zgemm<T, TX, TY>), grid, dim3(kNT), lds, st, a, pn);
+  if (depth <= 1)
+    hipLaunchKernelGGL((k_zgemm<T, TX, TY, 1>), grid, dim3(kNT), lds, st, a, pn);
+  else
+    hipLaunchKernelGGL((k_zgemm<T, TX, TY, 2>), grid, dim3(kNT), lds, st, a, pn);
   TBD_HIP(hipGetLastError());
 }
 
@@ -522,5 +663,25 @@ void wall_mean_rows(const double* op, int lda, int nz, const double2* x, long lo
   hipLaunchKernelGGL(k_wall_mean, dim3(1), dim3(384), 0, st, op, lda, nz, x, S, out);
   TBD_HIP(hipGetLastError());
 }
+
+template <typename T>
+std::vector<T> pack_parity(const double* A, int nz, int lda, int* ldp) {
+  TBD_CHECK(nz % 2 == 1 && nz >= 3, "pack_parity > nz must be odd");
+  const int M = (nz - 1) / 2;
+  const int ld = (M + 1 + 3) / 4 * 4;
+  std::vector<T> out((size_t)(2 * M + 2) * ld, (T)0);
+  for (int i = 0; i <= M; ++i) {
+    const double* ar = A + (size_t)i * lda;
+    for (int j = 0; j < M; ++j) {
+      out[(size_t)i * ld + j] = (T)(ar[j] + ar[nz - 1 - j]);
+      out[(size_t)(M + 1 + i) * ld + j] = (T)(ar[j] - ar[nz - 1 - j]);
+    }
+    out[(size_t)i * ld + M] = (T)ar[M];
+  }
+  *ldp = ld;
+  return out;
+}
+template std::vector<double> pack_parity<double>(const double*, int, int, int*);
+template std::vector<float> pack_parity<float>(const double*, int, int, int*);
 
 }  // namespace tbd
