@@ -12,11 +12,14 @@ inputs resident in HBM; the loop mirrors FlatChannelSolver::Execute
 alternation toggle).  The headline value is the mixed-precision rate; the
 plain FP64 rate is measured in the same run.
 
-Roofline: per kernel class, HIP events around every launch on the library's
-stream during the timed region give the average launch duration; the
-dominant class's algorithmic work per launch (field-transforms x
-(w*R + 2w*S) bytes, or contractions x 4*Nz^2*P flops) over that duration is
-`achieved`.  cpu_baseline: the FP64 CPU restatement (oracle/) timed on the
+Timing: after the warmup, K steps are timed as the library runs them (one
+captured HIP graph per step; `value`, `ms_per_step`), then K further steps
+run with HIP events around every launch on the library's stream (eager
+substages), which gives each kernel's launch count, average duration and
+algorithmic work.  Roofline: the kernel with the largest total time;
+`achieved` = its algorithmic work per launch (bytes read plus written by a
+transform pass or product kernel, or 4*Nz^2*P flops per contraction of one
+component) over its average launch duration.  cpu_baseline: the FP64 CPU restatement (oracle/) timed on the
 host for one step of the same workload.
 """
 import argparse
@@ -36,12 +39,24 @@ from turbid_amd import workloads as W  # noqa: E402
 HBM_PEAK = 8.0e12          # B/s
 MFMA_PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
 
-CLASSES = {  # name: (library class, kind, precision)
-    "fft64": (0, "bytes", 8),
-    "zgemm64": (1, "flops", "fp64"),
-    "helmholtz": (2, "flops", "fp64"),
-    "zgemm32": (4, "flops", "fp32"),
-    "fft32": (5, "bytes", 4),
+# profiler class -> (kernel symbol, kind, peak)
+KERNELS = {
+    "ypass_bwd64": ("k_ypb2<double,...>", "bytes", HBM_PEAK),
+    "xpass_bwd64": ("k_xpb2<double,...>", "bytes", HBM_PEAK),
+    "xpass_fwd64": ("k_xpf2<double,...>", "bytes", HBM_PEAK),
+    "ypass_fwd64": ("k_ypf2<double,...>", "bytes", HBM_PEAK),
+    "ypass_bwd32": ("k_ypb2<float,...>", "bytes", HBM_PEAK),
+    "xpass_bwd32": ("k_xpb2<float,...>", "bytes", HBM_PEAK),
+    "xpass_fwd32": ("k_xpf2<float,...>", "bytes", HBM_PEAK),
+    "ypass_fwd32": ("k_ypf2<float,...>", "bytes", HBM_PEAK),
+    "xfused64": ("k_xfused2<double,...>", "bytes", HBM_PEAK),
+    "xfused32": ("k_xfused2<float,...>", "bytes", HBM_PEAK),
+    "zgemm64": ("k_zgemm<double,...>", "flops", MFMA_PEAK["fp64"]),
+    "zgemm32": ("k_zgemm<float,...>", "flops", MFMA_PEAK["fp32"]),
+    "helmholtz": ("k_helmholtz<1>", "flops", MFMA_PEAK["fp64"]),
+    "products64": ("k_products_*<double>", "bytes", HBM_PEAK),
+    "products32": ("k_products_*<float>", "bytes", HBM_PEAK),
+    "other": ("(boundary rows, H update, filter, diagnostics, copies)", None, None),
 }
 
 
@@ -49,61 +64,49 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run(cfg, state, steps, warmup, profile=True):
+def run(cfg, state, steps, warmup):
+    """Warm up, time `steps` steps as the library runs them (one captured HIP
+    graph per step), then time the same number of steps again with HIP events
+    around every launch (eager substages) for the per-kernel figures."""
     s = RT.Solver(cfg)
     s.upload(*state)
     st = RT.Stepper(cfg["dt"])
     RT.run_steps(s, st, warmup)
     s.sync()
-    if profile:
-        s.profile(True)
-        s.profile_reset()
     t0 = time.perf_counter()
     feat = RT.run_steps(s, st, steps)
     s.sync()
     t1 = time.perf_counter()
-    prof = {}
-    if profile:
-        for name, (k, _, _) in CLASSES.items():
-            prof[name] = s.profile_read(k)
-        prof["other"] = s.profile_read(3)
+    s.profile(True)
+    s.profile_reset()
+    tp0 = time.perf_counter()
+    RT.run_steps(s, st, steps)
+    s.sync()
+    tp1 = time.perf_counter()
+    prof = {name: s.profile_read(name) for name in KERNELS}
+    s.profile(False)
     s.close()
-    return (t1 - t0), feat, prof
+    return (t1 - t0), (tp1 - tp0), feat, prof
 
 
-def roofline(prof, cfg):
-    d = W.dims(cfg["nx"], cfg["ny"], cfg["nz"])
-    nz = cfg["nz"]
-    P = d[1] * d[2]
-    S = nz * P
-    R = cfg["nx"] * cfg["ny"] * nz
-    best = None
-    for name, (_, kind, prec) in CLASSES.items():
-        n, ms, units = prof[name]
-        if n == 0 or ms <= 0:
-            continue
-        if best is None or ms > prof[best][1]:
-            best = name
-    n, ms, units = prof[best]
-    _, kind, prec = CLASSES[best]
+def roofline(prof):
+    best = max((n for n in KERNELS if KERNELS[n][1] and prof[n][0] > 0), key=lambda n: prof[n][1])
+    n, ms, work = prof[best]
+    sym, kind, peak = KERNELS[best]
     avg_s = ms / n / 1e3
+    per_launch = work / n
+    achieved = per_launch / avg_s
     if kind == "bytes":
-        w = prec
-        per_unit = w * R + 2 * w * S
-        work = units / n * per_unit
-        achieved = work / avg_s
-        out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+        out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": peak / 1e9, "unit": "GB/s"}
     else:
-        per_unit = 4.0 * nz * nz * P
-        work = units / n * per_unit
-        achieved = work / avg_s
-        out = {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_PEAK[prec] / 1e12, "unit": "TFLOP/s"}
+        out = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s"}
     out["frac"] = out["achieved"] / out["peak"]
     out["traffic"] = None
-    out["kernel"] = best
+    out["kernel"] = sym
+    out["class"] = best
     out["launches"] = n
     out["avg_launch_ms"] = ms / n
-    out["work_per_launch"] = work
+    out["work_per_launch"] = per_launch
     return out
 
 
@@ -147,12 +150,18 @@ def main():
     for prec in precs:
         cfg = dict(cfg64)
         cfg["precision"] = 1 if prec == "mixed" else 0
-        log("bench: %s warmup %d, timing %d steps" % (prec, args.warmup, args.steps))
-        t, feat, prof = run(cfg, state, args.steps, args.warmup)
+        log("bench: %s warmup %d, timing %d steps (graph), then %d steps with per-launch events"
+            % (prec, args.warmup, args.steps, args.steps))
+        t, tp, feat, prof = run(cfg, state, args.steps, args.warmup)
         results[prec] = {"seconds": t, "steps_per_s": args.steps / t, "ms_per_step": 1e3 * t / args.steps,
-                         "roofline": roofline(prof, cfg), "prof": {k: list(v) for k, v in prof.items()},
+                         "profiled_ms_per_step": 1e3 * tp / args.steps,
+                         "roofline": roofline(prof), "prof": {k: list(v) for k, v in prof.items()},
                          "cfl": float(feat[19] * cfg["dt"]), "max_u": float(feat[0])}
-        log("bench: %s %.2f steps/s (%.3f ms/step)" % (prec, args.steps / t, 1e3 * t / args.steps))
+        log("bench: %s %.2f steps/s (%.3f ms/step; %.3f ms/step with events)"
+            % (prec, args.steps / t, 1e3 * t / args.steps, 1e3 * tp / args.steps))
+        if not np.isfinite(feat[0]):
+            log("bench: %s state is not finite" % prec)
+            sys.exit(3)
     head = "mixed" if "mixed" in results else "fp64"
     line = {
         "metric": "RK3 steps/s @256x256x257 mixed vs FP64; spin-up GPU-s surrogate vs plain",
@@ -177,8 +186,9 @@ def main():
         line["fp64_roofline"] = results["fp64"]["roofline"]
     if "mixed" in results and "fp64" in results:
         line["mixed_vs_fp64"] = results["mixed"]["steps_per_s"] / results["fp64"]["steps_per_s"]
-    line["kernel_ms_per_step"] = {prec: {k: v[1] / args.steps for k, v in r["prof"].items()}
+    line["kernel_ms_per_step"] = {prec: {k: round(v[1] / args.steps, 4) for k, v in r["prof"].items() if v[0]}
                                   for prec, r in results.items()}
+    line["profiled_ms_per_step"] = {prec: r["profiled_ms_per_step"] for prec, r in results.items()}
     if rank == 0 and not args.no_cpu_baseline:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("bench: cpu baseline (oracle, %d threads)" % nthreads)

@@ -91,14 +91,21 @@ int tbd_set_dt(tbd_ctx* ctx, double dt);
 int tbd_features(tbd_ctx* ctx, double* out50);
 int tbd_sync(tbd_ctx* ctx);
 
-/* Kernel timing with HIP events on the context's stream.
- * class: 0 FP64 plane transforms, 1 FP64 z-contractions, 2 Helmholtz solves
- * (FP64), 3 other, 4 FP32 z-contractions, 5 FP32 plane transforms.
- * units: algorithmic work of the timed launches -- field-transforms of one
- * Nx*Ny*Nz component (class 0) or full nz*nz contractions over the P
- * spectral columns of one component (classes 1 and 2). */
+/* Kernel timing with HIP events on the context's stream (while enabled, a
+ * step runs its substages eagerly instead of replaying its graph).  One
+ * class per kernel:
+ *   0..3  y-pass backward, x-pass backward, x-pass forward, y-pass forward (FP64)
+ *   4..7  the same passes in FP32
+ *   8, 9  fused x transforms + nonlinear products (FP64, FP32)
+ *   10, 11 wall-normal contractions (FP64, FP32)
+ *   12    Helmholtz / pressure solves (FP64)
+ *   13, 14 pointwise nonlinear products (FP64, FP32)
+ *   15    everything else
+ * work: algorithmic work of the timed launches -- bytes read plus written
+ * (classes 0..9, 13, 14) or flops, 4*nz^2*P per contraction of one component
+ * (classes 10..12; a Helmholtz solve is two contractions). */
 int tbd_profile_enable(tbd_ctx* ctx, int enable);
-int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* units);
+int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* work);
 int tbd_profile_reset(tbd_ctx* ctx);
 
 /* Operators built by the context (pinned against closed forms in tests):

@@ -206,3 +206,28 @@ def test_steps_parity_sizes(nz):
     ug, pg, cg = g.download()
     assert rel_l2(ug, uo) < 1e-9
     assert rel_l2(cg, co) < 1e-9
+
+
+def _run_solver_only(cfg, nsteps):
+    g = RT.Solver(cfg)
+    uvw, p, c = W.initial_state(cfg)
+    g.upload(uvw, p, c if cfg["nphase"] > 1 else None)
+    st = RT.Stepper(cfg["dt"])
+    RT.run_steps(g, st, nsteps)
+    return g.download(), g.features()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_replay_matches_eager(precision, monkeypatch):
+    # every replay of a captured step (four graphs: alternation x wall-row slot)
+    # reproduces the eager substages bit for bit; features are read between
+    # replays, as the time stepper does
+    cfg = small_config(precision=precision)
+    monkeypatch.setenv("TBD_GRAPH", "0")
+    (ue, pe, ce), fe = _run_solver_only(cfg, 7)
+    monkeypatch.setenv("TBD_GRAPH", "1")
+    (ug, pg, cg), fg = _run_solver_only(cfg, 7)
+    assert np.array_equal(ue, ug)
+    assert np.array_equal(pe, pg)
+    assert np.array_equal(ce, cg)
+    assert np.array_equal(fe, fg)

@@ -96,7 +96,7 @@ int tbd_substage(tbd_ctx* ctx, int stage, double time, int alternation) {
 int tbd_step(tbd_ctx* ctx, double time, int alternation) {
   return guard([&] {
     TBD_CHECK(ctx, "tbd_step > null context");
-    for (int s = 0; s < 3; ++s) ctx->c->substage(s, time, alternation);
+    ctx->c->step(time, alternation);
   });
 }
 
@@ -128,14 +128,14 @@ int tbd_profile_enable(tbd_ctx* ctx, int enable) {
   });
 }
 
-int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* units) {
+int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* work) {
   return guard([&] {
-    TBD_CHECK(ctx && kclass >= 0 && kclass < 6, "tbd_profile_read > bad argument");
+    TBD_CHECK(ctx && kclass >= 0 && kclass < tbd::KC_N, "tbd_profile_read > bad argument");
     ctx->c->sync();
     ctx->c->prof.collect();
     if (launches) *launches = ctx->c->prof.launches[kclass];
     if (total_ms) *total_ms = ctx->c->prof.ms[kclass];
-    if (units) *units = ctx->c->prof.units[kclass];
+    if (work) *work = ctx->c->prof.work[kclass];
   });
 }
 

@@ -47,7 +47,7 @@ void Profiler::collect() {
     TBD_HIP(hipEventElapsedTime(&t, r.a, r.b));
     launches[r.kclass] += 1;
     ms[r.kclass] += t;
-    units[r.kclass] += r.units;
+    work[r.kclass] += r.work;
     pool.push_back(r.a);
     pool.push_back(r.b);
   }
@@ -55,7 +55,7 @@ void Profiler::collect() {
 }
 void Profiler::reset() {
   collect();
-  for (int i = 0; i < 6; ++i) { launches[i] = 0; ms[i] = 0; units[i] = 0; }
+  for (int i = 0; i < KC_N; ++i) { launches[i] = 0; ms[i] = 0; work[i] = 0; }
 }
 Profiler::~Profiler() {
   for (auto& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -63,21 +63,46 @@ Profiler::~Profiler() {
 }
 
 template <class F>
-void Context::launch(int kclass, double units, F&& f) {
+void Context::launch(int kclass, double work, F&& f) {
   if (!prof.on) {
     f();
     return;
   }
   Profiler::Rec r;
-  r.units = units;
+  r.work = work;
   prof.begin(kclass, st_, r);
   f();
   prof.end(r, st_);
 }
 
-enum { KC_FFT = 0, KC_ZGEMM = 1, KC_HELM = 2, KC_OTHER = 3, KC_ZGEMM32 = 4, KC_FFT32 = 5 };
-template <typename T> static int kc_z() { return std::is_same<T, float>::value ? KC_ZGEMM32 : KC_ZGEMM; }
-template <typename T> static int kc_f() { return std::is_same<T, float>::value ? KC_FFT32 : KC_FFT; }
+template <typename T> static constexpr bool is32() { return std::is_same<T, float>::value; }
+
+// algorithmic work: a contraction of one field is 4*nz^2*P flops (real
+// operator times complex columns); each pass moves its input and its output
+static double zflops(const Dims& d) { return 4.0 * d.nz * (double)d.nz * d.P; }
+
+template <typename T, typename SpecT>
+void Context::ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol) {
+  const double b = (double)nv * nplanes * d.mx * (d.nyc * sizeof(SpecT) + (double)d.ny * sizeof(C2<T>));
+  launch(is32<T>() ? KC_YB32 : KC_YB64, b,
+         [&] { fft_ypass_backward<T, SpecT>(fft_, spec, sstride, tmp, nv, zoff, nplanes, st_, scol); });
+}
+template <typename T>
+void Context::xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes) {
+  const double b = (double)nv * nplanes * d.ny * (d.mx * sizeof(C2<T>) + (double)d.nx * sizeof(T));
+  launch(is32<T>() ? KC_XB32 : KC_XB64, b, [&] { fft_xpass_backward<T>(fft_, tmp, phys, pstride, nv, nplanes, st_); });
+}
+template <typename T>
+void Context::xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes) {
+  const double b = (double)nv * nplanes * d.ny * (d.mx * sizeof(C2<T>) + (double)d.nx * sizeof(T));
+  launch(is32<T>() ? KC_XF32 : KC_XF64, b, [&] { fft_xpass_forward<T>(fft_, phys, pstride, tmp, nv, nplanes, st_); });
+}
+template <typename T, typename SpecT>
+void Context::ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol) {
+  const double b = (double)nv * nplanes * d.mx * (d.nyc * sizeof(SpecT) + (double)d.ny * sizeof(C2<T>));
+  launch(is32<T>() ? KC_YF32 : KC_YF64, b,
+         [&] { fft_ypass_forward<T, SpecT>(fft_, tmp, spec, sstride, nv, zoff, nplanes, st_, scol); });
+}
 
 void* Context::alloc(size_t bytes) {
   void* p = nullptr;
@@ -102,6 +127,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   TBD_CHECK(cfg.nphase == 1 || cfg.nphase == 2, "Context > total_phase_number must be 1 or 2");
   TBD_CHECK(cfg.precision == 0 || cfg.precision == 1, "Context > precision must be 0 (FP64) or 1 (mixed)");
   d = Dims::make(cfg.nx, cfg.ny, cfg.nz);
+  xpb_ = (double)d.nz * d.mx * d.ny;
   TBD_CHECK(d.nyc >= 2 && d.mx >= 1, "Context > grid too small for the 3/2 rule");
   TBD_HIP(hipStreamCreate(&st_));
   fft_.create(d);
@@ -236,11 +262,14 @@ Context::Context(const tbd_config& c) : cfg(c) {
   fg_ = (double*)alloc(sizeof(double) * 64);
   wallmean_ = (double*)alloc(sizeof(double) * 8);
   diagpart_ = (double*)alloc(sizeof(double) * 10 * kDiagMaxBlocks);
+  force_ = (double*)alloc(sizeof(double) * 2);
+  if (const char* e = std::getenv("TBD_GRAPH")) use_graphs_ = std::string(e) != "0";
   TBD_HIP(hipStreamSynchronize(st_));
 }
 
 Context::~Context() {
   if (st_) (void)hipStreamSynchronize(st_);
+  drop_graphs();
   for (void* p : allocs_) (void)hipFree(p);
   fft_.destroy();
   if (st_) (void)hipStreamDestroy(st_);
@@ -250,7 +279,61 @@ void Context::sync() { TBD_HIP(hipStreamSynchronize(st_)); }
 
 void Context::set_dt(double dt) {
   TBD_CHECK(dt > 0, "Context::set_dt > dt must be positive");
+  if (dt != dt_) drop_graphs();
   dt_ = dt;
+}
+
+void Context::drop_graphs() {
+  for (auto& g : graphs_) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  graphs_.clear();
+}
+
+// One RK3 step.  Outside profiling the three substages are captured once per
+// (alternation, wall-row slot, dt, previous dt) into a HIP graph and replayed;
+// the only per-step input, the mean-mode force, is written to device memory
+// before the launch.  The host-side state a step advances (wall-row slot,
+// previous dt) is advanced the same way as by the eager substages.
+void Context::step(double time, int alternation) {
+  if (!use_graphs_ || prof.on) {
+    for (int s = 0; s < 3; ++s) substage(s, time, alternation);
+    return;
+  }
+  double f1 = 0, f2 = 0;
+  forces(time, f1, f2);
+  set_pair(force_, f1, f2, st_);
+  const int slot0 = wslot_;
+  for (auto& g : graphs_) {
+    if (g.alternation == alternation && g.slot == slot0 && g.dt == dt_ && g.olddt == olddt_) {
+      TBD_HIP(hipGraphLaunch(g.exec, st_));
+      wslot_ = 1 - wslot_;   // three toggles
+      olddt_ = dt_;
+      return;
+    }
+  }
+  StepGraph g;
+  g.alternation = alternation;
+  g.slot = slot0;
+  g.dt = dt_;
+  g.olddt = olddt_;
+  capturing_ = true;
+  TBD_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int s = 0; s < 3; ++s) substage(s, time, alternation);
+  } catch (...) {
+    capturing_ = false;
+    hipGraph_t broken = nullptr;
+    (void)hipStreamEndCapture(st_, &broken);
+    if (broken) (void)hipGraphDestroy(broken);
+    throw;
+  }
+  capturing_ = false;
+  TBD_HIP(hipStreamEndCapture(st_, &g.graph));
+  TBD_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  graphs_.push_back(g);
+  TBD_HIP(hipGraphLaunch(g.exec, st_));
 }
 
 void Context::upload(const double* uvw, const double* p, const double* c) {
@@ -379,23 +462,23 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     ZArgs a;
     a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = uvw_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
-    launch(kc_z<T>(), 3.0, [&] { zgemm<T, double2, CT>(a, 3, st_); });
+    launch(is32<T>() ? KC_Z32 : KC_Z64, 3 * zflops(d), [&] { zgemm<T, double2, CT>(a, 3, st_); });
     const bool erosion_planes = cfg.nphase > 1 && cfg.scalar_bc == 1;
     if (fused_) {
       // y^-1 of the nine gradients; x^-1, u.grad u_i, x in one kernel; y
-      launch(kc_f<T>(), 9.0, [&] { fft_ypass_backward<T, CT>(fft_, grad, S, tmp, 9, 0, d.nz, st_); });
+      ypb<T, CT>(grad, S, tmp, 9, 0, d.nz);
       // outputs go to a separate x-pencil buffer: the three outputs of a row
       // tile are written by different workgroups while the inputs are read
       CT* xout = (CT*)gphys_;
-      launch(kc_f<T>(), 0.0, [&] {
+      launch(is32<T>() ? KC_XFU32 : KC_XFU64, 12 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] {
         xfused_conv<T>(fft_, tmp, uvwnl, R, (T)0, (T)0, (T)0, 3, xout, erosion_planes ? (T*)dudz_bot_ : nullptr,
                        erosion_planes ? (T*)dvdz_bot_ : nullptr, st_);
       });
-      launch(kc_f<T>(), 3.0, [&] { fft_ypass_forward<T, CT>(fft_, xout, adv, S, 3, 0, d.nz, st_); });
+      ypf<T, CT>(xout, adv, S, 3, 0, d.nz);
     } else {
-      launch(kc_f<T>(), 9.0, [&] { fft_backward<T, CT>(fft_, grad, S, gphys, R, 9, 0, d.nz, tmp, st_); });
-      launch(KC_OTHER, 0.0, [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
-      launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, aphys, R, adv, S, 3, 0, d.nz, tmp, st_); });
+      bwd<T, CT>(grad, S, gphys, R, 9, 0, d.nz, tmp);
+      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 15.0 * R * sizeof(T), [&] { products_conv<T>(uvwnl, gphys, aphys, R, st_); });
+      fwd<T, CT>(aphys, R, adv, S, 3, 0, d.nz, tmp);
       if (erosion_planes) {
         const size_t pl = (size_t)d.nx * d.ny * sizeof(T);
         TBD_HIP(hipMemcpyAsync(dudz_bot_, gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
@@ -405,11 +488,11 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   } else {
     // divergence form (phase_carrier_flat.cc:725-779): six distinct products
     if (fused_) {
-      launch(kc_f<T>(), 0.0, [&] { xfused_div6<T>(fft_, uvwnl, R, tmp, st_); });
-      launch(kc_f<T>(), 6.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, prod, S, 6, 0, d.nz, st_); });
+      launch(is32<T>() ? KC_XFU32 : KC_XFU64, 6 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] { xfused_div6<T>(fft_, uvwnl, R, tmp, st_); });
+      ypf<T, CT>(tmp, prod, S, 6, 0, d.nz);
     } else {
-      launch(KC_OTHER, 0.0, [&] { products_div<T>(uvwnl, gphys, R, st_); });
-      launch(kc_f<T>(), 6.0, [&] { fft_forward<T, CT>(fft_, gphys, R, prod, S, 6, 0, d.nz, tmp, st_); });
+      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 9.0 * R * sizeof(T), [&] { products_div<T>(uvwnl, gphys, R, st_); });
+      fwd<T, CT>(gphys, R, prod, S, 6, 0, d.nz, tmp);
     }
     // adv_i = (i kx F_i0 + i ky F_i1) + D1 F_i2 ; rows (uu,uv,uw) (uv,vv,vw) (uw,vw,ww)
     const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
@@ -418,7 +501,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
       a.x = prod + rows[f][2] * S; a.e0 = prod + rows[f][0] * S; a.e1 = prod + rows[f][1] * S;
       a.y = adv + f * S; a.kx = kx_; a.ky = ky_;
-      launch(kc_z<T>(), 1.0, [&] { zgemm<T, CT, CT>(a, 1, st_); });
+      launch(is32<T>() ? KC_Z32 : KC_Z64, zflops(d), [&] { zgemm<T, CT, CT>(a, 1, st_); });
     }
     // Gradient(u), Gradient(v) + DftBackward for the erosion map: only the
     // bottom plane of du/dz, dv/dz is read (phase_scalar_flat.cc:349-351)
@@ -428,10 +511,8 @@ void Context::carrier_predict(int stage, double time, int alternation) {
         zrows_wall(d1_, lda_, d.nz, uvw_ + S, d.P, wrows_ + 2 * d.P, wrows_ + 3 * d.P, st_);
       });
       const long long plane = (long long)d.nx * d.ny;
-      launch(kc_f<T>(), 2.0 / d.nz, [&] {
-        fft_backward<T, double2>(fft_, wrows_, 2 * d.P, (T*)dudz_bot_, plane, 1, 0, 1, tmp, st_, 1);
-        fft_backward<T, double2>(fft_, wrows_ + 2 * d.P, 2 * d.P, (T*)dvdz_bot_, plane, 1, 0, 1, tmp, st_, 1);
-      });
+      bwd<T, double2>(wrows_, 2 * d.P, (T*)dudz_bot_, plane, 1, 0, 1, tmp, 1);
+      bwd<T, double2>(wrows_ + 2 * d.P, 2 * d.P, (T*)dvdz_bot_, plane, 1, 0, 1, tmp, 1);
     }
   }
   // H = c1 H - dt adv, buoyancy, mean-mode force (:294-504)
@@ -441,6 +522,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   u.K = dt_ * cfg.averaged_concentration / std::pow(cfg.froude, 2) * (cfg.specific_gravity - 1);
   u.sn = std::sin(cfg.slope); u.cs = std::cos(cfg.slope); u.slope_dir = cfg.slope_direction;
   forces(time, u.f1, u.f2);
+  if (capturing_) u.fdev = force_;
   launch(KC_OTHER, 0.0, [&] { h_update<CT>(huvw_, adv, c_, u, 3, S, st_); });
   // lap = D2 u - kx^2 u - ky^2 u ; rhs = sigma (c2 H + u) - lap (:517-536)
   {
@@ -449,7 +531,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
     a.x = uvw_; a.xs = S; a.y = rhs_; a.ys = S; a.h = huvw_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
-    launch(KC_ZGEMM, 3.0, [&] { zgemm<double, double2, double2>(a, 3, st_); });
+    launch(KC_Z64, 3 * zflops(d), [&] { zgemm<double, double2, double2>(a, 3, st_); });
   }
   // boundary rows from the extrapolated wall pressure gradient (boundary_carrier_flat.cc:185-246)
   {
@@ -482,7 +564,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       h.f[f].x = rhs_ + f * S;
       h.f[f].y = suvw_ + f * S;
     }
-    launch(KC_HELM, 6.0, [&] { helmholtz(h, 3, st_); });
+    launch(KC_HELM, 6 * zflops(d), [&] { helmholtz(h, 3, st_); });
   }
 }
 
@@ -499,7 +581,7 @@ void Context::carrier_correct(int stage) {
     a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
     a.x = suvw_ + 2 * S; a.e0 = suvw_; a.e1 = suvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
     a.q = 0.5 / dt_ / c3;
-    launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_Z64, zflops(d), [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // pressure solve (:594, :955-987)
   {
@@ -510,7 +592,7 @@ void Context::carrier_correct(int stage) {
     h.f[0].nullrow = fac[3].nullrow;
     h.f[0].x = rhsp_;
     h.f[0].y = p_;
-    launch(KC_HELM, 2.0, [&] { helmholtz(h, 1, st_); });
+    launch(KC_HELM, 2 * zflops(d), [&] { helmholtz(h, 1, st_); });
   }
   // grad p, boundary Update (m2 <- m1, m1 <- walls), u = s - 2 dt c3 grad p,
   // mean w := 0 (:597-620, boundary_carrier_flat.cc:131-170)
@@ -524,10 +606,10 @@ void Context::carrier_correct(int stage) {
     a.wall_bot = wall_[wslot_][0]; a.wall_top = wall_[wslot_][1];
     a.wall2_bot = wall2_[wslot_][0]; a.wall2_top = wall2_[wslot_][1];
     a.open_top = opentop_ ? 1 : 0;
-    launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_Z64, zflops(d), [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // uvwnonlinear = DftBackward(uvw) (:625-626)
-  launch(kc_f<T>(), 3.0, [&] { fft_backward<T, double2>(fft_, uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_, st_); });
+  bwd<T, double2>(uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_);
   if (stage == 2) {   // diagnostics (:633-703)
     fill_zero(fl_, sizeof(double) * 20, st_);
     launch(KC_OTHER, 0.0, [&] {
@@ -564,35 +646,33 @@ void Context::scalar_correct(int stage, int alternation) {
   const T sw = (T)(-ws * std::cos(cfg.slope));
   if (alternation) {   // divergence form (:563-580)
     if (fused_) {
-      launch(kc_f<T>(), 1.0, [&] { fft_ypass_backward<T, double2>(fft_, c_, S, tmp, 1, 0, d.nz, st_); });
-      launch(kc_f<T>(), 0.0, [&] { xfused_cdiv<T>(fft_, tmp, uvwnl, R, su, sv, sw, tmp, st_); });
-      launch(kc_f<T>(), 3.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, prod, S, 3, 0, d.nz, st_); });
+      ypb<T, double2>(c_, S, tmp, 1, 0, d.nz);
+      launch(is32<T>() ? KC_XFU32 : KC_XFU64, 4 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] { xfused_cdiv<T>(fft_, tmp, uvwnl, R, su, sv, sw, tmp, st_); });
+      ypf<T, CT>(tmp, prod, S, 3, 0, d.nz);
     } else {
-      launch(kc_f<T>(), 1.0, [&] { fft_backward<T, double2>(fft_, c_, S, aphys, R, 1, 0, d.nz, tmp, st_); });
-      launch(KC_OTHER, 0.0, [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
-      launch(kc_f<T>(), 3.0, [&] { fft_forward<T, CT>(fft_, gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp, st_); });
+      bwd<T, double2>(c_, S, aphys, R, 1, 0, d.nz, tmp);
+      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 7.0 * R * sizeof(T), [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
+      fwd<T, CT>(gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp);
     }
     ZArgs a;
     a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = prod + 2 * S; a.e0 = prod; a.e1 = prod + S; a.y = advc; a.kx = kx_; a.ky = ky_;
-    launch(kc_z<T>(), 1.0, [&] { zgemm<T, CT, CT>(a, 1, st_); });
+    launch(is32<T>() ? KC_Z32 : KC_Z64, zflops(d), [&] { zgemm<T, CT, CT>(a, 1, st_); });
   } else {             // convective form (:584-592)
     ZArgs a;
     a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = c_; a.xs = S; a.y = grad + 6 * S; a.ys = S; a.kx = kx_; a.ky = ky_;
-    launch(kc_z<T>(), 1.0, [&] { zgemm<T, double2, CT>(a, 1, st_); });
+    launch(is32<T>() ? KC_Z32 : KC_Z64, zflops(d), [&] { zgemm<T, double2, CT>(a, 1, st_); });
     if (fused_) {
-      launch(kc_f<T>(), 3.0, [&] { fft_ypass_backward<T, CT>(fft_, grad + 6 * S, S, tmp, 3, 0, d.nz, st_); });
-      launch(kc_f<T>(), 0.0, [&] {
+      ypb<T, CT>(grad + 6 * S, S, tmp, 3, 0, d.nz);
+      launch(is32<T>() ? KC_XFU32 : KC_XFU64, 4 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] {
         xfused_conv<T>(fft_, tmp, uvwnl, R, su, sv, sw, 1, tmp, nullptr, nullptr, st_);
       });
-      launch(kc_f<T>(), 1.0, [&] { fft_ypass_forward<T, CT>(fft_, tmp, advc, S, 1, 0, d.nz, st_); });
+      ypf<T, CT>(tmp, advc, S, 1, 0, d.nz);
     } else {
-      launch(kc_f<T>(), 3.0, [&] {
-        fft_backward<T, CT>(fft_, grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp, st_);
-      });
-      launch(KC_OTHER, 0.0, [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
-      launch(kc_f<T>(), 1.0, [&] { fft_forward<T, CT>(fft_, aphys, R, advc, S, 1, 0, d.nz, tmp, st_); });
+      bwd<T, CT>(grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp);
+      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 7.0 * R * sizeof(T), [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
+      fwd<T, CT>(aphys, R, advc, S, 1, 0, d.nz, tmp);
     }
   }
   // H = c1 H - dt adv (:313-315)
@@ -608,7 +688,7 @@ void Context::scalar_correct(int stage, int alternation) {
     a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
     a.x = c_; a.xs = S; a.y = rhsc_; a.ys = S; a.h = hc_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
-    launch(KC_ZGEMM, 1.0, [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_Z64, zflops(d), [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // boundary rows + erosion/deposition (boundary_scalar_flat.cc:151-152, :340-384)
   const int erosion = cfg.scalar_bc == 1 ? 1 : 0;
@@ -618,9 +698,7 @@ void Context::scalar_correct(int stage, int alternation) {
       erosion_map<T>((const T*)dudz_bot_, (const T*)dvdz_bot_, ephys_, plane, cfg.reynolds,
                      cfg.critical_shear_stress, cfg.erosion_rate, st_);
     });
-    launch(KC_FFT, 1.0 / d.nz, [&] {
-      fft_forward<double, double2>(fft_, ephys_, plane, ehat_, d.P, 1, 0, 1, (double2*)tmp_, st_, 1);
-    });
+    fwd<double, double2>(ephys_, plane, ehat_, d.P, 1, 0, 1, (double2*)tmp_, 1);
   }
   const double wsc = cfg.settling_velocity * std::cos(cfg.slope);
   launch(KC_OTHER, 0.0, [&] {
@@ -640,12 +718,12 @@ void Context::scalar_correct(int stage, int alternation) {
     h.f[0].nullrow = -1;
     h.f[0].x = rhsc_;
     h.f[0].y = c_;
-    launch(KC_HELM, 2.0, [&] { helmholtz(h, 1, st_); });
+    launch(KC_HELM, 2 * zflops(d), [&] { helmholtz(h, 1, st_); });
   }
   // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
-  launch(KC_FFT, 1.0, [&] { fft_backward<double, double2>(fft_, c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_, st_); });
+  bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_);
   launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, st_); });
-  launch(KC_FFT, 1.0, [&] { fft_forward<double, double2>(fft_, cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_, st_); });
+  fwd<double, double2>(cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_);
   launch(KC_OTHER, 0.0, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });
 }
 

@@ -13,14 +13,28 @@
 
 namespace tbd {
 
+// Kernel classes of the profiler: one class per kernel (and precision), with
+// the algorithmic work each launch does (bytes for the HBM-bound passes,
+// flops for the contractions).
+enum KClass {
+  KC_YB64 = 0, KC_XB64, KC_XF64, KC_YF64,     // y/x transform passes, FP64
+  KC_YB32, KC_XB32, KC_XF32, KC_YF32,         // the same in FP32
+  KC_XFU64, KC_XFU32,                         // fused x transforms + products
+  KC_Z64, KC_Z32,                             // wall-normal contractions
+  KC_HELM,                                    // Helmholtz / pressure solves
+  KC_PROD64, KC_PROD32,                       // pointwise nonlinear products
+  KC_OTHER,
+  KC_N
+};
+
 struct Profiler {
   bool on = false;
-  struct Rec { int kclass; double units; hipEvent_t a, b; };
+  struct Rec { int kclass; double work; hipEvent_t a, b; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
-  long long launches[6] = {0, 0, 0, 0, 0, 0};
-  double ms[6] = {0, 0, 0, 0, 0, 0};
-  double units[6] = {0, 0, 0, 0, 0, 0};   // contractions / field-transforms
+  long long launches[KC_N] = {};
+  double ms[KC_N] = {};
+  double work[KC_N] = {};   // bytes or flops
   hipEvent_t get();
   void begin(int kclass, hipStream_t st, Rec& r);
   void end(Rec& r, hipStream_t st);
@@ -38,6 +52,7 @@ class Context {
   void download(double* uvw, double* p, double* c);
   void download_physical_velocity(double* out);
   void substage(int stage, double time, int alternation);
+  void step(double time, int alternation);
   void set_dt(double dt);
   void features(double* out50);
   void sync();
@@ -56,12 +71,34 @@ class Context {
   template <typename T> void carrier_correct(int stage);
   template <typename T> void scalar_correct(int stage, int alternation);
   void forces(double time, double& f1, double& f2) const;
-  template <class F> void launch(int kclass, double units, F&& f);
+  template <class F> void launch(int kclass, double work, F&& f);
+  // profiled transform passes (one launch each)
+  template <typename T, typename SpecT>
+  void ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol = -1);
+  template <typename T>
+  void xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes);
+  template <typename T>
+  void xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes);
+  template <typename T, typename SpecT>
+  void ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol = -1);
+  template <typename T, typename SpecT>
+  void fwd(const T* phys, long long pstride, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
+           C2<T>* tmp, int scol = -1) {
+    xpf<T>(phys, pstride, tmp, nv, nplanes);
+    ypf<T, SpecT>(tmp, spec, sstride, nv, zoff, nplanes, scol);
+  }
+  template <typename T, typename SpecT>
+  void bwd(const SpecT* spec, long long sstride, T* phys, long long pstride, int nv, int zoff, int nplanes,
+           C2<T>* tmp, int scol = -1) {
+    ypb<T, SpecT>(spec, sstride, tmp, nv, zoff, nplanes, scol);
+    xpb<T>(tmp, phys, pstride, nv, nplanes);
+  }
   void* alloc(size_t bytes);
 
   hipStream_t st_ = nullptr;
   FftPlan fft_;
   int lda_ = 0;
+  double xpb_ = 0;          // complex elements of one x-pencil component (nz*mx*ny)
   bool opentop_ = false;
   bool fused_ = false;      // x transforms fused around the nonlinear products (TBD_FUSED=1)
   double dt_ = 0, olddt_ = 0;
@@ -116,6 +153,19 @@ class Context {
   double* fg_ = nullptr;       // featureglobal_ (64)
   double* wallmean_ = nullptr; // 6: D1 u/v/w rows 0 and nz-1 on the mean column
   double* diagpart_ = nullptr; // per-block maxima of the diagnostics
+
+  // captured steps (graph replay; TBD_GRAPH=0 runs the substages eagerly)
+  struct StepGraph {
+    int alternation = 0, slot = 0;
+    double dt = 0, olddt = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<StepGraph> graphs_;
+  bool use_graphs_ = true;
+  bool capturing_ = false;
+  double* force_ = nullptr;    // mean-mode force read by the captured H update
+  void drop_graphs();
 
   std::vector<void*> allocs_;
 };

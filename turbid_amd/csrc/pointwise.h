@@ -29,8 +29,11 @@ struct HUpd {
   double K = 0;          // dt*Cbar/Fr^2*(sg-1)
   double sn = 0, cs = 0; int slope_dir = 0;
   double f1 = 0, f2 = 0; // mean-mode force (u, v)
+  const double* fdev = nullptr;   // if set: the force is read from fdev[0], fdev[1]
   int nz = 0;
 };
+// dst[0] = a, dst[1] = b (values that a captured step graph reads at replay)
+void set_pair(double* dst, double a, double b, hipStream_t st);
 template <typename TA>
 void h_update(double2* h, const TA* adv, const double2* c, const HUpd& u, int nf, long long S, hipStream_t st);
 

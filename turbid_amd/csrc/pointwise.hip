@@ -97,8 +97,9 @@ __global__ void k_h_update(double2* __restrict__ h, const TA* __restrict__ adv, 
       }
     }
     if (e < u.nz) {                 // mean column (kx = ky = 0), all z
-      if (f == 0) hv.x += u.dt * u.f1;
-      else if (f == 1) hv.x += u.dt * u.f2;
+      const double f1 = u.fdev ? u.fdev[0] : u.f1, f2 = u.fdev ? u.fdev[1] : u.f2;
+      if (f == 0) hv.x += u.dt * f1;
+      else if (f == 1) hv.x += u.dt * f2;
     }
     h[i] = hv;
   }
@@ -106,6 +107,17 @@ __global__ void k_h_update(double2* __restrict__ h, const TA* __restrict__ adv, 
 template <typename TA>
 void h_update(double2* h, const TA* adv, const double2* c, const HUpd& u, int nf, long long S, hipStream_t st) {
   hipLaunchKernelGGL(k_h_update<TA>, grid1d(nf * S), dim3(kNT), 0, st, h, adv, c, u, nf, S);
+  TBD_HIP(hipGetLastError());
+}
+
+__global__ void k_set_pair(double* dst, double a, double b) {
+  if (threadIdx.x == 0) {
+    dst[0] = a;
+    dst[1] = b;
+  }
+}
+void set_pair(double* dst, double a, double b, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_pair, dim3(1), dim3(64), 0, st, dst, a, b);
   TBD_HIP(hipGetLastError());
 }
 

@@ -39,7 +39,14 @@ EXPORTS = [
     "tbd_get_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
 ]
 
-KCLASS = {"fft": 0, "zgemm": 1, "helmholtz": 2, "other": 3, "zgemm32": 4, "fft32": 5}
+# profiler classes (one per kernel): name -> (index, work kind)
+KCLASSES = {
+    "ypass_bwd64": (0, "bytes"), "xpass_bwd64": (1, "bytes"), "xpass_fwd64": (2, "bytes"), "ypass_fwd64": (3, "bytes"),
+    "ypass_bwd32": (4, "bytes"), "xpass_bwd32": (5, "bytes"), "xpass_fwd32": (6, "bytes"), "ypass_fwd32": (7, "bytes"),
+    "xfused64": (8, "bytes"), "xfused32": (9, "bytes"), "zgemm64": (10, "flops"), "zgemm32": (11, "flops"),
+    "helmholtz": (12, "flops"), "products64": (13, "bytes"), "products32": (14, "bytes"), "other": (15, None),
+}
+KCLASS = {k: v[0] for k, v in KCLASSES.items()}
 
 
 class TbdConfig(ctypes.Structure):
@@ -217,7 +224,8 @@ class Solver:
         _check(lib().tbd_profile_reset(self._h))
 
     def profile_read(self, kclass):
-        """(launches, total_ms, units) of a kernel class since the last reset."""
+        """(launches, total_ms, work) of a kernel class since the last reset; work in
+        bytes or flops (KCLASSES)."""
         n = ctypes.c_longlong()
         ms = ctypes.c_double()
         un = ctypes.c_double()
