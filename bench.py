@@ -124,6 +124,22 @@ def cpu_baseline(cfg, state, nthreads):
             "sample": "1 RK3 step (3 substages, carrier + scalar) of the same workload after %.1f s setup" % setup}
 
 
+def copy_calibration():
+    """One plain device-to-device copy of 2^30 bytes after a warm-up copy: the
+    FETCH_SIZE / WRITE_SIZE counters of its dispatch read 2^30 bytes each."""
+    import torch
+    n = 1 << 30
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes": n, "seconds": dt, "GB/s": 2 * n / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,7 +149,12 @@ def main():
     ap.add_argument("--workload", default="turb256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 1 GiB (reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the single-DNS shards and the ensemble members are independent per GPU); use --gpus 1",

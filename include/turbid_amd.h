@@ -128,6 +128,41 @@ int tbd_zcontract(int nz, long long ncol, int fp32, const double* m, const doubl
  * fix, 1 pressure (src/phase_carrier_flat.cc:861-994). */
 int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x, double* y);
 
+/* Divergence-free projection of the context's velocity, for states that do
+ * not come out of a time step (a proposed initial state): the pressure-type
+ * solve of FlatCarrierPhase::Correct (src/phase_carrier_flat.cc:586-620)
+ * with a unit step, u -= grad phi, mean w := 0, physical velocity refreshed.
+ * The wall pressure-gradient history is cleared, as after a load. */
+int tbd_project(tbd_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Learned spin-up proposal (no counterpart in the reference): a factorized
+ * spectral neural operator on an nx*ny Fourier x nz Chebyshev grid with
+ * `channels` channels and `layers` residual layers, cin input and cout output
+ * fields per point, trained and evaluated with BF16 MFMA kernels (FP32
+ * accumulation).  Tensors are float, [sample][z][y][x][field]; parameters are
+ * one float block (tbd_sno_param_count floats) in the order lift (cin*C, C),
+ * per layer (W C*C, b C, Rx (nx/2+1)*2*C*C, Ry (ny/2+1)*2*C*C, Rz nz*C*C),
+ * projection (C*cout, cout).
+ * ------------------------------------------------------------------------- */
+typedef struct tbd_sno tbd_sno;
+int tbd_sno_create(int nx, int ny, int nz, int channels, int layers, int cin, int cout, int batch,
+                   tbd_sno** out);
+void tbd_sno_destroy(tbd_sno* net);
+int tbd_sno_param_count(tbd_sno* net, long long* count);
+int tbd_sno_set_params(tbd_sno* net, const float* params);
+int tbd_sno_get_params(tbd_sno* net, float* params);
+int tbd_sno_get_grads(tbd_sno* net, float* grads);
+/* y[nb][points][cout] = network(x[nb][points][cin]), nb <= batch */
+int tbd_sno_forward(tbd_sno* net, const float* x, float* y, int nb);
+/* loss = mean over samples, points and outputs of w[o] (y - t)^2, and its
+ * gradient (tbd_sno_get_grads); train_step also applies one Adam update */
+int tbd_sno_loss_grad(tbd_sno* net, const float* x, const float* t, const float* w, int nb, double* loss);
+int tbd_sno_train_step(tbd_sno* net, const float* x, const float* t, const float* w, int nb, float lr,
+                       double* loss);
+/* the BF16 GEMM on host buffers: C (m x n) = A (m x k) B (k x n), row-major */
+int tbd_bf16_gemm(int m, int n, int k, const float* a, const float* b, float* c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,7 @@
 // C ABI (include/turbid_amd.h).  Exceptions never cross the boundary: they
 // become negative return codes with the message kept for tbd_last_error().
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -9,6 +10,7 @@
 #include "fft.h"
 #include "pointwise.h"
 #include "zgemm.h"
+#include "sno.h"
 
 struct tbd_ctx {
   tbd::Context* c;
@@ -37,6 +39,11 @@ struct DBuf {
   explicit DBuf(size_t n) { TBD_HIP(hipMalloc(&p, n < 256 ? 256 : n)); }
   ~DBuf() { if (p) (void)hipFree(p); }
   template <class T> T* as() { return (T*)p; }
+};
+struct FBuf {
+  float* p = nullptr;
+  explicit FBuf(size_t n) { TBD_HIP(hipMalloc((void**)&p, n * sizeof(float) < 256 ? 256 : n * sizeof(float))); }
+  ~FBuf() { if (p) (void)hipFree(p); }
 };
 }  // namespace
 
@@ -268,6 +275,87 @@ int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x,
   return guard([&] {
     TBD_CHECK(ctx && x && y && stage >= 0 && stage < 3 && (mode == 0 || mode == 1), "tbd_helmholtz > bad argument");
     ctx->c->helmholtz_host(which, stage, mode, x, y);
+  });
+}
+
+int tbd_project(tbd_ctx* ctx) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_project > null context");
+    ctx->c->project();
+  });
+}
+
+struct tbd_sno {
+  tbd::Sno* n;
+};
+
+int tbd_sno_create(int nx, int ny, int nz, int channels, int layers, int cin, int cout, int batch, tbd_sno** out) {
+  return guard([&] {
+    TBD_CHECK(out, "tbd_sno_create > null output");
+    tbd::SnoShape s;
+    s.nx = nx; s.ny = ny; s.nz = nz; s.C = channels; s.L = layers; s.cin = cin; s.cout = cout; s.batch = batch;
+    std::unique_ptr<tbd::Sno> n(new tbd::Sno(s));
+    *out = new tbd_sno{n.release()};
+  });
+}
+void tbd_sno_destroy(tbd_sno* net) {
+  if (!net) return;
+  delete net->n;
+  delete net;
+}
+int tbd_sno_param_count(tbd_sno* net, long long* count) {
+  return guard([&] {
+    TBD_CHECK(net && count, "tbd_sno_param_count > null argument");
+    *count = net->n->lay.total;
+  });
+}
+int tbd_sno_set_params(tbd_sno* net, const float* params) {
+  return guard([&] {
+    TBD_CHECK(net && params, "tbd_sno_set_params > null argument");
+    net->n->set_params(params);
+  });
+}
+int tbd_sno_get_params(tbd_sno* net, float* params) {
+  return guard([&] {
+    TBD_CHECK(net && params, "tbd_sno_get_params > null argument");
+    net->n->get_params(params);
+  });
+}
+int tbd_sno_get_grads(tbd_sno* net, float* grads) {
+  return guard([&] {
+    TBD_CHECK(net && grads, "tbd_sno_get_grads > null argument");
+    net->n->get_grads(grads);
+  });
+}
+int tbd_sno_forward(tbd_sno* net, const float* x, float* y, int nb) {
+  return guard([&] {
+    TBD_CHECK(net && x && y, "tbd_sno_forward > null argument");
+    net->n->forward(x, y, nb);
+  });
+}
+int tbd_sno_loss_grad(tbd_sno* net, const float* x, const float* t, const float* w, int nb, double* loss) {
+  return guard([&] {
+    TBD_CHECK(net && x && t && w && loss, "tbd_sno_loss_grad > null argument");
+    *loss = net->n->loss_and_grad(x, t, w, nb);
+  });
+}
+int tbd_sno_train_step(tbd_sno* net, const float* x, const float* t, const float* w, int nb, float lr,
+                       double* loss) {
+  return guard([&] {
+    TBD_CHECK(net && x && t && w && loss, "tbd_sno_train_step > null argument");
+    *loss = net->n->train_step(x, t, w, nb, lr);
+  });
+}
+int tbd_bf16_gemm(int m, int n, int k, const float* a, const float* b, float* c) {
+  return guard([&] {
+    TBD_CHECK(m > 0 && n > 0 && k > 0 && a && b && c, "tbd_bf16_gemm > bad argument");
+    FBuf da((size_t)m * k), db((size_t)k * n), dc((size_t)m * n);
+    TBD_HIP(hipMemcpy(da.p, a, sizeof(float) * m * k, hipMemcpyHostToDevice));
+    TBD_HIP(hipMemcpy(db.p, b, sizeof(float) * k * n, hipMemcpyHostToDevice));
+    tbd::gemm_bf16(m, n, k, 1, 1, 1.f, tbd::GemmOp{da.p, 0, 0, k, 1}, tbd::GemmOp{db.p, 0, 0, n, 1}, 0.f,
+                   tbd::GemmOut{dc.p, 0, 0, n, 1}, 0);
+    TBD_HIP(hipDeviceSynchronize());
+    TBD_HIP(hipMemcpy(c, dc.p, sizeof(float) * m * n, hipMemcpyDeviceToHost));
   });
 }
 

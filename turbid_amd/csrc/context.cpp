@@ -743,6 +743,51 @@ void Context::substage(int stage, double time, int alternation) {
   }
 }
 
+// Projection of an arbitrary velocity onto divergence-free fields: the
+// pressure-type solve of Correct (phase_carrier_flat.cc:586-620) with
+// 2 dt c3 = 1, phi in place of p.
+void Context::project() {
+  const long long S = d.S;
+  {
+    ZArgs a;
+    a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
+    a.x = uvw_ + 2 * S; a.e0 = uvw_; a.e1 = uvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
+    a.q = 1.0;
+    zgemm<double, double2, double2>(a, 1, st_);
+  }
+  {
+    HArgs h;
+    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 1; h.kx = kx_; h.ky = ky_;
+    h.f[0].vinv = vinv_[3]; h.f[0].vnorm = vnorm_[3]; h.f[0].lambda = lam_[3]; h.f[0].fix = fix_[3];
+    h.f[0].sigma = 0.0;
+    h.f[0].nullrow = fac[3].nullrow;
+    h.f[0].x = rhsp_;
+    h.f[0].y = rhsp_;
+    helmholtz(h, 1, st_);
+  }
+  {
+    ZArgs a;
+    a.mode = Z_CORRECT; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
+    a.x = rhsp_; a.e0 = uvw_; a.es = S; a.y = uvw_; a.ys = S; a.kx = kx_; a.ky = ky_;
+    a.cf = 1.0;
+    a.wall_bot = wall_[0][0]; a.wall_top = wall_[0][1];
+    a.wall2_bot = wall2_[0][0]; a.wall2_top = wall2_[0][1];
+    a.open_top = opentop_ ? 1 : 0;
+    zgemm<double, double2, double2>(a, 1, st_);
+  }
+  for (int s = 0; s < 2; ++s)
+    for (int b = 0; b < 2; ++b) {
+      fill_zero(wall_[s][b], sizeof(double2) * 3 * d.P, st_);
+      fill_zero(wall2_[s][b], sizeof(double2) * 2 * d.P, st_);
+    }
+  olddt_ = dt_;
+  if (cfg.precision == 1)
+    fft_backward<float, double2>(fft_, uvw_, d.S, (float*)uvwnl_, d.R, 3, 0, d.nz, (float2*)tmp_, st_);
+  else
+    fft_backward<double, double2>(fft_, uvw_, d.S, (double*)uvwnl_, d.R, 3, 0, d.nz, (double2*)tmp_, st_);
+  sync();
+}
+
 void Context::helmholtz_host(int which, int stage, int mode, const double* x, double* y) {
   TBD_CHECK(which >= 0 && which < (cfg.nphase > 1 ? 5 : 4), "Context::helmholtz > bad factorisation index");
   const size_t S16 = sizeof(double2) * d.S;

@@ -53,6 +53,7 @@ class Context {
   void download_physical_velocity(double* out);
   void substage(int stage, double time, int alternation);
   void step(double time, int alternation);
+  void project();
   void set_dt(double dt);
   void features(double* out50);
   void sync();

@@ -37,6 +37,9 @@ EXPORTS = [
     "tbd_download_physical_velocity", "tbd_substage", "tbd_step", "tbd_set_dt", "tbd_features",
     "tbd_sync", "tbd_profile_enable", "tbd_profile_read", "tbd_profile_reset", "tbd_get_operators",
     "tbd_get_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
+    "tbd_project", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
+    "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
+    "tbd_bf16_gemm",
 ]
 
 # profiler classes (one per kernel): name -> (index, work kind)
@@ -105,6 +108,19 @@ def lib():
         L.tbd_dft_backward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
         L.tbd_zcontract.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp, dp, dp]
         L.tbd_helmholtz.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.tbd_project.argtypes = [vp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.tbd_sno_create.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(vp)]
+        L.tbd_sno_destroy.argtypes = [vp]
+        L.tbd_sno_destroy.restype = None
+        L.tbd_sno_param_count.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        L.tbd_sno_set_params.argtypes = [vp, fp]
+        L.tbd_sno_get_params.argtypes = [vp, fp]
+        L.tbd_sno_get_grads.argtypes = [vp, fp]
+        L.tbd_sno_forward.argtypes = [vp, fp, fp, ctypes.c_int]
+        L.tbd_sno_loss_grad.argtypes = [vp, fp, fp, fp, ctypes.c_int, dp]
+        L.tbd_sno_train_step.argtypes = [vp, fp, fp, fp, ctypes.c_int, ctypes.c_float, dp]
+        L.tbd_bf16_gemm.argtypes = [ctypes.c_int] * 3 + [fp, fp, fp]
         _lib = L
     return _lib
 
@@ -209,6 +225,10 @@ class Solver:
     def set_dt(self, dt):
         _check(lib().tbd_set_dt(self._h, dt))
 
+    def project(self):
+        """Make the current velocity divergence-free (proposed initial states)."""
+        _check(lib().tbd_project(self._h))
+
     def features(self):
         f = np.zeros(50)
         _check(lib().tbd_features(self._h, _p(f)))
@@ -299,3 +319,86 @@ def run_steps(solver, stepper, nsteps):
         stepper.toggle()
         feat = solver.features()
     return feat
+
+
+# ---------------------------------------------------------------------------
+# learned spin-up proposal network
+# ---------------------------------------------------------------------------
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def bf16_gemm(a, b):
+    """C = A B on the device BF16 MFMA GEMM (operands rounded to BF16, FP32 sums)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    m, k = a.shape
+    k2, n = b.shape
+    assert k == k2
+    c = np.zeros((m, n), dtype=np.float32)
+    _check(lib().tbd_bf16_gemm(m, n, k, _fp(a), _fp(b), _fp(c)))
+    return c
+
+
+class SpectralOperatorNet:
+    """Factorized spectral neural operator on an nx*ny Fourier x nz Chebyshev
+    grid (device BF16 MFMA forward, backward and Adam)."""
+
+    def __init__(self, nx=24, ny=24, nz=33, channels=32, layers=4, cin=7, cout=4, batch=8):
+        self.shape = dict(nx=nx, ny=ny, nz=nz, channels=channels, layers=layers, cin=cin, cout=cout, batch=batch)
+        self.points = nx * ny * nz
+        h = ctypes.c_void_p()
+        _check(lib().tbd_sno_create(nx, ny, nz, channels, layers, cin, cout, batch, ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_longlong()
+        _check(lib().tbd_sno_param_count(self._h, ctypes.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if self._h:
+            lib().tbd_sno_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        _check(lib().tbd_sno_set_params(self._h, _fp(p)))
+
+    def params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().tbd_sno_get_params(self._h, _fp(p)))
+        return p
+
+    def grads(self):
+        g = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().tbd_sno_get_grads(self._h, _fp(g)))
+        return g
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        nb = x.shape[0]
+        y = np.zeros((nb, self.points, self.shape["cout"]), dtype=np.float32)
+        _check(lib().tbd_sno_forward(self._h, _fp(x), _fp(y), nb))
+        return y
+
+    def loss_grad(self, x, t, w):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().tbd_sno_loss_grad(self._h, _fp(x), _fp(t), _fp(w), x.shape[0], ctypes.byref(loss)))
+        return loss.value
+
+    def train_step(self, x, t, w, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().tbd_sno_train_step(self._h, _fp(x), _fp(t), _fp(w), x.shape[0], float(lr), ctypes.byref(loss)))
+        return loss.value
