@@ -314,8 +314,10 @@ bool FlatChannelSolver::Execute() {   // solver_channel_flat.cc:179-313
       Check(tbd_set_dt(ctx_, tstepper_->GetDt()));   // Phase::Update reassembly
     }
     const int step = tstepper_->GetStep();
-    for (int stage = 0; stage < 3; ++stage)
-      Check(tbd_substage(ctx_, stage, tstepper_->GetTime(), tstepper_->GetAlternationFlag() ? 1 : 0));
+    // the three stages of carrier Predict/Correct and scalar Predict/Correct
+    // (solver_channel_flat.cc:223-234) as one call: the library replays them
+    // from a captured HIP graph
+    Check(tbd_step(ctx_, tstepper_->GetTime(), tstepper_->GetAlternationFlag() ? 1 : 0));
     tstepper_->SetAlternationFlag();
     Check(tbd_features(ctx_, feature));
     if (tstepper_->isOutput()) {

@@ -1,0 +1,113 @@
+"""Ensemble of independent realisations on one GPU.
+
+    python -m turbid_amd.ensemble --members 16 --steps 20 --warmup 3
+
+Members draw, from a seed, Nx = Ny in {64, 96, 128}, Nz in {65, 97, 129}, a
+Stokes Reynolds number uniform in [400, 1000] and a settling velocity
+log-uniform in [1e-4, 2e-3]; the rest is the 256^3 turbulent configuration
+(same domain, forcing, boundary conditions, noise).  Each member is its own
+context (own stream, own captured step graph); an ensemble step issues every
+member's step before waiting for any, so members of different sizes run
+concurrently on the GPU.  The member loop is the reference's step loop
+(TimeStepper::Update with the member's previous CFL, three substages,
+alternation toggle).
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import runtime as RT
+from . import workloads as W
+
+
+def member_configs(n=16, seed=0, precision=1):
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        nxy = int(rng.choice([64, 96, 128]))
+        nz = int(rng.choice([65, 97, 129]))
+        re_d = float(rng.uniform(400.0, 1000.0))
+        ws = float(math.exp(rng.uniform(math.log(1e-4), math.log(2e-3))))
+        cfg = W.base_config(nxy, nxy, nz, 20.0, 10.0, W.reynolds_from_stokes(re_d), W.TWO_PI / 2000.0,
+                            settling=ws, schmidt=0.5, precision=precision)
+        cfg.update(noise_u=0.1, noise_c=0.01, seed=1000 + m, re_delta=re_d)
+        out.append(cfg)
+    return out
+
+
+class Ensemble:
+    def __init__(self, cfgs):
+        self.cfgs = cfgs
+        self.solvers = []
+        self.steppers = []
+        self.feats = []
+        for cfg in cfgs:
+            s = RT.Solver(cfg)
+            s.upload(*W.initial_state(cfg))
+            self.solvers.append(s)
+            self.steppers.append(RT.Stepper(cfg["dt"]))
+            self.feats.append(np.zeros(50))
+
+    def step(self, members=None):
+        """One step of the given members (all by default), concurrently."""
+        idx = range(len(self.solvers)) if members is None else members
+        for m in idx:
+            st = self.steppers[m]
+            st.update(self.feats[m][19])
+            self.solvers[m].step(st.time, st.alternation)
+            st.toggle()
+        for m in idx:
+            self.feats[m] = self.solvers[m].features()
+
+    def run_concurrent(self, nsteps):
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            self.step()
+        return time.perf_counter() - t0
+
+    def run_sequential(self, nsteps):
+        t0 = time.perf_counter()
+        for m in range(len(self.solvers)):
+            for _ in range(nsteps):
+                self.step([m])
+        return time.perf_counter() - t0
+
+    def finite(self):
+        return all(np.isfinite(f[0]) for f in self.feats)
+
+    def close(self):
+        for s in self.solvers:
+            s.close()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--members", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    cfgs = member_configs(args.members, args.seed, args.precision)
+    ens = Ensemble(cfgs)
+    for _ in range(args.warmup):
+        ens.step()
+    tc = ens.run_concurrent(args.steps)
+    ts = ens.run_sequential(args.steps)
+    total = args.members * args.steps
+    out = dict(metric="ensemble RK3 member-steps/s", members=args.members, steps=args.steps,
+               concurrent_steps_per_s=total / tc, sequential_steps_per_s=total / ts, speedup=ts / tc,
+               precision="mixed" if args.precision == 1 else "fp64", finite=ens.finite(),
+               grids=[[c["nx"], c["ny"], c["nz"]] for c in cfgs],
+               re_delta=[round(c["re_delta"], 1) for c in cfgs],
+               settling=[float("%.3g" % c["settling_velocity"]) for c in cfgs])
+    ens.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
