@@ -89,6 +89,23 @@ def run(cfg, state, steps, warmup):
     return (t1 - t0), (tp1 - tp0), feat, prof
 
 
+TRAFFIC = os.path.join(ROOT, "profiles", "turb256_hbm_traffic.json")
+
+
+def measured_traffic(workload, cls):
+    """HBM bytes per dispatch of the roofline kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
+    copy), when that summary is for the same workload and kernel."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("class") != cls:
+        return None
+    return float(t["bytes_per_dispatch"])
+
+
 def roofline(prof):
     best = max((n for n in KERNELS if KERNELS[n][1] and prof[n][0] > 0), key=lambda n: prof[n][1])
     n, ms, work = prof[best]
@@ -202,6 +219,7 @@ def main():
                    "grid": [cfg64["nx"], cfg64["ny"], cfg64["nz"]], "phases": 2},
         "roofline": {k: v for k, v in results[head]["roofline"].items()},
     }
+    line["roofline"]["traffic"] = measured_traffic(args.workload, line["roofline"]["class"])
     if "fp64" in results:
         line["fp64_steps_per_s"] = results["fp64"]["steps_per_s"]
         line["fp64_roofline"] = results["fp64"]["roofline"]

@@ -104,16 +104,21 @@ __host__ inline Panel make_panel(int nz, size_t elem, size_t lds_limit, int pari
     pn.sg[1] = make_seg(M + 1, M, M + 1, M + 1, M + 1);
     pn.ldx = (nz + 1) | 1;    // indices 0 .. 2M+1
   }
-  int pairs = 0;
-  for (int i = 0; i < pn.nseg; ++i) pairs += pn.sg[i].npair;
-  int ncg = pairs > 0 ? (kWaves + pairs - 1) / pairs : 4;
+  int pairs = 0, tailrows = 0;
+  for (int i = 0; i < pn.nseg; ++i) {
+    pairs += pn.sg[i].npair;
+    tailrows += pn.sg[i].rows - pn.sg[i].RM;
+  }
+  int ncg = pairs > 0 ? (kWaves + pairs - 1) / pairs : 1;
   if (ncg > 4) ncg = 4;
   while (ncg > 1 && (size_t)16 * ncg * pn.ldx * elem > lds_limit) --ncg;
+  // keep the rows past the last full tile within what one pass of the
+  // workgroup (or the register-buffered mode) holds
+  while (ncg > 1 && tailrows * 16 * ncg > kNT) --ncg;
   pn.ncg = ncg;
   pn.ncol = 16 * ncg;
   pn.nunit = pairs * ncg;
-  pn.ntail = 0;
-  for (int i = 0; i < pn.nseg; ++i) pn.ntail += (pn.sg[i].rows - pn.sg[i].RM) * pn.ncol;
+  pn.ntail = tailrows * pn.ncol;
   return pn;
 }
 

@@ -211,6 +211,12 @@ class Solver:
                                         None if c is None else _p(c.view(np.float64))))
         return u, p, c
 
+    def state(self):
+        """Downloaded state as uvw (3, mx, nyc, nz), p (1, ...), c (1, ...) or None."""
+        u, p, c = self.download()
+        shp = (self.mx, self.nyc, self.nz)
+        return (u.reshape((3,) + shp), p.reshape((1,) + shp), None if c is None else c.reshape((1,) + shp))
+
     def physical_velocity(self):
         out = np.zeros(3 * self.nx * self.ny * self.nz)
         _check(lib().tbd_download_physical_velocity(self._h, _p(out)))

@@ -55,7 +55,7 @@ class Run:
         feat = RT.run_steps(self.s, self.st, n)
         self.s.sync()
         self.gpu_s += time.perf_counter() - t0
-        uvw, p, c = self.s.download()
+        uvw, p, c = self.s.state()
         st = SG.flow_stats(uvw, c, feat)
         st.update(t=self.st.time, gpu_s=self.gpu_s, finite=bool(np.isfinite(st["tke"])))
         self.hist.append(st)
