@@ -7,12 +7,13 @@
    periods, then `sample` more periods during which 12 snapshots per period
    (one per phase bin) are reduced to the network grid.  The statistics of
    those periods (turbulent kinetic energy, mean-mode wall shear, mean
-   concentration) are the developed reference.
+   concentration) are the reference state.
 2. The proposal network is trained on the snapshots (device BF16 MFMA).
 3. Two fresh runs of `eval` periods each: the plain start, and the proposed
    state (from the same plain start, projected divergence-free).  For each,
-   the GPU seconds until the period-mean TKE and wall-shear amplitude first
-   come within a tolerance of the developed reference.
+   the GPU seconds until the period means of TKE, wall shear and mean
+   concentration first stay within a tolerance of the reference for two
+   consecutive periods.
 
 Every time step is the DNS's own; the network only sets the initial state.
 """
@@ -49,13 +50,19 @@ class Run:
         self.gpu_s = 0.0
         self.hist = []      # per bin: dict(t, tke, tau, cmean)
 
-    def advance_bin(self):
+    def advance_bin(self, parts=1, on_part=None):
+        """One phase bin (1/12 period) in `parts` equal pieces; on_part(uvw, c)
+        is called with the state after each piece."""
         n = self.steps_per_period // PHASE_BINS
-        t0 = time.perf_counter()
-        feat = RT.run_steps(self.s, self.st, n)
-        self.s.sync()
-        self.gpu_s += time.perf_counter() - t0
-        uvw, p, c = self.s.state()
+        for q in range(parts):
+            t0 = time.perf_counter()
+            feat = RT.run_steps(self.s, self.st, n // parts if q + 1 < parts else n - (parts - 1) * (n // parts))
+            self.s.sync()
+            self.gpu_s += time.perf_counter() - t0
+            if on_part is not None or q + 1 == parts:
+                uvw, p, c = self.s.state()
+                if on_part is not None:
+                    on_part(uvw, c)
         st = SG.flow_stats(uvw, c, feat)
         st.update(t=self.st.time, gpu_s=self.gpu_s, finite=bool(np.isfinite(st["tke"])))
         self.hist.append(st)
@@ -74,18 +81,22 @@ def period_means(hist):
         seg = hist[i:i + PHASE_BINS]
         out.append(dict(tke=float(np.mean([h["tke"] for h in seg])),
                         tau=float(np.mean([h["tau"] for h in seg])),
+                        cmean=float(np.mean([h["cmean"] for h in seg])),
                         gpu_s=seg[-1]["gpu_s"], period=i // PHASE_BINS + 1))
     return out
 
 
-def time_to_reference(hist, ref, tol):
-    """GPU seconds until the period-mean TKE and wall shear are within tol of
-    the reference and stay there for the rest of the run (None if never)."""
+STATS = ("tke", "tau", "cmean")
+
+
+def time_to_reference(hist, ref, tol, hold=2):
+    """GPU seconds (and period) at the end of the first of `hold` consecutive
+    periods whose period means of TKE, wall shear and mean concentration are
+    all within tol (relative) of the reference; None if that never happens."""
     pm = period_means(hist)
-    ok = [abs(p["tke"] - ref["tke"]) <= tol * ref["tke"] and abs(p["tau"] - ref["tau"]) <= tol * ref["tau"]
-          for p in pm]
-    for i in range(len(pm)):
-        if all(ok[i:]):
+    ok = [all(abs(p[k] - ref[k]) <= tol * abs(ref[k]) for k in STATS) for p in pm]
+    for i in range(len(pm) - hold + 1):
+        if all(ok[i:i + hold]):
             return pm[i]["gpu_s"], pm[i]["period"]
     return None, None
 
@@ -96,6 +107,7 @@ def main(argv=None):
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--spin", type=int, default=10)
     ap.add_argument("--sample", type=int, default=20)
+    ap.add_argument("--per-bin", type=int, default=1, help="snapshots per phase bin while sampling")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--eval", type=int, default=12)
     ap.add_argument("--channels", type=int, default=32)
@@ -114,20 +126,22 @@ def main(argv=None):
     run = Run(cfg, uvw0, p0, c0)
     snaps, phases = [], []
     t0 = time.perf_counter()
+    def keep(uvw, c):
+        snaps.append(SG.reduce_state(uvw, c))
+        phases.append(run.phase())
+
     for per in range(args.spin + args.sample):
         for b in range(PHASE_BINS):
-            uvw, c, st = run.advance_bin()
+            sampling = per >= args.spin
+            uvw, c, st = run.advance_bin(args.per_bin if sampling else 1, keep if sampling else None)
             if not st["finite"]:
                 log("spinup: state not finite at period %d" % per)
                 sys.exit(3)
-            if per >= args.spin:
-                snaps.append(SG.reduce_state(uvw, c))
-                phases.append(run.phase())
         pm = period_means(run.hist)[-1]
         log("spinup: reference period %d tke %.4g tau %.4g (%.1f s)" % (per + 1, pm["tke"], pm["tau"],
                                                                         time.perf_counter() - t0))
     ref_hist = run.hist[args.spin * PHASE_BINS:]
-    ref = dict(tke=float(np.mean([h["tke"] for h in ref_hist])), tau=float(np.mean([h["tau"] for h in ref_hist])))
+    ref = {k: float(np.mean([h[k] for h in ref_hist])) for k in STATS}
     result["reference"] = ref
     result["reference_periods"] = period_means(run.hist)
     result["snapshots"] = len(snaps)

@@ -11,12 +11,12 @@ points (the modes |kx|, |ky| < nx_r/2 of the DNS) times nz_r Chebyshev
 Gauss-Lobatto points (the first nz_r Chebyshev coefficients of each
 wall-normal profile).  Fields are [z][y][x][field], bottom first.
 
-Training pairs: target = a DNS snapshot at phase phi; input = the laminar
-Stokes state at phi plus the snapshot's own deviation from its phase mean
-scaled down to the amplitude of the plain start's noise, plus the phase
-(cos phi, sin phi) and z.  At inference the input is the plain start itself
-(laminar + seeded noise), so the network maps weak perturbations of the
-laminar state to a developed state.
+Training pairs: target = a DNS snapshot at phase phi minus the laminar
+Stokes state at phi; input = the snapshot's own deviation from its phase
+mean scaled to the amplitude of the plain start's noise, plus the phase
+(cos phi, sin phi) and z.  At inference the input is the plain start minus
+the laminar state (its noise), and the proposal is the laminar state plus
+the network's output.
 """
 import json
 import math
@@ -159,21 +159,25 @@ class Proposal:
         x[:, :NFIELD] = (x[:, :NFIELD] - self.mean) / self.std
         return x
 
+    # Inputs and targets are deviations from the laminar state at the phase,
+    # normalised per field: the network learns the mean-profile correction and
+    # the fluctuations, not the (known) laminar part.
     def training_pair(self, snap, phase, phase_mean):
         """(input, target) for one snapshot (reduced grid) at `phase`."""
         lam = laminar_grid(self.cfg, phase, self.grid)
         dev = snap - phase_mean
         rms = math.sqrt(float(np.mean(dev[..., :3] ** 2))) + 1e-30
-        x = network_input(lam + dev * (self.noise_rms / rms), phase, self.grid)
-        t = (snap.reshape(-1, NFIELD) - self.mean) / self.std
+        x = network_input(dev * (self.noise_rms / rms), phase, self.grid)
+        t = ((snap - lam).reshape(-1, NFIELD) - self.mean) / self.std
         return self._norm_in(x), t
 
     def fit(self, snaps, phases, epochs=30, lr=2e-3, log=None, seed=0):
         """snaps (n, nz_r, ny_r, nx_r, 4), phases (n,) in radians."""
         snaps = np.asarray(snaps, dtype=np.float64)
         phases = np.asarray(phases, dtype=np.float64)
-        self.mean = snaps.reshape(-1, NFIELD).mean(0)
-        self.std = snaps.reshape(-1, NFIELD).std(0) + 1e-12
+        devs = np.stack([s - laminar_grid(self.cfg, ph, self.grid) for s, ph in zip(snaps, phases)])
+        self.mean = devs.reshape(-1, NFIELD).mean(0)
+        self.std = devs.reshape(-1, NFIELD).std(0) + 1e-12
         # phase means (12 bins) of the snapshots
         bins = np.floor((np.mod(phases, 2 * math.pi)) / (2 * math.pi) * 12).astype(int) % 12
         pmean = np.stack([snaps[bins == b].mean(0) if np.any(bins == b) else snaps.mean(0) for b in range(12)])
@@ -204,9 +208,10 @@ class Proposal:
         return hist
 
     def propose_grid(self, vals_in, phase):
-        x = self._norm_in(network_input(vals_in, phase, self.grid)).astype(np.float32)[None]
+        lam = laminar_grid(self.cfg, phase, self.grid)
+        x = self._norm_in(network_input(vals_in - lam, phase, self.grid)).astype(np.float32)[None]
         y = self.net.forward(x)[0]
-        return (y * self.std + self.mean).reshape(vals_in.shape)
+        return lam + (y * self.std + self.mean).reshape(vals_in.shape)
 
     def propose(self, uvw, c, phase):
         """DNS spectral plain start -> proposed DNS spectral state (not yet
