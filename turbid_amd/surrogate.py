@@ -222,14 +222,9 @@ class Proposal:
         # wall values of the start: no-slip bottom, no flow through the top
         out[0, :, :, :3] = 0.0
         out[-1, :, :, 2] = 0.0
-        u2, c2 = expand_state(out, mx, nyc, nz, self.grid)
-        # fine scales of the start are kept above the reduced modes
-        kmax = min(self.grid["nx"], self.grid["ny"]) // 2 - 1
-        mask = np.ones((mx, nyc), dtype=bool)
-        mask[:kmax + 1, _ky_index(nyc, kmax)] = False
-        u2[:, mask, :] = uvw[:, mask, :]
-        c2[:, mask, :] = c[:, mask, :]
-        return u2, c2
+        # the proposal holds the reduced modes only: the start's noise above
+        # them is not part of a developed state
+        return expand_state(out, mx, nyc, nz, self.grid)
 
     def save(self, path):
         np.savez(path, params=self.net.params(), mean=self.mean, std=self.std,
