@@ -128,6 +128,17 @@ int tbd_zcontract(int nz, long long ncol, int fp32, const double* m, const doubl
  * fix, 1 pressure (src/phase_carrier_flat.cc:861-994). */
 int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x, double* y);
 
+/* Phase-binned statistics on the device (reference log and postprocessing:
+ * solver_channel_flat.cc:372-414 writes the mean-mode wall gradients each
+ * step; the MATLAB companions average profiles by phase).  accumulate adds
+ * the current state's profiles to bin `bin` (0..63): per z, rows
+ * 0..3 mean-mode u, v, w, c; rows 4..7 plane means of u'^2, v'^2, w'^2, c'^2
+ * (sums over the non-mean spectral columns, conjugate half included).
+ * read returns the 8*nz sums of a bin and the number of states added. */
+int tbd_stats_accumulate(tbd_ctx* ctx, int bin);
+int tbd_stats_read(tbd_ctx* ctx, int bin, double* out, long long* count);
+int tbd_stats_reset(tbd_ctx* ctx);
+
 /* Divergence-free projection of the context's velocity, for states that do
  * not come out of a time step (a proposed initial state): the pressure-type
  * solve of FlatCarrierPhase::Correct (src/phase_carrier_flat.cc:586-620)

@@ -231,3 +231,32 @@ def test_graph_replay_matches_eager(precision, monkeypatch):
     assert np.array_equal(pe, pg)
     assert np.array_equal(ce, cg)
     assert np.array_equal(fe, fg)
+
+
+def test_phase_stats_match_host():
+    # on-device phase statistics against the same sums on the downloaded state
+    cfg = small_config()
+    g = RT.Solver(cfg)
+    g.upload(*W.initial_state(cfg))
+    st = RT.Stepper(cfg["dt"])
+    g.stats_reset()
+    ref = np.zeros((8, g.nz))
+    for n in range(3):
+        RT.run_steps(g, st, 1)
+        g.stats_accumulate(5)
+        u, p, c = g.state()
+        w = np.where(np.arange(g.mx) > 0, 2.0, 1.0)[:, None, None]
+        for f in range(3):
+            ref[f] += u[f, 0, 0, :].real
+            fl = u[f].copy()
+            fl[0, 0, :] = 0
+            ref[4 + f] += np.sum(w * np.abs(fl) ** 2, axis=(0, 1))
+        ref[3] += c[0, 0, 0, :].real
+        fc = c[0].copy()
+        fc[0, 0, :] = 0
+        ref[7] += np.sum(w * np.abs(fc) ** 2, axis=(0, 1))
+    got, count = g.stats_read(5)
+    assert count == 3
+    np.testing.assert_allclose(got, ref / 3, rtol=1e-12, atol=1e-14)
+    empty, n0 = g.stats_read(6)
+    assert n0 == 0 and not np.any(empty)

@@ -278,6 +278,25 @@ int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x,
   });
 }
 
+int tbd_stats_accumulate(tbd_ctx* ctx, int bin) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_stats_accumulate > null context");
+    ctx->c->stats_accumulate(bin);
+  });
+}
+int tbd_stats_read(tbd_ctx* ctx, int bin, double* out, long long* count) {
+  return guard([&] {
+    TBD_CHECK(ctx && out, "tbd_stats_read > null argument");
+    ctx->c->stats_read(bin, out, count);
+  });
+}
+int tbd_stats_reset(tbd_ctx* ctx) {
+  return guard([&] {
+    TBD_CHECK(ctx, "tbd_stats_reset > null context");
+    ctx->c->stats_reset();
+  });
+}
+
 int tbd_project(tbd_ctx* ctx) {
   return guard([&] {
     TBD_CHECK(ctx, "tbd_project > null context");

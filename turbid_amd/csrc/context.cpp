@@ -263,6 +263,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   wallmean_ = (double*)alloc(sizeof(double) * 8);
   diagpart_ = (double*)alloc(sizeof(double) * 10 * kDiagMaxBlocks);
   force_ = (double*)alloc(sizeof(double) * 2);
+  stats_ = (double*)alloc(sizeof(double) * kStatBins * kPhaseStats * d.nz);
   if (const char* e = std::getenv("TBD_GRAPH")) use_graphs_ = std::string(e) != "0";
   TBD_HIP(hipStreamSynchronize(st_));
 }
@@ -741,6 +742,27 @@ void Context::substage(int stage, double time, int alternation) {
   if (stage == 2 && cfg.nphase > 1) {
     // the log keeps the scalar values of stage 1 (copied at the carrier's stage 2)
   }
+}
+
+void Context::stats_accumulate(int bin) {
+  TBD_CHECK(bin >= 0 && bin < kStatBins, "Context::stats_accumulate > phase bin out of range");
+  phase_stats(uvw_, cfg.nphase > 1 ? c_ : nullptr, d.S, d.nz, d.nyc, d.P,
+              stats_ + (size_t)bin * kPhaseStats * d.nz, st_);
+  stat_count_[bin] += 1;
+}
+
+void Context::stats_read(int bin, double* out, long long* count) {
+  TBD_CHECK(bin >= 0 && bin < kStatBins, "Context::stats_read > phase bin out of range");
+  sync();
+  TBD_HIP(hipMemcpy(out, stats_ + (size_t)bin * kPhaseStats * d.nz, sizeof(double) * kPhaseStats * d.nz,
+                    hipMemcpyDeviceToHost));
+  if (count) *count = stat_count_[bin];
+}
+
+void Context::stats_reset() {
+  fill_zero(stats_, sizeof(double) * kStatBins * kPhaseStats * d.nz, st_);
+  for (auto& c : stat_count_) c = 0;
+  sync();
 }
 
 // Projection of an arbitrary velocity onto divergence-free fields: the

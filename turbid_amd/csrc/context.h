@@ -54,6 +54,11 @@ class Context {
   void substage(int stage, double time, int alternation);
   void step(double time, int alternation);
   void project();
+  // phase-binned statistics of the current state (on the device)
+  void stats_accumulate(int bin);
+  void stats_read(int bin, double* out, long long* count);
+  void stats_reset();
+  static constexpr int kStatBins = 64;
   void set_dt(double dt);
   void features(double* out50);
   void sync();
@@ -166,6 +171,8 @@ class Context {
   bool use_graphs_ = true;
   bool capturing_ = false;
   double* force_ = nullptr;    // mean-mode force read by the captured H update
+  double* stats_ = nullptr;    // [kStatBins][kPhaseStats][nz] sums
+  long long stat_count_[64] = {};
   void drop_graphs();
 
   std::vector<void*> allocs_;

@@ -74,6 +74,17 @@ void diagnostics(const T* uvw, long long R, int nx, int ny, int nz, double glx, 
 // feat[21] = trapz(gridz, Re c(:,0)) / 2 (phase_scalar_flat.cc:485-486)
 void trapz_mean(const double2* c, const double* gridz, int nz, double* out, hipStream_t st);
 
+// Phase statistics of one state, added into acc[kPhaseStats][nz]:
+//   0..2  mean-mode profiles of u, v, w (real part of column kx = ky = 0)
+//   3     mean-mode profile of c (zero if c is null)
+//   4..6  <u'u'>, <v'v'>, <w'w'>: plane means of the squared fluctuations,
+//         sum over the non-mean columns of |u_hat|^2 (kx > 0 counted twice,
+//         the conjugate half of the plane)
+//   7     <c'c'>
+static constexpr int kPhaseStats = 8;
+void phase_stats(const double2* uvw, const double2* c, long long S, int nz, int nyc, long long P, double* acc,
+                 hipStream_t st);
+
 // small copies: dst[i] = Re(src[idx[i]]) for the log features
 void gather_real(const double2* src, const long long* idx, int n, double* dst, hipStream_t st);
 void fill_zero(void* p, size_t bytes, hipStream_t st);

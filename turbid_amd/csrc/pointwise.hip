@@ -302,6 +302,49 @@ void diagnostics(const T* uvw, long long R, int nx, int ny, int nz, double glx, 
   TBD_HIP(hipGetLastError());
 }
 
+// one workgroup per z: reductions over the P spectral columns of that plane
+__global__ __launch_bounds__(256) void k_phase_stats(const double2* __restrict__ uvw, const double2* __restrict__ c,
+                                                     long long S, int nz, int nyc, long long P, double* acc) {
+  const int z = blockIdx.x;
+  double s[4] = {0, 0, 0, 0};
+  for (long long col = 1 + threadIdx.x; col < P; col += blockDim.x) {
+    const double w = (col / nyc) > 0 ? 2.0 : 1.0;
+    const long long off = col * nz + z;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double2 v = uvw[f * S + off];
+      s[f] += w * (v.x * v.x + v.y * v.y);
+    }
+    if (c) {
+      const double2 v = c[off];
+      s[3] += w * (v.x * v.x + v.y * v.y);
+    }
+  }
+  __shared__ double red[4][4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    double v = s[f];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[f][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int f = threadIdx.x;
+    double v = 0;
+    for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) v += red[f][wv];
+    acc[(long long)(4 + f) * nz + z] += v;
+  } else if (threadIdx.x < 8) {
+    const int f = threadIdx.x - 4;
+    const double m = f < 3 ? uvw[f * S + z].x : (c ? c[z].x : 0.0);
+    acc[(long long)f * nz + z] += m;
+  }
+}
+void phase_stats(const double2* uvw, const double2* c, long long S, int nz, int nyc, long long P, double* acc,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_phase_stats, dim3(nz), dim3(256), 0, st, uvw, c, S, nz, nyc, P, acc);
+  TBD_HIP(hipGetLastError());
+}
+
 __global__ void k_trapz(const double2* __restrict__ c, const double* __restrict__ z, int nz, double* out) {
   __shared__ double part[64];
   double s = 0.0;

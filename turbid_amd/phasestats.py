@@ -1,0 +1,78 @@
+"""Phase-averaged statistics of FP64 and mixed-precision runs of the same
+start, accumulated on the device (tbd_stats_*), and their difference.
+
+    python -m turbid_amd.phasestats --workload turb256 --periods 10 --bins 12
+
+For each precision the run starts from the same seeded state, advances
+`periods` wave periods with the reference's step loop, and at the end of
+each of the `bins` phase bins adds the state's profiles to that bin: mean
+u, v, w, c and the plane means of the squared fluctuations, plus the
+mean-mode wall shear (features 10, 11) on the host.  Reported per
+quantity: the largest difference between the two precisions over bins
+and heights, relative to the largest FP64 magnitude of that quantity.
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import runtime as RT
+from . import workloads as W
+
+NAMES = ["U", "V", "W", "C", "uu", "vv", "ww", "cc"]
+
+
+def run(cfg, periods, bins, log):
+    s = RT.Solver(cfg)
+    s.upload(*W.initial_state(cfg))
+    s.stats_reset()
+    st = RT.Stepper(cfg["dt"])
+    spp = int(round(2 * math.pi / cfg["force_amplitude"][1] / cfg["dt"]))
+    per_bin = spp // bins
+    tau = np.zeros((bins, 2))
+    t0 = time.perf_counter()
+    for p in range(periods):
+        for b in range(bins):
+            feat = RT.run_steps(s, st, per_bin)
+            s.stats_accumulate(b)
+            tau[b] += feat[10:12]
+        log("phasestats: precision %d period %d done (%.1f s)" % (cfg["precision"], p + 1, time.perf_counter() - t0))
+    prof = np.stack([s.stats_read(b)[0] for b in range(bins)])      # (bins, 8, nz)
+    s.close()
+    return prof, tau / periods, time.perf_counter() - t0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="turb256")
+    ap.add_argument("--periods", type=int, default=10)
+    ap.add_argument("--bins", type=int, default=12)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    res = {}
+    for prec in (0, 1):
+        cfg = W.workload(args.workload, precision=prec)
+        res[prec] = run(cfg, args.periods, args.bins, log)
+    p64, t64, s64 = res[0]
+    p32, t32, s32 = res[1]
+    diff = {}
+    for i, n in enumerate(NAMES):
+        scale = np.max(np.abs(p64[:, i, :]))
+        diff[n] = float(np.max(np.abs(p32[:, i, :] - p64[:, i, :])) / scale) if scale > 0 else 0.0
+    scale = np.max(np.abs(t64))
+    diff["wall_shear"] = float(np.max(np.abs(t32 - t64)) / scale)
+    out = dict(workload=args.workload, periods=args.periods, bins=args.bins, seconds_fp64=s64, seconds_mixed=s32,
+               max_relative_difference=diff, max_over_quantities=max(diff.values()))
+    line = json.dumps(out)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(line + "\n")
+    print(line, flush=True)
+
+
+if __name__ == "__main__":
+    main()

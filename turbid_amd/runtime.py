@@ -37,7 +37,7 @@ EXPORTS = [
     "tbd_download_physical_velocity", "tbd_substage", "tbd_step", "tbd_set_dt", "tbd_features",
     "tbd_sync", "tbd_profile_enable", "tbd_profile_read", "tbd_profile_reset", "tbd_get_operators",
     "tbd_get_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
-    "tbd_project", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
+    "tbd_project", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
     "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
     "tbd_bf16_gemm",
 ]
@@ -109,6 +109,9 @@ def lib():
         L.tbd_zcontract.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp, dp, dp]
         L.tbd_helmholtz.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.tbd_project.argtypes = [vp]
+        L.tbd_stats_accumulate.argtypes = [vp, ctypes.c_int]
+        L.tbd_stats_read.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.tbd_stats_reset.argtypes = [vp]
         fp = ctypes.POINTER(ctypes.c_float)
         L.tbd_sno_create.argtypes = [ctypes.c_int] * 8 + [ctypes.POINTER(vp)]
         L.tbd_sno_destroy.argtypes = [vp]
@@ -230,6 +233,19 @@ class Solver:
 
     def set_dt(self, dt):
         _check(lib().tbd_set_dt(self._h, dt))
+
+    def stats_accumulate(self, phase_bin):
+        _check(lib().tbd_stats_accumulate(self._h, int(phase_bin)))
+
+    def stats_read(self, phase_bin):
+        """(8, nz) phase-bin mean: U, V, W, C profiles and <u'^2>, <v'^2>, <w'^2>, <c'^2>."""
+        out = np.zeros(8 * self.nz)
+        n = ctypes.c_longlong()
+        _check(lib().tbd_stats_read(self._h, int(phase_bin), _p(out), ctypes.byref(n)))
+        return out.reshape(8, self.nz) / max(1, n.value), n.value
+
+    def stats_reset(self):
+        _check(lib().tbd_stats_reset(self._h))
 
     def project(self):
         """Make the current velocity divergence-free (proposed initial states)."""
