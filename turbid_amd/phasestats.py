@@ -8,8 +8,10 @@ For each precision the run starts from the same seeded state, advances
 each of the `bins` phase bins adds the state's profiles to that bin: mean
 u, v, w, c and the plane means of the squared fluctuations, plus the
 mean-mode wall shear (features 10, 11) on the host.  Reported per
-quantity: the largest difference between the two precisions over bins
-and heights, relative to the largest FP64 magnitude of that quantity.
+quantity: the largest difference between the two precisions over bins and
+heights, relative to the FP64 scale of its kind: mean velocities to the
+largest |U|, fluctuation variances to the largest u'^2 + v'^2 + w'^2, the
+concentration quantities and the wall shear to their own largest magnitude.
 """
 import argparse
 import json
@@ -60,9 +62,12 @@ def main(argv=None):
     p64, t64, s64 = res[0]
     p32, t32, s32 = res[1]
     diff = {}
+    uscale = np.max(np.abs(p64[:, 0, :]))
+    kscale = np.max(np.abs(p64[:, 4, :] + p64[:, 5, :] + p64[:, 6, :]))
+    scales = [uscale, uscale, uscale, np.max(np.abs(p64[:, 3, :])), kscale, kscale, kscale,
+              np.max(np.abs(p64[:, 7, :]))]
     for i, n in enumerate(NAMES):
-        scale = np.max(np.abs(p64[:, i, :]))
-        diff[n] = float(np.max(np.abs(p32[:, i, :] - p64[:, i, :])) / scale) if scale > 0 else 0.0
+        diff[n] = float(np.max(np.abs(p32[:, i, :] - p64[:, i, :])) / scales[i]) if scales[i] > 0 else 0.0
     scale = np.max(np.abs(t64))
     diff["wall_shear"] = float(np.max(np.abs(t32 - t64)) / scale)
     out = dict(workload=args.workload, periods=args.periods, bins=args.bins, seconds_fp64=s64, seconds_mixed=s32,
@@ -71,6 +76,7 @@ def main(argv=None):
     if args.out:
         with open(args.out, "w") as f:
             f.write(line + "\n")
+        np.savez(args.out.rsplit(".", 1)[0] + ".npz", fp64=p64, mixed=p32, tau_fp64=t64, tau_mixed=t32)
     print(line, flush=True)
 
 
