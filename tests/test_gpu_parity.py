@@ -260,3 +260,16 @@ def test_phase_stats_match_host():
     np.testing.assert_allclose(got, ref / 3, rtol=1e-12, atol=1e-14)
     empty, n0 = g.stats_read(6)
     assert n0 == 0 and not np.any(empty)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_steps_positivity_filter_active(precision):
+    # c = 1 + 2 N(0,1): many negative values, the filter changes every plane
+    # and the transform back runs; with c > 0 elsewhere it is skipped
+    cfg = small_config(noise_c=2.0, precision=precision)
+    o, g = _run_pair(cfg, 2)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    tol = 1e-9 if precision == 0 else 1e-5
+    assert rel_l2(cg, co) < tol
+    assert rel_l2(ug, uo) < tol

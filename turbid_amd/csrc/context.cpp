@@ -93,15 +93,18 @@ void Context::xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int npla
   launch(is32<T>() ? KC_XB32 : KC_XB64, b, [&] { fft_xpass_backward<T>(fft_, tmp, phys, pstride, nv, nplanes, st_); });
 }
 template <typename T>
-void Context::xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes) {
-  const double b = (double)nv * nplanes * d.ny * (d.mx * sizeof(C2<T>) + (double)d.nx * sizeof(T));
-  launch(is32<T>() ? KC_XF32 : KC_XF64, b, [&] { fft_xpass_forward<T>(fft_, phys, pstride, tmp, nv, nplanes, st_); });
+void Context::xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes, const int* gate) {
+  // a gated launch may do nothing: its work is not counted
+  const double b = gate ? 0.0 : (double)nv * nplanes * d.ny * (d.mx * sizeof(C2<T>) + (double)d.nx * sizeof(T));
+  launch(is32<T>() ? KC_XF32 : KC_XF64, b,
+         [&] { fft_xpass_forward<T>(fft_, phys, pstride, tmp, nv, nplanes, st_, gate); });
 }
 template <typename T, typename SpecT>
-void Context::ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol) {
-  const double b = (double)nv * nplanes * d.mx * (d.nyc * sizeof(SpecT) + (double)d.ny * sizeof(C2<T>));
+void Context::ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol,
+                  const int* gate) {
+  const double b = gate ? 0.0 : (double)nv * nplanes * d.mx * (d.nyc * sizeof(SpecT) + (double)d.ny * sizeof(C2<T>));
   launch(is32<T>() ? KC_YF32 : KC_YF64, b,
-         [&] { fft_ypass_forward<T, SpecT>(fft_, tmp, spec, sstride, nv, zoff, nplanes, st_, scol); });
+         [&] { fft_ypass_forward<T, SpecT>(fft_, tmp, spec, sstride, nv, zoff, nplanes, st_, scol, gate); });
 }
 
 void* Context::alloc(size_t bytes) {
@@ -263,6 +266,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   wallmean_ = (double*)alloc(sizeof(double) * 8);
   diagpart_ = (double*)alloc(sizeof(double) * 10 * kDiagMaxBlocks);
   force_ = (double*)alloc(sizeof(double) * 2);
+  filt_ = (int*)alloc(sizeof(int) * 2);
   stats_ = (double*)alloc(sizeof(double) * kStatBins * kPhaseStats * d.nz);
   if (const char* e = std::getenv("TBD_GRAPH")) use_graphs_ = std::string(e) != "0";
   TBD_HIP(hipStreamSynchronize(st_));
@@ -723,8 +727,11 @@ void Context::scalar_correct(int stage, int alternation) {
   }
   // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
   bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_);
-  launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, st_); });
-  fwd<double, double2>(cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_);
+  // the filter raises filt_ when it changes a plane; otherwise the forward
+  // transform back would only reproduce c (up to round-off) and is skipped
+  fill_zero(filt_, sizeof(int) * 2, st_);
+  launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, filt_, st_); });
+  fwd<double, double2>(cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_, -1, filt_);
   launch(KC_OTHER, 0.0, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });
 }
 

@@ -84,14 +84,15 @@ class Context {
   template <typename T>
   void xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes);
   template <typename T>
-  void xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes);
+  void xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes, const int* gate = nullptr);
   template <typename T, typename SpecT>
-  void ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol = -1);
+  void ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol = -1,
+           const int* gate = nullptr);
   template <typename T, typename SpecT>
   void fwd(const T* phys, long long pstride, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
-           C2<T>* tmp, int scol = -1) {
-    xpf<T>(phys, pstride, tmp, nv, nplanes);
-    ypf<T, SpecT>(tmp, spec, sstride, nv, zoff, nplanes, scol);
+           C2<T>* tmp, int scol = -1, const int* gate = nullptr) {
+    xpf<T>(phys, pstride, tmp, nv, nplanes, gate);
+    ypf<T, SpecT>(tmp, spec, sstride, nv, zoff, nplanes, scol, gate);
   }
   template <typename T, typename SpecT>
   void bwd(const SpecT* spec, long long sstride, T* phys, long long pstride, int nv, int zoff, int nplanes,
@@ -171,6 +172,7 @@ class Context {
   bool use_graphs_ = true;
   bool capturing_ = false;
   double* force_ = nullptr;    // mean-mode force read by the captured H update
+  int* filt_ = nullptr;        // set by the positivity filter when it changes c
   double* stats_ = nullptr;    // [kStatBins][kPhaseStats][nz] sums
   long long stat_count_[64] = {};
   void drop_graphs();

@@ -49,10 +49,11 @@ inline long long fft_tmp_elems(const Dims& d, int nv) { return (long long)nv * d
 // physical (nv comps, each nplanes*Ny*Nx, stride phys_stride) -> spectral
 // scol: distance between spectral columns (default nz; 1 for a single row
 // stored compactly, e.g. the wall rows)
+// gate (optional, device int): the forward passes do nothing when *gate == 0
 template <typename T, typename SpecT>
 void fft_forward(const FftPlan& p, const T* phys, long long phys_stride, SpecT* spec,
                  long long spec_stride, int nv, int zoff, int nplanes, C2<T>* tmp,
-                 hipStream_t st, int scol = -1);
+                 hipStream_t st, int scol = -1, const int* gate = nullptr);
 
 // spectral -> physical
 template <typename T, typename SpecT>
@@ -63,10 +64,11 @@ void fft_backward(const FftPlan& p, const SpecT* spec, long long spec_stride, T*
 // Individual passes (exposed for fused callers and tests).
 template <typename T>
 void fft_xpass_forward(const FftPlan& p, const T* phys, long long phys_stride, C2<T>* tmp,
-                       int nv, int nplanes, hipStream_t st);
+                       int nv, int nplanes, hipStream_t st, const int* gate = nullptr);
 template <typename T, typename SpecT>
 void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long spec_stride,
-                       int nv, int zoff, int nplanes, hipStream_t st, int scol = -1);
+                       int nv, int zoff, int nplanes, hipStream_t st, int scol = -1,
+                       const int* gate = nullptr);
 template <typename T, typename SpecT>
 void fft_ypass_backward(const FftPlan& p, const SpecT* spec, long long spec_stride, C2<T>* tmp,
                         int nv, int zoff, int nplanes, hipStream_t st, int scol = -1);

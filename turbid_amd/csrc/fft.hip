@@ -75,7 +75,8 @@ template <> __host__ __device__ inline const float2* twy<float>(const FftPlan& p
 template <typename T, int NT, int MAXB>
 __global__ __launch_bounds__(NT) void k_xpass_fwd(const T* __restrict__ phys, long long pstride,
                                                   C2<T>* __restrict__ tmp, FftLen L, const C2<T>* twg,
-                                                  int ny, int mx, int nplanes, int nseq) {
+                                                  int ny, int mx, int nplanes, int nseq, const int* gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ char smem_raw[];
   C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
   const int N = L.n;
@@ -162,7 +163,9 @@ __global__ __launch_bounds__(NT) void k_xpass_bwd(const C2<T>* __restrict__ tmp,
 template <typename T, typename SpecT, int NT, int MAXB>
 __global__ __launch_bounds__(NT) void k_ypass_fwd(const C2<T>* __restrict__ tmp, SpecT* __restrict__ spec,
                                                   long long sstride, FftLen L, const C2<T>* twg, int nz,
-                                                  int mx, int nyc, int nplanes, int zoff, int zt, T scale) {
+                                                  int mx, int nyc, int nplanes, int zoff, int zt, T scale,
+                                                  const int* gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ char smem_raw[];
   C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
   const int N = L.n;
@@ -253,7 +256,8 @@ __host__ __device__ constexpr int lds2_elems(bool with_rows) {
 template <typename T, typename SpecT, int N1, int N2, int NT>
 __global__ __launch_bounds__(NT) void k_ypf2(const C2<T>* __restrict__ tmp, SpecT* __restrict__ spec,
                                              long long sstride, const C2<T>* __restrict__ tw, int scol, int mx,
-                                             int nyc, int nplanes, int zoff, T scale) {
+                                             int nyc, int nplanes, int zoff, T scale, const int* gate) {
+  if (gate && *gate == 0) return;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(false)];
@@ -342,7 +346,9 @@ __global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, lon
 // x pass, forward: rows (y0 .. y0+2S) of plane pl -> T[pl][kx][y]
 template <typename T, int N1, int N2, int NT>
 __global__ __launch_bounds__(NT) void k_xpf2(const T* __restrict__ phys, long long pstride, C2<T>* __restrict__ tmp,
-                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes) {
+                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes,
+                                             const int* gate) {
+  if (gate && *gate == 0) return;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(true)];
@@ -458,7 +464,7 @@ template <> struct Nt2<float> { static constexpr int v = 256; };
 
 template <typename T, typename SpecT>
 static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
-                 hipStream_t st, int scol) {
+                 hipStream_t st, int scol, const int* gate) {
   constexpr int NT = Nt2<T>::v;
   const T scale = (T)(1.0 / ((double)p.d.nx * (double)p.d.ny));
   const int sc = scol > 0 ? scol : p.d.nz;
@@ -468,7 +474,7 @@ static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstr
     constexpr int S = seqs2<A, B, NT>();                                                                    \
     dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);                                                            \
     hipLaunchKernelGGL((k_ypf2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, tmp, spec, sstride, twy<T>(p), sc, \
-                       p.d.mx, p.d.nyc, nplanes, zoff, scale);                                              \
+                       p.d.mx, p.d.nyc, nplanes, zoff, scale, gate);                                        \
     return true;                                                                                            \
   }
     TBD_FOR_LENGTHS(TBD_CASE)
@@ -501,7 +507,7 @@ static bool ypb2(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* 
 
 template <typename T>
 static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
-                 hipStream_t st) {
+                 hipStream_t st, const int* gate) {
   constexpr int NT = Nt2<T>::v;
   switch (p.d.nx) {
 #define TBD_CASE(NN, A, B)                                                                              \
@@ -509,7 +515,7 @@ static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp,
     constexpr int S = seqs2<A, B, NT>();                                                                \
     dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);                                                   \
     hipLaunchKernelGGL((k_xpf2<T, A, B, NT>), grid, dim3(NT), 0, st, phys, pstride, tmp, twx<T>(p), p.d.ny, \
-                       p.d.mx, nplanes);                                                                \
+                       p.d.mx, nplanes, gate);                                                          \
     return true;                                                                                        \
   }
     TBD_FOR_LENGTHS(TBD_CASE)
@@ -566,8 +572,8 @@ static size_t lds_bytes(int n, int nseq) {
 
 template <typename T>
 void fft_xpass_forward(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
-                       hipStream_t st) {
-  if (p.two_level && xpf2<T>(p, phys, pstride, tmp, nv, nplanes, st)) {
+                       hipStream_t st, const int* gate) {
+  if (p.two_level && xpf2<T>(p, phys, pstride, tmp, nv, nplanes, st, gate)) {
     TBD_HIP(hipGetLastError());
     return;
   }
@@ -575,7 +581,7 @@ void fft_xpass_forward(const FftPlan& p, const T* phys, long long pstride, C2<T>
   const int nseq = std::max(1, std::min(seq_per_block<T>(n), (p.d.ny + 1) / 2));
   dim3 grid(ceil_div(p.d.ny, 2 * nseq), nv * nplanes);
   hipLaunchKernelGGL((k_xpass_fwd<T, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, nseq), st, phys,
-                     pstride, tmp, p.lx, twx<T>(p), p.d.ny, p.d.mx, nplanes, nseq);
+                     pstride, tmp, p.lx, twx<T>(p), p.d.ny, p.d.mx, nplanes, nseq, gate);
   TBD_HIP(hipGetLastError());
 }
 
@@ -596,8 +602,8 @@ void fft_xpass_backward(const FftPlan& p, const C2<T>* tmp, T* phys, long long p
 
 template <typename T, typename SpecT>
 void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff,
-                       int nplanes, hipStream_t st, int scol) {
-  if (p.two_level && ypf2<T, SpecT>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol)) {
+                       int nplanes, hipStream_t st, int scol, const int* gate) {
+  if (p.two_level && ypf2<T, SpecT>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol, gate)) {
     TBD_HIP(hipGetLastError());
     return;
   }
@@ -606,7 +612,8 @@ void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long lon
   dim3 grid(ceil_div(nplanes, zt), p.d.mx, nv);
   const T scale = (T)(1.0 / ((double)p.d.nx * (double)p.d.ny));
   hipLaunchKernelGGL((k_ypass_fwd<T, SpecT, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, zt), st, tmp, spec,
-                     sstride, p.ly, twy<T>(p), scol > 0 ? scol : p.d.nz, p.d.mx, p.d.nyc, nplanes, zoff, zt, scale);
+                     sstride, p.ly, twy<T>(p), scol > 0 ? scol : p.d.nz, p.d.mx, p.d.nyc, nplanes, zoff, zt, scale,
+                     gate);
   TBD_HIP(hipGetLastError());
 }
 
@@ -627,9 +634,9 @@ void fft_ypass_backward(const FftPlan& p, const SpecT* spec, long long sstride, 
 
 template <typename T, typename SpecT>
 void fft_forward(const FftPlan& p, const T* phys, long long pstride, SpecT* spec, long long sstride, int nv,
-                 int zoff, int nplanes, C2<T>* tmp, hipStream_t st, int scol) {
-  fft_xpass_forward<T>(p, phys, pstride, tmp, nv, nplanes, st);
-  fft_ypass_forward<T, SpecT>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol);
+                 int zoff, int nplanes, C2<T>* tmp, hipStream_t st, int scol, const int* gate) {
+  fft_xpass_forward<T>(p, phys, pstride, tmp, nv, nplanes, st, gate);
+  fft_ypass_forward<T, SpecT>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol, gate);
 }
 
 template <typename T, typename SpecT>
@@ -641,19 +648,21 @@ void fft_backward(const FftPlan& p, const SpecT* spec, long long sstride, T* phy
 
 #define TBD_FFT_INST(T, S)                                                                              \
   template void fft_forward<T, S>(const FftPlan&, const T*, long long, S*, long long, int, int, int,  \
-                                  C2<T>*, hipStream_t, int);                                          \
+                                  C2<T>*, hipStream_t, int, const int*);                              \
   template void fft_backward<T, S>(const FftPlan&, const S*, long long, T*, long long, int, int, int, \
                                    C2<T>*, hipStream_t, int);                                         \
   template void fft_ypass_forward<T, S>(const FftPlan&, const C2<T>*, S*, long long, int, int, int,   \
-                                        hipStream_t, int);                                            \
+                                        hipStream_t, int, const int*);                                \
   template void fft_ypass_backward<T, S>(const FftPlan&, const S*, long long, C2<T>*, int, int, int,  \
                                          hipStream_t, int);
 
 TBD_FFT_INST(double, double2)
 TBD_FFT_INST(float, float2)
 TBD_FFT_INST(float, double2)
-template void fft_xpass_forward<double>(const FftPlan&, const double*, long long, double2*, int, int, hipStream_t);
-template void fft_xpass_forward<float>(const FftPlan&, const float*, long long, float2*, int, int, hipStream_t);
+template void fft_xpass_forward<double>(const FftPlan&, const double*, long long, double2*, int, int, hipStream_t,
+                                        const int*);
+template void fft_xpass_forward<float>(const FftPlan&, const float*, long long, float2*, int, int, hipStream_t,
+                                       const int*);
 template void fft_xpass_backward<double>(const FftPlan&, const double2*, double*, long long, int, int, hipStream_t);
 template void fft_xpass_backward<float>(const FftPlan&, const float2*, float*, long long, int, int, hipStream_t);
 

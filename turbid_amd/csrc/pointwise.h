@@ -61,7 +61,8 @@ void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re
                  hipStream_t st);
 
 // Bartnicki positivity filter, 5 iterations per z-plane (phase_scalar_flat.cc:427-475)
-void bartnicki_filter(double* c, int nx, int ny, int nz, hipStream_t st);
+// touched (optional, device int): set to 1 when any plane is changed
+void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st);
 
 // Diagnostics (phase_carrier_flat.cc:639-671): feat[0..8] max |u|,|v|,|w| over
 // all / bottom / top planes, feat[19] max(|u|/dx + |v|/dy + |w|/dz(z)).

@@ -187,7 +187,7 @@ void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re
 // One workgroup per z-plane; the plane statistics are FP64 sums.  A plane
 // with no negative entries is left untouched after the first reduction (each
 // further pass would add 0 to the positive entries and zero the zeros).
-__global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long long plane) {
+__global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long long plane, int* touched) {
   __shared__ double red[4][16];
   double* pl = c + (long long)blockIdx.x * plane;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -212,6 +212,7 @@ __global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long lo
     for (int w = 0; w < nw; ++w) { P += red[0][w]; NP += red[1][w]; N += red[2][w]; NN += red[3][w]; }
     __syncthreads();
     if (NN == 0) break;                 // uniform across the block
+    if (touched && threadIdx.x == 0) *touched = 1;
     if (P + 0.0 + N < 0) {
       for (long long e = threadIdx.x; e < plane; e += blockDim.x) pl[e] = 0;
     } else {
@@ -224,8 +225,8 @@ __global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long lo
     __syncthreads();
   }
 }
-void bartnicki_filter(double* c, int nx, int ny, int nz, hipStream_t st) {
-  hipLaunchKernelGGL(k_filter, dim3(nz), dim3(1024), 0, st, c, (long long)nx * ny);
+void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter, dim3(nz), dim3(1024), 0, st, c, (long long)nx * ny, touched);
   TBD_HIP(hipGetLastError());
 }
 
