@@ -70,6 +70,28 @@ def test_dft_roundtrip_bandlimited():
     assert rel_l2(O.dft_forward(back), uvw) < 1e-14
 
 
+def test_dft_roundtrip_projects_kx0_plane():
+    # a general spectrum comes back from backward-then-forward with only its
+    # kx = 0 plane changed: made Hermitian in ky, the unpaired ky = -nyc/2 row
+    # halved (its +nyc/2 partner is not retained).  The solver applies this
+    # projection directly when the positivity filter leaves c untouched.
+    nx, ny, nz = 24, 18, 5
+    _, nyc, mx = O.dims(nx, ny, nz)
+    rng = np.random.default_rng(3)
+    s = rng.standard_normal((1, mx, nyc, nz)) + 1j * rng.standard_normal((1, mx, nyc, nz))
+    back = O.dft_forward(O.dft_backward(s, nx, ny))
+    ky = np.fft.fftfreq(nyc, 1.0 / nyc).astype(int)
+    plane = s[0, 0]
+    expect = s.copy()
+    for iy, k in enumerate(ky):
+        if k == -(nyc // 2):
+            expect[0, 0, iy] = 0.5 * plane[iy]
+        else:
+            expect[0, 0, iy] = 0.5 * (plane[iy] + np.conj(plane[list(ky).index(-k)]))
+    np.testing.assert_allclose(back, expect, atol=1e-13)
+    assert np.abs(back[0, 1:] - s[0, 1:]).max() < 1e-13
+
+
 def test_dealiasing_three_halves():
     # the product of two retained modes transformed on the 3/2 grid carries no
     # aliased energy on the retained modes: compare with the exact product

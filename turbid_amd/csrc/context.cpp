@@ -727,11 +727,13 @@ void Context::scalar_correct(int stage, int alternation) {
   }
   // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
   bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_);
-  // the filter raises filt_ when it changes a plane; otherwise the forward
-  // transform back would only reproduce c (up to round-off) and is skipped
+  // the filter raises filt_ when it changes a plane; otherwise the transform
+  // back would reproduce c except for the kx = 0 plane's projection, which is
+  // applied directly
   fill_zero(filt_, sizeof(int) * 2, st_);
   launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, filt_, st_); });
   fwd<double, double2>(cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_, -1, filt_);
+  launch(KC_OTHER, 0.0, [&] { kx0_round_trip(c_, d.nyc, d.nz, filt_, st_); });
   launch(KC_OTHER, 0.0, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });
 }
 

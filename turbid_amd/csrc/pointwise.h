@@ -64,6 +64,13 @@ void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re
 // touched (optional, device int): set to 1 when any plane is changed
 void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st);
 
+// What DftForward(DftBackward(s)) does to a truncated spectrum: the
+// halfcomplex inverse keeps the real part of the kx = 0 sequences, so the
+// kx = 0 plane becomes Hermitian in ky, s(ky) <- (s(ky) + conj s(-ky))/2, with
+// s(+nyc/2) = 0 (not retained); every other column is reproduced.  Applied
+// only when *gate == 0.
+void kx0_round_trip(double2* s, int nyc, int nz, const int* gate, hipStream_t st);
+
 // Diagnostics (phase_carrier_flat.cc:639-671): feat[0..8] max |u|,|v|,|w| over
 // all / bottom / top planes, feat[19] max(|u|/dx + |v|/dy + |w|/dz(z)).
 // part: kDiagMaxBlocks * 10 doubles of scratch for the per-block maxima.

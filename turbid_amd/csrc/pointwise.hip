@@ -230,6 +230,31 @@ void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream
   TBD_HIP(hipGetLastError());
 }
 
+__global__ void k_kx0_round_trip(double2* __restrict__ s, int nyc, int nz, const int* gate) {
+  if (gate && *gate != 0) return;
+  const int half = nyc / 2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (half + 1) * nz; i += gridDim.x * blockDim.x) {
+    const int iy = i / nz, z = i - iy * nz;
+    double2* a = s + (long long)iy * nz + z;
+    if (iy == 0) {
+      a->y = 0.0;
+    } else if (iy == half) {          // ky = -nyc/2: its partner +nyc/2 is not retained
+      a->x = 0.5 * a->x;
+      a->y = 0.5 * a->y;
+    } else {
+      double2* b = s + (long long)(nyc - iy) * nz + z;
+      const double2 av = *a, bv = *b;
+      const double hx = 0.5 * (av.x + bv.x), hy = 0.5 * (av.y - bv.y);
+      *a = make_double2(hx, hy);
+      *b = make_double2(hx, -hy);
+    }
+  }
+}
+void kx0_round_trip(double2* s, int nyc, int nz, const int* gate, hipStream_t st) {
+  hipLaunchKernelGGL(k_kx0_round_trip, dim3(((nyc / 2 + 1) * nz + 255) / 256), dim3(256), 0, st, s, nyc, nz, gate);
+  TBD_HIP(hipGetLastError());
+}
+
 // Max-norm and CFL diagnostics: per-block partial maxima (z planes strided
 // over blockIdx.y, plane points over blockIdx.x), then one block reduces the
 // partials.  No atomics, so the result does not depend on scheduling.
