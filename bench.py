@@ -54,6 +54,7 @@ KERNELS = {
     "zgemm64": ("k_zgemm<double,...>", "flops", MFMA_PEAK["fp64"]),
     "zgemm32": ("k_zgemm<float,...>", "flops", MFMA_PEAK["fp32"]),
     "helmholtz": ("k_helmholtz<1>", "flops", MFMA_PEAK["fp64"]),
+    "helmholtz_split": ("k_helmholtz_split<1>", "flops", MFMA_PEAK["fp64"]),
     "products64": ("k_products_*<double>", "bytes", HBM_PEAK),
     "products32": ("k_products_*<float>", "bytes", HBM_PEAK),
     "other": ("(boundary rows, H update, filter, diagnostics, copies)", None, None),
@@ -124,6 +125,20 @@ def roofline(prof):
     out["launches"] = n
     out["avg_launch_ms"] = ms / n
     out["work_per_launch"] = per_launch
+    return out
+
+
+def class_rates(prof):
+    """Achieved rate and fraction of peak of every kernel class with a
+    roofline (algorithmic bytes or flops per launch over its average time)."""
+    out = {}
+    for name, (sym, kind, peak) in KERNELS.items():
+        n, ms, work = prof[name]
+        if not kind or n == 0 or ms <= 0:
+            continue
+        rate = work / (ms / 1e3)
+        unit, scale = ("GB/s", 1e9) if kind == "bytes" else ("TFLOP/s", 1e12)
+        out[name] = {"achieved": round(rate / scale, 2), "unit": unit, "frac": round(rate / peak, 3)}
     return out
 
 
@@ -228,6 +243,7 @@ def main():
     line["kernel_ms_per_step"] = {prec: {k: round(v[1] / args.steps, 4) for k, v in r["prof"].items() if v[0]}
                                   for prec, r in results.items()}
     line["profiled_ms_per_step"] = {prec: r["profiled_ms_per_step"] for prec, r in results.items()}
+    line["class_rates"] = {prec: class_rates(r["prof"]) for prec, r in results.items()}
     if rank == 0 and not args.no_cpu_baseline:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("bench: cpu baseline (oracle, %d threads)" % nthreads)

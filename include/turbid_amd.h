@@ -101,9 +101,13 @@ int tbd_sync(tbd_ctx* ctx);
  *   12    Helmholtz / pressure solves (FP64)
  *   13, 14 pointwise nonlinear products (FP64, FP32)
  *   15    everything else
+ *   16    Helmholtz / pressure solves through the even/odd form of the
+ *         factorisation (FP64; fields with the same boundary type at both walls)
  * work: algorithmic work of the timed launches -- bytes read plus written
- * (classes 0..9, 13, 14) or flops, 4*nz^2*P per contraction of one component
- * (classes 10..12; a Helmholtz solve is two contractions). */
+ * (classes 0..9, 13, 14) or the flops executed: 4*nz^2*P per contraction of
+ * one component with the full operator, 4*((M+1)^2 + (M+1)*M)*P (nz = 2M+1)
+ * through the even/odd split (classes 10, 11); a Helmholtz solve is two
+ * contractions (class 12; class 16: 4*(2(M+1)^2 + M^2 + (M+1)*M)*P). */
 int tbd_profile_enable(tbd_ctx* ctx, int enable);
 int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* work);
 int tbd_profile_reset(tbd_ctx* ctx);

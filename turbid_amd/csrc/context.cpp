@@ -81,6 +81,38 @@ template <typename T> static constexpr bool is32() { return std::is_same<T, floa
 // operator times complex columns); each pass moves its input and its output
 static double zflops(const Dims& d) { return 4.0 * d.nz * (double)d.nz * d.P; }
 
+// flops one contraction of one field executes: the even/odd split applies an
+// (M+1) x (M+1) and an (M+1) x M block instead of the nz x nz operator
+double Context::zwork() const {
+  if (!parity_) return zflops(d);
+  const double M = (d.nz - 1) / 2;
+  return 4.0 * ((M + 1) * (M + 1) + (M + 1) * M) * (double)d.P;
+}
+
+void Context::solve(int mode, int n, const int* k, const double* sigma, const double2* const* x,
+                    double2* const* y) {
+  HArgs h;
+  h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = mode; h.kx = kx_; h.ky = ky_;
+  int nplain = 0;
+  for (int i = 0; i < n; ++i) {
+    const int f = k[i];
+    const int nullrow = sigma[i] == 0.0 ? (hsplit_[f] ? fac[f].pnull : fac[f].nullrow) : -1;
+    if (hsplit_[f]) {
+      HArgs hs = h;
+      hs.lda = pld_[f];
+      HField& q = hs.f[0];
+      q.vinv = pinv_[f]; q.vnorm = pnorm_[f]; q.lambda = plam_[f]; q.fix = fix_[f]; q.pfix = pfix_[f];
+      q.sigma = sigma[i]; q.nullrow = nullrow; q.x = x[i]; q.y = y[i];
+      launch(KC_HELMS, helmholtz_split_flops(d.nz, d.P), [&] { helmholtz_split(hs, st_); });
+    } else {
+      HField& q = h.f[nplain++];
+      q.vinv = vinv_[f]; q.vnorm = vnorm_[f]; q.lambda = lam_[f]; q.fix = fix_[f];
+      q.sigma = sigma[i]; q.nullrow = nullrow; q.x = x[i]; q.y = y[i];
+    }
+  }
+  if (nplain > 0) launch(KC_HELM, 2 * nplain * zflops(d), [&] { helmholtz(h, nplain, st_); });
+}
+
 template <typename T, typename SpecT>
 void Context::ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol) {
   const double b = (double)nv * nplanes * d.mx * (d.nyc * sizeof(SpecT) + (double)d.ny * sizeof(C2<T>));
@@ -194,11 +226,20 @@ Context::Context(const tbd_config& c) : cfg(c) {
       TBD_HIP(hipMemcpy(d1pf_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
     }
   }
+  const char* nps = std::getenv("TBD_NO_PARITY");
   for (int k = 0; k < (cfg.nphase > 1 ? 5 : 4); ++k) {
     vinv_[k] = up(rowmajor(fac[k].vinv, nz, lda_));
     vnorm_[k] = up(rowmajor(fac[k].vnorm, nz, lda_));
     lam_[k] = up(fac[k].lambda);
     fix_[k] = up(fac[k].fix);
+    hsplit_[k] = fac[k].pM > 0 && !(nps && nps[0] == '1');
+    if (hsplit_[k]) {
+      pld_[k] = fac[k].pld;
+      pinv_[k] = up(fac[k].pinv);
+      pnorm_[k] = up(fac[k].pnorm);
+      plam_[k] = up(fac[k].plam);
+      pfix_[k] = up(fac[k].pfix);
+    }
   }
   // Fourier multipliers with the reference's sign at iy = nyc/2 (stepper_space.cc:181-213)
   std::vector<double> kx(d.mx), ky(d.nyc);
@@ -467,7 +508,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     ZArgs a;
     a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = uvw_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
-    launch(is32<T>() ? KC_Z32 : KC_Z64, 3 * zflops(d), [&] { zgemm<T, double2, CT>(a, 3, st_); });
+    launch(is32<T>() ? KC_Z32 : KC_Z64, 3 * zwork(), [&] { zgemm<T, double2, CT>(a, 3, st_); });
     const bool erosion_planes = cfg.nphase > 1 && cfg.scalar_bc == 1;
     if (fused_) {
       // y^-1 of the nine gradients; x^-1, u.grad u_i, x in one kernel; y
@@ -506,7 +547,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
       a.x = prod + rows[f][2] * S; a.e0 = prod + rows[f][0] * S; a.e1 = prod + rows[f][1] * S;
       a.y = adv + f * S; a.kx = kx_; a.ky = ky_;
-      launch(is32<T>() ? KC_Z32 : KC_Z64, zflops(d), [&] { zgemm<T, CT, CT>(a, 1, st_); });
+      launch(is32<T>() ? KC_Z32 : KC_Z64, zwork(), [&] { zgemm<T, CT, CT>(a, 1, st_); });
     }
     // Gradient(u), Gradient(v) + DftBackward for the erosion map: only the
     // bottom plane of du/dz, dv/dz is read (phase_scalar_flat.cc:349-351)
@@ -536,7 +577,7 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
     a.x = uvw_; a.xs = S; a.y = rhs_; a.ys = S; a.h = huvw_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
-    launch(KC_Z64, 3 * zflops(d), [&] { zgemm<double, double2, double2>(a, 3, st_); });
+    launch(KC_Z64, 3 * zwork(), [&] { zgemm<double, double2, double2>(a, 3, st_); });
   }
   // boundary rows from the extrapolated wall pressure gradient (boundary_carrier_flat.cc:185-246)
   {
@@ -560,16 +601,12 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   }
   // intermediate velocity: three Helmholtz solves (:554, :867-931)
   {
-    HArgs h;
-    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 0; h.kx = kx_; h.ky = ky_;
-    for (int f = 0; f < 3; ++f) {
-      h.f[f].vinv = vinv_[f]; h.f[f].vnorm = vnorm_[f]; h.f[f].lambda = lam_[f]; h.f[f].fix = fix_[f];
-      h.f[f].sigma = -cfg.reynolds / dt_ / coef_[2][stage];
-      h.f[f].nullrow = -1;
-      h.f[f].x = rhs_ + f * S;
-      h.f[f].y = suvw_ + f * S;
-    }
-    launch(KC_HELM, 6 * zflops(d), [&] { helmholtz(h, 3, st_); });
+    const int k[3] = {0, 1, 2};
+    const double sg = -cfg.reynolds / dt_ / coef_[2][stage];
+    const double sigma[3] = {sg, sg, sg};
+    const double2* x[3] = {rhs_, rhs_ + S, rhs_ + 2 * S};
+    double2* y[3] = {suvw_, suvw_ + S, suvw_ + 2 * S};
+    solve(0, 3, k, sigma, x, y);
   }
 }
 
@@ -586,18 +623,15 @@ void Context::carrier_correct(int stage) {
     a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
     a.x = suvw_ + 2 * S; a.e0 = suvw_; a.e1 = suvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
     a.q = 0.5 / dt_ / c3;
-    launch(KC_Z64, zflops(d), [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_Z64, zwork(), [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // pressure solve (:594, :955-987)
   {
-    HArgs h;
-    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 1; h.kx = kx_; h.ky = ky_;
-    h.f[0].vinv = vinv_[3]; h.f[0].vnorm = vnorm_[3]; h.f[0].lambda = lam_[3]; h.f[0].fix = fix_[3];
-    h.f[0].sigma = 0.0;
-    h.f[0].nullrow = fac[3].nullrow;
-    h.f[0].x = rhsp_;
-    h.f[0].y = p_;
-    launch(KC_HELM, 2 * zflops(d), [&] { helmholtz(h, 1, st_); });
+    const int k = 3;
+    const double sigma = 0.0;
+    const double2* x = rhsp_;
+    double2* y = p_;
+    solve(1, 1, &k, &sigma, &x, &y);
   }
   // grad p, boundary Update (m2 <- m1, m1 <- walls), u = s - 2 dt c3 grad p,
   // mean w := 0 (:597-620, boundary_carrier_flat.cc:131-170)
@@ -611,7 +645,7 @@ void Context::carrier_correct(int stage) {
     a.wall_bot = wall_[wslot_][0]; a.wall_top = wall_[wslot_][1];
     a.wall2_bot = wall2_[wslot_][0]; a.wall2_top = wall2_[wslot_][1];
     a.open_top = opentop_ ? 1 : 0;
-    launch(KC_Z64, zflops(d), [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_Z64, zwork(), [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // uvwnonlinear = DftBackward(uvw) (:625-626)
   bwd<T, double2>(uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_);
@@ -662,12 +696,12 @@ void Context::scalar_correct(int stage, int alternation) {
     ZArgs a;
     a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = prod + 2 * S; a.e0 = prod; a.e1 = prod + S; a.y = advc; a.kx = kx_; a.ky = ky_;
-    launch(is32<T>() ? KC_Z32 : KC_Z64, zflops(d), [&] { zgemm<T, CT, CT>(a, 1, st_); });
+    launch(is32<T>() ? KC_Z32 : KC_Z64, zwork(), [&] { zgemm<T, CT, CT>(a, 1, st_); });
   } else {             // convective form (:584-592)
     ZArgs a;
     a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = c_; a.xs = S; a.y = grad + 6 * S; a.ys = S; a.kx = kx_; a.ky = ky_;
-    launch(is32<T>() ? KC_Z32 : KC_Z64, zflops(d), [&] { zgemm<T, double2, CT>(a, 1, st_); });
+    launch(is32<T>() ? KC_Z32 : KC_Z64, zwork(), [&] { zgemm<T, double2, CT>(a, 1, st_); });
     if (fused_) {
       ypb<T, CT>(grad + 6 * S, S, tmp, 3, 0, d.nz);
       launch(is32<T>() ? KC_XFU32 : KC_XFU64, 4 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] {
@@ -693,7 +727,7 @@ void Context::scalar_correct(int stage, int alternation) {
     a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
     a.x = c_; a.xs = S; a.y = rhsc_; a.ys = S; a.h = hc_; a.hs = S; a.kx = kx_; a.ky = ky_;
     a.s1 = redtc3; a.s2 = redtc3 * c2;
-    launch(KC_Z64, zflops(d), [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    launch(KC_Z64, zwork(), [&] { zgemm<double, double2, double2>(a, 1, st_); });
   }
   // boundary rows + erosion/deposition (boundary_scalar_flat.cc:151-152, :340-384)
   const int erosion = cfg.scalar_bc == 1 ? 1 : 0;
@@ -716,14 +750,11 @@ void Context::scalar_correct(int stage, int alternation) {
   }
   // Helmholtz (:399, :640-661)
   {
-    HArgs h;
-    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 0; h.kx = kx_; h.ky = ky_;
-    h.f[0].vinv = vinv_[4]; h.f[0].vnorm = vnorm_[4]; h.f[0].lambda = lam_[4]; h.f[0].fix = fix_[4];
-    h.f[0].sigma = -cfg.reynolds * cfg.schmidt / dt_ / coef_[2][stage];
-    h.f[0].nullrow = -1;
-    h.f[0].x = rhsc_;
-    h.f[0].y = c_;
-    launch(KC_HELM, 2 * zflops(d), [&] { helmholtz(h, 1, st_); });
+    const int k = 4;
+    const double sigma = -cfg.reynolds * cfg.schmidt / dt_ / coef_[2][stage];
+    const double2* x = rhsc_;
+    double2* y = c_;
+    solve(0, 1, &k, &sigma, &x, &y);
   }
   // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
   bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_);
@@ -787,14 +818,11 @@ void Context::project() {
     zgemm<double, double2, double2>(a, 1, st_);
   }
   {
-    HArgs h;
-    h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = 1; h.kx = kx_; h.ky = ky_;
-    h.f[0].vinv = vinv_[3]; h.f[0].vnorm = vnorm_[3]; h.f[0].lambda = lam_[3]; h.f[0].fix = fix_[3];
-    h.f[0].sigma = 0.0;
-    h.f[0].nullrow = fac[3].nullrow;
-    h.f[0].x = rhsp_;
-    h.f[0].y = rhsp_;
-    helmholtz(h, 1, st_);
+    const int k = 3;
+    const double sigma = 0.0;
+    const double2* x = rhsp_;
+    double2* y = rhsp_;
+    solve(1, 1, &k, &sigma, &x, &y);
   }
   {
     ZArgs a;
@@ -825,17 +853,11 @@ void Context::helmholtz_host(int which, int stage, int mode, const double* x, do
   double2* dx = (double2*)alloc(S16);
   double2* dy = (double2*)alloc(S16);
   TBD_HIP(hipMemcpyAsync(dx, x, S16, hipMemcpyHostToDevice, st_));
-  HArgs h;
-  h.nz = d.nz; h.lda = lda_; h.P = (int)d.P; h.nyc = d.nyc; h.mode = mode; h.kx = kx_; h.ky = ky_;
-  h.f[0].vinv = vinv_[which]; h.f[0].vnorm = vnorm_[which]; h.f[0].lambda = lam_[which]; h.f[0].fix = fix_[which];
   double sig = 0.0;
   if (which < 3) sig = -cfg.reynolds / dt_ / coef_[2][stage];
   else if (which == 4) sig = -cfg.reynolds * cfg.schmidt / dt_ / coef_[2][stage];
-  h.f[0].sigma = sig;
-  h.f[0].nullrow = which == 3 ? fac[3].nullrow : -1;
-  h.f[0].x = dx;
-  h.f[0].y = dy;
-  helmholtz(h, 1, st_);
+  const double2* xx = dx;
+  solve(mode, 1, &which, &sig, &xx, &dy);
   TBD_HIP(hipMemcpyAsync(y, dy, S16, hipMemcpyDeviceToHost, st_));
   sync();
   // temporary buffers are released with the context

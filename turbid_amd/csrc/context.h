@@ -24,6 +24,7 @@ enum KClass {
   KC_HELM,                                    // Helmholtz / pressure solves
   KC_PROD64, KC_PROD32,                       // pointwise nonlinear products
   KC_OTHER,
+  KC_HELMS,                                   // Helmholtz / pressure solves through the even/odd form
   KC_N
 };
 
@@ -78,6 +79,12 @@ class Context {
   template <typename T> void scalar_correct(int stage, int alternation);
   void forces(double time, double& f1, double& f2) const;
   template <class F> void launch(int kclass, double work, F&& f);
+  double zwork() const;
+  // Helmholtz solves of n <= 3 fields, field i with factorisation k[i] and
+  // stage coefficient sigma[i]: fields whose factorisation has an even/odd
+  // form go through helmholtz_split (one launch each), the rest share one
+  // helmholtz launch
+  void solve(int mode, int n, const int* k, const double* sigma, const double2* const* x, double2* const* y);
   // profiled transform passes (one launch each)
   template <typename T, typename SpecT>
   void ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol = -1);
@@ -124,6 +131,13 @@ class Context {
   double* vnorm_[5] = {nullptr};
   double* lam_[5] = {nullptr};
   double* fix_[5] = {nullptr};
+  // even/odd form of the factorisations that have one (hsplit_)
+  bool hsplit_[5] = {false};
+  int pld_[5] = {0};
+  double* pinv_[5] = {nullptr};
+  double* pnorm_[5] = {nullptr};
+  double* plam_[5] = {nullptr};
+  double* pfix_[5] = {nullptr};
   double* kx_ = nullptr;
   double* ky_ = nullptr;
   double* gridz_ = nullptr;

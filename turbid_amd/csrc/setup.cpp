@@ -377,6 +377,97 @@ int real_eigen(int n, const double* a, std::vector<double>& lambda, std::vector<
   return 0;
 }
 
+// Even/odd form of a factorisation (the layout is described with
+// Factorisation).  The eigenvectors are symmetrised, grouped by parity, and
+// the two half-size blocks inverted separately; nothing is built when the
+// operator is not centro-symmetric or an eigenvector is not clearly even or
+// odd (a near-degenerate pair of opposite parity).
+static void make_split(Factorisation& f, const std::vector<double>& fwd, const std::vector<double>& lam,
+                       const std::vector<double>& vec) {
+  const int N = f.nz;
+  if (N % 2 == 0 || N < 7) return;
+  const int m = N - 2, M = (N - 1) / 2;
+  double mx = 0.0, dv = 0.0;
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      mx = std::max(mx, std::fabs(fwd[i + (size_t)m * j]));
+      dv = std::max(dv, std::fabs(fwd[i + (size_t)m * j] - fwd[(m - 1 - i) + (size_t)m * (m - 1 - j)]));
+    }
+  if (!(dv <= 1e-12 * mx)) return;
+  std::vector<int> ev, od;
+  for (int k = 0; k < m; ++k) {
+    const double* v = &vec[(size_t)m * k];
+    double se = 0.0, so = 0.0, nn = 0.0;
+    for (int i = 0; i < m; ++i) {
+      se += (v[i] + v[m - 1 - i]) * (v[i] + v[m - 1 - i]);
+      so += (v[i] - v[m - 1 - i]) * (v[i] - v[m - 1 - i]);
+      nn += v[i] * v[i];
+    }
+    if (so <= 1e-16 * nn) ev.push_back(k);
+    else if (se <= 1e-16 * nn) od.push_back(k);
+    else return;
+  }
+  if ((int)ev.size() != M || (int)od.size() != M - 1) return;
+  // half blocks on interior split indices 1..M (even) and 1..M-1 (odd), column-major
+  const int me = M, mo = M - 1;
+  std::vector<double> re((size_t)me * me), ro((size_t)mo * mo), rei(re.size()), roi(ro.size());
+  for (int c = 0; c < me; ++c) {
+    const double* v = &vec[(size_t)m * ev[c]];
+    for (int r = 0; r < M - 1; ++r) re[r + (size_t)me * c] = 0.5 * (v[r] + v[m - 1 - r]);
+    re[(M - 1) + (size_t)me * c] = v[M - 1];
+  }
+  for (int c = 0; c < mo; ++c) {
+    const double* v = &vec[(size_t)m * od[c]];
+    for (int r = 0; r < M - 1; ++r) ro[r + (size_t)mo * c] = 0.5 * (v[r] - v[m - 1 - r]);
+  }
+  if (!invert(me, re.data(), rei.data()) || !invert(mo, ro.data(), roi.data())) return;
+  const int ld = (M + 1 + 3) / 4 * 4;
+  f.pinv.assign((size_t)(2 * M + 2) * ld, 0.0);
+  f.pnorm.assign((size_t)(2 * M + 2) * ld, 0.0);
+  auto PI = [&](int r, int c) -> double& { return f.pinv[(size_t)r * ld + c]; };
+  auto PN = [&](int r, int c) -> double& { return f.pnorm[(size_t)r * ld + c]; };
+  // eigen-coordinates from (s, d): the even part of x is s_i / 2 (s_M at i = M)
+  PI(0, 0) = 1.0;
+  for (int k = 0; k < me; ++k) {
+    for (int i = 1; i < M; ++i) PI(1 + k, i) = 0.5 * rei[k + (size_t)me * (i - 1)];
+    PI(1 + k, M) = rei[k + (size_t)me * (M - 1)];
+  }
+  PI(M + 1, 0) = 1.0;
+  for (int k = 0; k < mo; ++k)
+    for (int i = 1; i < M; ++i) PI(M + 2 + k, i) = 0.5 * roi[k + (size_t)mo * (i - 1)];
+  // columns from the coordinates: the wall pair gives y_0 = (s_0 + d_0)/2,
+  // y_{nz-1} = (s_0 - d_0)/2
+  PN(0, 0) = 0.5;
+  for (int i = 1; i <= M; ++i)
+    for (int k = 0; k < me; ++k) PN(i, 1 + k) = re[(i - 1) + (size_t)me * k];
+  PN(M + 1, 0) = 0.5;
+  for (int i = 1; i < M; ++i)
+    for (int k = 0; k < mo; ++k) PN(M + 1 + i, 1 + k) = ro[(i - 1) + (size_t)mo * k];
+  f.plam.assign(N, 0.0);
+  for (int k = 0; k < me; ++k) f.plam[1 + k] = lam[ev[k]];
+  for (int k = 0; k < mo; ++k) f.plam[M + 2 + k] = lam[od[k]];
+  f.pnull = -1;
+  if (f.nullrow >= 1) {
+    const int j = f.nullrow - 1;
+    for (int k = 0; k < me; ++k)
+      if (ev[k] == j) f.pnull = 1 + k;
+    for (int k = 0; k < mo; ++k)
+      if (od[k] == j) f.pnull = M + 2 + k;
+  }
+  f.pfix.assign(2 * (size_t)N, 0.0);
+  for (int t = 0; t < 2; ++t) {
+    const double* w = &f.fix[(2 + t) * (size_t)N];
+    double* o = &f.pfix[t * (size_t)N];
+    for (int i = 0; i < M; ++i) {
+      o[i] = w[i] + w[N - 1 - i];
+      o[M + 1 + i] = w[i] - w[N - 1 - i];
+    }
+    o[M] = w[M];
+  }
+  f.pM = M;
+  f.pld = ld;
+}
+
 Factorisation make_factorisation(const Chebyshev& ch, const double* vt) {
   const int N = ch.nz;
   Factorisation f;
@@ -441,6 +532,7 @@ Factorisation make_factorisation(const Chebyshev& ch, const double* vt) {
     if (std::fabs(f.lambda[i]) < bmin) { bmin = std::fabs(f.lambda[i]); best = i; }
   }
   f.nullrow = (bmin < 10E-12 || bmin < 1e-9 * bmax) ? best : -1;
+  make_split(f, fwd, lam, vec);
   return f;
 }
 

@@ -33,6 +33,24 @@ struct Factorisation {
   std::vector<double> lambda;       // nz, rows 0 and nz-1 unused
   std::vector<double> fix;          // 4*nz: forward bottom, forward top, backward bottom, backward top
   int nullrow = -1;                 // row whose reciprocal is zeroed on the mean column when sigma == 0
+
+  // Even/odd form, present (pM > 0) when nz = 2M+1 and the interior operator
+  // is centro-symmetric (the same boundary type at both walls on an
+  // unmapped grid), so every interior eigenvector is even or odd under
+  // i -> nz-1-i.  Column entries are split as s_i = x_i + x_{nz-1-i},
+  // s_M = x_M (i < M) at split index i, d_i = x_i - x_{nz-1-i} at M+1+i.
+  // Eigen-coordinates: split index 0 the even wall pair, 1..M the even
+  // eigenvectors, M+1 the odd wall pair, M+2..2M the odd eigenvectors.
+  //   pinv  (2M+2) x pld row-major: rows 0..M act on s (K = M+1), rows
+  //         M+1..2M on d (K = M): the eigen-coordinates of the column.
+  //   pnorm (2M+2) x pld row-major: rows 0..M (a) from the even
+  //         coordinates, M+1..2M+1 (b) from the odd ones; the column is
+  //         y_i = a_i + b_i, y_{nz-1-i} = a_i - b_i (i <= M, b_M = 0).
+  //   plam  eigenvalue per split index;  pnull  split index of nullrow.
+  //   pfix  backward-fix weights on (a, b): bottom at [0, nz), top at
+  //         [nz, 2nz), each as (w_i + w_{nz-1-i}, w_M) then (w_i - w_{nz-1-i}).
+  int pM = 0, pld = 0, pnull = -1;
+  std::vector<double> pinv, pnorm, plam, pfix;
 };
 
 // Builds the factorisation from D1, D2 and the boundary rows a*f + b*df/dz.

@@ -73,6 +73,7 @@ struct HField {
   int nullrow = -1;               // gated row on the mean column (sigma == 0 only)
   const double2* x = nullptr;     // right-hand side
   double2* y = nullptr;           // solution (may alias x)
+  const double* pfix = nullptr;   // helmholtz_split only: backward-fix weights on the split column
 };
 
 struct HArgs {
@@ -84,6 +85,14 @@ struct HArgs {
 };
 
 void helmholtz(const HArgs& a, int nfield, hipStream_t st);
+
+// The same solve for one field through the even/odd form of its
+// factorisation (Factorisation::pM > 0): vinv, vnorm, lambda, nullrow are the
+// split pinv, pnorm, plam, pnull (lda = pld), fix the natural 4*nz fix array,
+// pfix the split backward-fix weights.  Half the MFMA work of helmholtz().
+void helmholtz_split(const HArgs& a, hipStream_t st);
+// flops of one helmholtz_split field (both contractions, complex columns)
+double helmholtz_split_flops(int nz, long long P);
 
 // Rows 0 and nz-1 of M x for every column (thin contraction: wall values of
 // dU/dz, dV/dz used by the erosion map and the log features).

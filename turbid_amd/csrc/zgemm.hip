@@ -611,6 +611,159 @@ void helmholtz(const HArgs& a, int nfield, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
+// Helmholtz through the even/odd form: the column is loaded with the forward
+// fix applied and split into s (indices 0..M) and d (M+1..2M); the
+// block-diagonal pinv gives the eigen-coordinates, scaled in registers; the
+// split V~ gives a (0..M) and b (M+1..2M+1); the backward fixes are dot
+// products over (a, b) and the column is written as y_i = a_i + b_i,
+// y_{nz-1-i} = a_i - b_i.
+// ---------------------------------------------------------------------------
+static constexpr int kSplitMaxCol = 64;
+
+__host__ inline void make_split_panels(int nz, size_t lds_limit, Panel& p1, Panel& p2) {
+  const int M = (nz - 1) / 2;
+  p2 = make_panel(nz, sizeof(double2), lds_limit, 1);
+  p1 = p2;
+  p1.sg[1] = make_seg(M, M, M + 1, M + 1, M + 1);
+  const int pairs = p1.sg[0].npair + p1.sg[1].npair;
+  const int tailrows = (p1.sg[0].rows - p1.sg[0].RM) + (p1.sg[1].rows - p1.sg[1].RM);
+  p1.nunit = pairs * p1.ncg;
+  p1.ntail = tailrows * p1.ncol;
+}
+
+template <int DEPTH>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HArgs a, Panel p1, Panel p2) {
+  extern __shared__ char smem_raw[];
+  __shared__ double2 tfix[2][kSplitMaxCol];
+  const int nz = a.nz;
+  const int M = (nz - 1) / 2;
+  const int ldx = p1.ldx;
+  double2* xs = reinterpret_cast<double2*>(smem_raw);
+  const double* fix = a.f[0].fix;
+  const double* lam = a.f[0].lambda;
+  const double sigma = a.f[0].sigma;
+  const int nullrow = a.f[0].nullrow;
+  const long long c0 = (long long)blockIdx.x * p1.ncol;
+  const int ncol = (int)min((long long)p1.ncol, (long long)a.P - c0);
+  const double2* x = a.f[0].x + c0 * nz;
+  for (int idx = threadIdx.x; idx < p1.ncol * (M + 1); idx += kNT) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    double2 sv = make_double2(0.0, 0.0), dv = sv;
+    if (c < ncol) {
+      const double2* xc = x + (long long)c * nz;
+      double2 lo = xc[j], hi = xc[nz - 1 - j];
+      if (a.mode == 0) {
+        if (j > 0) {   // X_i += fbf_i X_0 + ftf_i X_{N-1} on interior rows
+          const double2 x0 = xc[0], xn = xc[nz - 1];
+          lo = cadd<double>(lo, cadd<double>(cscale<double>(fix[j], x0), cscale<double>(fix[nz + j], xn)));
+          hi = cadd<double>(hi, cadd<double>(cscale<double>(fix[nz - 1 - j], x0),
+                                             cscale<double>(fix[2 * nz - 1 - j], xn)));
+        }
+      } else if (j == 0) {
+        lo = make_double2(0.0, 0.0);
+        hi = lo;
+      }
+      if (j < M) {
+        sv = cadd<double>(lo, hi);
+        dv = csub<double>(lo, hi);
+      } else {
+        sv = lo;
+      }
+    }
+    xs[c * ldx + j] = sv;
+    if (j < M) xs[c * ldx + M + 1 + j] = dv;
+  }
+  __syncthreads();
+  gemm_panel<double, DEPTH>(a.f[0].vinv, a.lda, p1, xs, [&](int row, int c, double2 v) {
+    double r = 1.0;
+    if (row != 0 && row != M + 1) {
+      const long long col = c0 + c;
+      const long long cc = col < a.P ? col : 0;
+      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double k2 = (-(kx * kx)) + (-(ky * ky));
+      r = 1.0 / (((k2 + lam[row]) + sigma));
+      if (row == nullrow && col == 0) r = 0.0;
+    }
+    xs[c * ldx + row] = cscale<double>(r, v);
+  });
+  __syncthreads();
+  gemm_panel<double, DEPTH>(a.f[0].vnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
+  __syncthreads();
+  // row 0 = fbb . Y, then row N-1 = fbt . Y with row 0 replaced
+  const double* wb = a.f[0].pfix;
+  const double* wt = a.f[0].pfix + nz;
+  const double fbt0 = fix[3 * nz];
+  const int part = threadIdx.x & 15;
+  for (int cb = 0; cb < p1.ncol; cb += kNT / 16) {
+    const int c = cb + (threadIdx.x >> 4);
+    double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
+    if (c < p1.ncol) {
+      for (int j = part; j < nz; j += 16) {
+        const double2 v = xs[c * ldx + j];
+        br += wb[j] * v.x;
+        bi += wb[j] * v.y;
+        tr += wt[j] * v.x;
+        ti += wt[j] * v.y;
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      br += __shfl_xor(br, o, 16);
+      bi += __shfl_xor(bi, o, 16);
+      tr += __shfl_xor(tr, o, 16);
+      ti += __shfl_xor(ti, o, 16);
+    }
+    if (part == 0 && c < p1.ncol) {
+      const double2 a0 = xs[c * ldx], b0 = xs[c * ldx + M + 1];
+      const double y0r = a0.x + b0.x, y0i = a0.y + b0.y;
+      tfix[0][c] = make_double2(br, bi);
+      tfix[1][c] = make_double2(tr + fbt0 * (br - y0r), ti + fbt0 * (bi - y0i));
+    }
+  }
+  __syncthreads();
+  double2* y = a.f[0].y + c0 * nz;
+  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
+    const int c = idx / nz, k = idx - c * nz;
+    double2 v;
+    if (k == 0) {
+      v = tfix[0][c];
+    } else if (k == nz - 1) {
+      v = tfix[1][c];
+    } else if (k <= M) {
+      v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
+    } else {
+      const int i = nz - 1 - k;
+      v = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
+    }
+    y[idx] = v;
+  }
+}
+
+void helmholtz_split(const HArgs& a, hipStream_t st) {
+  TBD_CHECK(a.nz >= 7 && a.nz % 2 == 1 && a.P > 0 && a.lda >= (a.nz + 1) / 2 && a.lda % 4 == 0 && a.f[0].pfix,
+            "helmholtz_split > bad shape");
+  Panel p1, p2;
+  make_split_panels(a.nz, kLdsLimit, p1, p2);
+  TBD_CHECK(p1.ncol <= kSplitMaxCol, "helmholtz_split > too many columns per panel");
+  const int depth = std::max(buffered_depth(p1), buffered_depth(p2));
+  TBD_CHECK(depth <= kMaxU, "helmholtz_split > nz too large for the register-buffered contraction");
+  const size_t lds = sizeof(double2) * p1.ncol * p1.ldx;
+  TBD_CHECK(lds + sizeof(double2) * 2 * kSplitMaxCol <= 160 * 1024, "helmholtz_split > nz too large for one LDS panel");
+  dim3 grid(ceil_div(a.P, p1.ncol), 1);
+  if (depth <= 1)
+    hipLaunchKernelGGL(k_helmholtz_split<1>, grid, dim3(kNT), lds, st, a, p1, p2);
+  else
+    hipLaunchKernelGGL(k_helmholtz_split<2>, grid, dim3(kNT), lds, st, a, p1, p2);
+  TBD_HIP(hipGetLastError());
+}
+
+double helmholtz_split_flops(int nz, long long P) {
+  const double M = (nz - 1) / 2;
+  // pinv: (M+1)^2 + M^2, split V~: (M+1)^2 + (M+1) M multiply-adds per real column
+  return 4.0 * ((M + 1) * (M + 1) + M * M + (M + 1) * (M + 1) + (M + 1) * M) * (double)P;
+}
+
+// ---------------------------------------------------------------------------
 // wall rows of M x: one wave per column
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_zrows_wall(const double* __restrict__ op, int lda, int nz,
