@@ -55,6 +55,8 @@ KERNELS = {
     "zgemm32": ("k_zgemm<float,...>", "flops", MFMA_PEAK["fp32"]),
     "helmholtz": ("k_helmholtz<1>", "flops", MFMA_PEAK["fp64"]),
     "helmholtz_split": ("k_helmholtz_split<1>", "flops", MFMA_PEAK["fp64"]),
+    "rhs_helmholtz": ("k_rhs_helmholtz<1, false>", "flops", MFMA_PEAK["fp64"]),
+    "rhs_helmholtz_split": ("k_rhs_helmholtz<1, true>", "flops", MFMA_PEAK["fp64"]),
     "products64": ("k_products_*<double>", "bytes", HBM_PEAK),
     "products32": ("k_products_*<float>", "bytes", HBM_PEAK),
     "other": ("(boundary rows, H update, filter, diagnostics, copies)", None, None),

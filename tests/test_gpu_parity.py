@@ -200,6 +200,19 @@ def test_steps_plain_operators(precision, monkeypatch):
     assert rel_l2(cg, co) < tol
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_steps_rhs_unfused(precision, monkeypatch):
+    # right-hand side, boundary rows and Helmholtz solves as separate kernels
+    monkeypatch.setenv("TBD_RHS_UNFUSED", "1")
+    cfg = small_config(precision=precision)
+    o, g = _run_pair(cfg, 2)
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    tol = 1e-9 if precision == 0 else 1e-5
+    assert rel_l2(ug, uo) < tol
+    assert rel_l2(cg, co) < tol
+
+
 @pytest.mark.parametrize("nz", [65, 97])
 def test_steps_parity_sizes(nz):
     # even/odd split on the CGL sizes of the ensemble members

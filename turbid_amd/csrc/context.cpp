@@ -89,6 +89,33 @@ double Context::zwork() const {
   return 4.0 * ((M + 1) * (M + 1) + (M + 1) * M) * (double)d.P;
 }
 
+void Context::rhs_solve(RHArgs& base, int n, const int* k, const int* comp, const double2* const* x,
+                        const double2* const* h, double2* const* y, double sigma) {
+  base.nz = d.nz; base.P = (int)d.P; base.nyc = d.nyc; base.kx = kx_; base.ky = ky_;
+  base.d2p = d2p_; base.ldp = ldp_;
+  for (int pass = 0; pass < 2; ++pass) {
+    RHArgs a = base;
+    a.split = pass;
+    a.lda = pass ? 0 : lda_;
+    int nf = 0;
+    for (int i = 0; i < n; ++i) {
+      const int f = k[i];
+      if ((int)hsplit_[f] != pass) continue;
+      RHField& q = a.f[nf++];
+      q.x = x[i]; q.h = h[i]; q.y = y[i]; q.comp = comp[i]; q.sigma = sigma; q.fix = fix_[f];
+      if (pass) {
+        a.lda = pld_[f];
+        q.vinv = pinv_[f]; q.vnorm = pnorm_[f]; q.lambda = plam_[f]; q.pfix = pfix_[f];
+      } else {
+        q.vinv = vinvp_[f]; q.vnorm = vnorm_[f]; q.lambda = lam_[f];
+      }
+    }
+    if (nf == 0) continue;
+    launch(pass ? KC_RHELMS : KC_RHELM, nf * rhs_helmholtz_flops(d.nz, d.P, pass == 1),
+           [&] { rhs_helmholtz(a, nf, st_); });
+  }
+}
+
 void Context::solve(int mode, int n, const int* k, const double* sigma, const double2* const* x,
                     double2* const* y) {
   HArgs h;
@@ -232,6 +259,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
     vnorm_[k] = up(rowmajor(fac[k].vnorm, nz, lda_));
     lam_[k] = up(fac[k].lambda);
     fix_[k] = up(fac[k].fix);
+    vinvp_[k] = nz % 2 == 1 ? up(permute_columns(rowmajor(fac[k].vinv, nz, lda_), nz, lda_)) : nullptr;
     hsplit_[k] = fac[k].pM > 0 && !(nps && nps[0] == '1');
     if (hsplit_[k]) {
       pld_[k] = fac[k].pld;
@@ -240,6 +268,10 @@ Context::Context(const tbd_config& c) : cfg(c) {
       plam_[k] = up(fac[k].plam);
       pfix_[k] = up(fac[k].pfix);
     }
+  }
+  {
+    const char* e = std::getenv("TBD_RHS_UNFUSED");
+    fuse_rhs_ = parity_ && nz % 2 == 1 && nz >= 7 && !(e && e[0] == '1');
   }
   // Fourier multipliers with the reference's sign at iy = nyc/2 (stepper_space.cc:181-213)
   std::vector<double> kx(d.mx), ky(d.nyc);
@@ -571,8 +603,8 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   if (capturing_) u.fdev = force_;
   launch(KC_OTHER, 0.0, [&] { h_update<CT>(huvw_, adv, c_, u, 3, S, st_); });
   // lap = D2 u - kx^2 u - ky^2 u ; rhs = sigma (c2 H + u) - lap (:517-536)
-  {
-    const double redtc3 = -cfg.reynolds / dt_ / c3;
+  const double redtc3 = -cfg.reynolds / dt_ / c3;
+  if (!fuse_rhs_) {
     ZArgs a;
     a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
     a.x = uvw_; a.xs = S; a.y = rhs_; a.ys = S; a.h = huvw_; a.hs = S; a.kx = kx_; a.ky = ky_;
@@ -588,21 +620,37 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       case 1: xm0 = 2 * (c3m0 + c3m1 + 0) * dt_; xm1 = 2 * (0 + c3m1 + 0) * dt_; xm2 = 0; break;
       default: xm0 = 2 * (c3m0 + c3m1 + c3m2) * dt_; xm1 = 2 * (0 + c3m1 + c3m2) * dt_; xm2 = 2 * (0 + +0 + c3m2) * dt_;
     }
+    const double bs = (xm0 - xm1) / (xm1 - xm2), bk = 2 * dt_ * c3m0;
+    const double sg = -cfg.reynolds / dt_ / coef_[2][stage];
+    if (fuse_rhs_) {
+      // rhs, boundary rows and the three Helmholtz solves in one pass per field
+      RHArgs r;
+      r.s1 = redtc3; r.s2 = redtc3 * c2;
+      r.bc = 0; r.s = bs; r.k = bk;
+      r.m1b = wall_[wslot_][0]; r.m1t = wall_[wslot_][1];
+      r.m2b = wall_[1 - wslot_][0]; r.m2t = wall_[1 - wslot_][1];
+      r.n1t = wall2_[wslot_][1]; r.n2t = wall2_[1 - wslot_][1];
+      r.open_top = opentop_ ? 1 : 0;
+      r.top[0] = cfg.vt_u1[4]; r.top[1] = cfg.vt_u2[4]; r.top[2] = cfg.vt_u3[4];
+      const int k[3] = {0, 1, 2};
+      const double2* x[3] = {uvw_, uvw_ + S, uvw_ + 2 * S};
+      const double2* h[3] = {huvw_, huvw_ + S, huvw_ + 2 * S};
+      double2* y[3] = {suvw_, suvw_ + S, suvw_ + 2 * S};
+      rhs_solve(r, 3, k, k, x, h, y, sg);
+      return;
+    }
     BcArgs b;
     b.nz = d.nz; b.P = d.P; b.S = S;
-    b.s = (xm0 - xm1) / (xm1 - xm2);
-    b.k = 2 * dt_ * c3m0;
+    b.s = bs;
+    b.k = bk;
     b.m1b = wall_[wslot_][0]; b.m1t = wall_[wslot_][1];
     b.m2b = wall_[1 - wslot_][0]; b.m2t = wall_[1 - wslot_][1];
     b.n1t = wall2_[wslot_][1]; b.n2t = wall2_[1 - wslot_][1];
     b.open_top = opentop_ ? 1 : 0;
     b.top0 = cfg.vt_u1[4]; b.top1 = cfg.vt_u2[4]; b.top2 = cfg.vt_u3[4];
     launch(KC_OTHER, 0.0, [&] { bc_carrier(rhs_, b, st_); });
-  }
-  // intermediate velocity: three Helmholtz solves (:554, :867-931)
-  {
+    // intermediate velocity: three Helmholtz solves (:554, :867-931)
     const int k[3] = {0, 1, 2};
-    const double sg = -cfg.reynolds / dt_ / coef_[2][stage];
     const double sigma[3] = {sg, sg, sg};
     const double2* x[3] = {rhs_, rhs_ + S, rhs_ + 2 * S};
     double2* y[3] = {suvw_, suvw_ + S, suvw_ + 2 * S};
@@ -721,7 +769,7 @@ void Context::scalar_correct(int stage, int alternation) {
     launch(KC_OTHER, 0.0, [&] { h_update<CT>(hc_, advc, c_, u, 1, S, st_); });
   }
   // lap, rhs (:319-329)
-  {
+  if (!fuse_rhs_) {
     const double redtc3 = -cfg.reynolds * cfg.schmidt / dt_ / c3;
     ZArgs a;
     a.mode = Z_RHS; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d2(a);
@@ -740,10 +788,11 @@ void Context::scalar_correct(int stage, int alternation) {
     fwd<double, double2>(ephys_, plane, ehat_, d.P, 1, 0, 1, (double2*)tmp_, 1);
   }
   const double wsc = cfg.settling_velocity * std::cos(cfg.slope);
-  launch(KC_OTHER, 0.0, [&] {
-    bc_scalar(rhsc_, d.nz, d.P, ehat_, c_, wsc, cfg.averaged_concentration, cfg.reynolds, cfg.schmidt, erosion,
-              st_);
-  });
+  if (!fuse_rhs_)
+    launch(KC_OTHER, 0.0, [&] {
+      bc_scalar(rhsc_, d.nz, d.P, ehat_, c_, wsc, cfg.averaged_concentration, cfg.reynolds, cfg.schmidt, erosion,
+                st_);
+    });
   if (erosion) {   // feature 28: mean erosion flux, 29: mean deposition flux (:377-380)
     TBD_HIP(hipMemcpyAsync(fl_ + 28, ehat_, sizeof(double), hipMemcpyDeviceToDevice, st_));
     launch(KC_OTHER, 0.0, [&] { scaled_real(c_, wsc, cfg.averaged_concentration, fl_ + 29, st_); });
@@ -752,9 +801,22 @@ void Context::scalar_correct(int stage, int alternation) {
   {
     const int k = 4;
     const double sigma = -cfg.reynolds * cfg.schmidt / dt_ / coef_[2][stage];
-    const double2* x = rhsc_;
-    double2* y = c_;
-    solve(0, 1, &k, &sigma, &x, &y);
+    if (fuse_rhs_) {
+      // rhs, boundary rows and the solve in one pass; c is read before it is overwritten
+      RHArgs r;
+      r.s1 = sigma; r.s2 = sigma * coef_[1][stage];
+      r.bc = 1; r.ehat = ehat_; r.ws = wsc; r.cbar = cfg.averaged_concentration; r.re = cfg.reynolds;
+      r.sc = cfg.schmidt; r.erosion = erosion;
+      const int comp = 0;
+      const double2* x = c_;
+      const double2* h = hc_;
+      double2* y = c_;
+      rhs_solve(r, 1, &k, &comp, &x, &h, &y, sigma);
+    } else {
+      const double2* x = rhsc_;
+      double2* y = c_;
+      solve(0, 1, &k, &sigma, &x, &y);
+    }
   }
   // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
   bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_);

@@ -25,6 +25,7 @@ enum KClass {
   KC_PROD64, KC_PROD32,                       // pointwise nonlinear products
   KC_OTHER,
   KC_HELMS,                                   // Helmholtz / pressure solves through the even/odd form
+  KC_RHELM, KC_RHELMS,                        // rhs + boundary rows + Helmholtz (full / even/odd form)
   KC_N
 };
 
@@ -80,6 +81,11 @@ class Context {
   void forces(double time, double& f1, double& f2) const;
   template <class F> void launch(int kclass, double work, F&& f);
   double zwork() const;
+  // rhs + boundary rows + Helmholtz of n fields (factorisations k[i],
+  // carrier components comp[i]) through rhs_helmholtz: one launch for the
+  // fields with a full factorisation, one for those with an even/odd form
+  void rhs_solve(RHArgs& base, int n, const int* k, const int* comp, const double2* const* x,
+                 const double2* const* h, double2* const* y, double sigma);
   // Helmholtz solves of n <= 3 fields, field i with factorisation k[i] and
   // stage coefficient sigma[i]: fields whose factorisation has an even/odd
   // form go through helmholtz_split (one launch each), the rest share one
@@ -138,6 +144,8 @@ class Context {
   double* pnorm_[5] = {nullptr};
   double* plam_[5] = {nullptr};
   double* pfix_[5] = {nullptr};
+  double* vinvp_[5] = {nullptr};   // V~^-1 with permuted columns (rhs_helmholtz)
+  bool fuse_rhs_ = false;          // rhs + boundary rows + Helmholtz fused (TBD_RHS_UNFUSED=1 disables)
   double* kx_ = nullptr;
   double* ky_ = nullptr;
   double* gridz_ = nullptr;

@@ -94,6 +94,60 @@ void helmholtz_split(const HArgs& a, hipStream_t st);
 // flops of one helmholtz_split field (both contractions, complex columns)
 double helmholtz_split_flops(int nz, long long P);
 
+// Right-hand side, boundary rows and Helmholtz solve in one kernel, for up to
+// three fields with the same kind of factorisation:
+//   rhs = s2 h + s1 x - ((D2 x - kx^2 x) - ky^2 x)   (Predict / Correct rhs,
+//   phase_carrier_flat.cc:517-536, phase_scalar_flat.cc:319-329), rows 0 and
+//   nz-1 replaced by the boundary rows (carrier: extrapolated wall pressure
+//   gradient, boundary_carrier_flat.cc:185-246; scalar: zero flux and the
+//   erosion/deposition bottom row, phase_scalar_flat.cc:340-384), then
+//   y = the Helmholtz solve of rhs (forward fix, V~^-1, 1/(lambda - k^2 + sigma),
+//   V~, sequential backward fixes).
+// D2 is applied through its even/odd split (nz odd).  Plain factorisations
+// take V~^-1 with its columns permuted to the order (0..M, nz-1..M+1)
+// (permute_columns); split ones take their even/odd form.  y may alias x.
+struct RHField {
+  const double2* x = nullptr;     // field (D2 operand, s1 term)
+  const double2* h = nullptr;     // explicit-term accumulator
+  double2* y = nullptr;           // solution
+  int comp = 0;                   // carrier component (0..2) for the wall-row arrays
+  const double* vinv = nullptr;   // permuted V~^-1, or pinv
+  const double* vnorm = nullptr;  // V~, or pnorm
+  const double* lambda = nullptr; // lambda, or plam
+  const double* fix = nullptr;    // natural 4*nz fix array
+  const double* pfix = nullptr;   // split only
+  double sigma = 0;
+};
+struct RHArgs {
+  int nz = 0, P = 0, nyc = 0;
+  const double* kx = nullptr;
+  const double* ky = nullptr;
+  const double* d2p = nullptr;    // D2 in its even/odd split (pack_parity), leading dimension ldp
+  int ldp = 0;
+  double s1 = 0, s2 = 0;
+  int split = 0;                  // factorisations in their even/odd form
+  int lda = 0;                    // leading dimension of vinv/vnorm
+  int bc = 0;                     // 0 carrier rows, 1 scalar rows
+  // carrier rows: k ((1+s) m1 - s m2), open-top tangential rows from n1/n2,
+  // mean column top = top[comp]
+  double k = 0, s = 0;
+  const double2* m1b = nullptr; const double2* m1t = nullptr;
+  const double2* m2b = nullptr; const double2* m2t = nullptr;
+  const double2* n1t = nullptr; const double2* n2t = nullptr;
+  int open_top = 0;
+  double top[3] = {0, 0, 0};
+  // scalar rows: bottom -(E - ws c_0 cbar) Re Sc / cbar 1e7 when erosion
+  const double2* ehat = nullptr;
+  double ws = 0, cbar = 1, re = 0, sc = 0;
+  int erosion = 0;
+  RHField f[3];
+};
+void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st);
+// flops of one rhs_helmholtz field
+double rhs_helmholtz_flops(int nz, long long P, bool split);
+// V~^-1 (row-major, nz x lda) with its columns in the order 0..M, nz-1..M+1
+std::vector<double> permute_columns(const std::vector<double>& rowmajor, int nz, int lda);
+
 // Rows 0 and nz-1 of M x for every column (thin contraction: wall values of
 // dU/dz, dV/dz used by the erosion map and the log features).
 void zrows_wall(const double* op_rowmajor, int lda, int nz, const double2* x, long long P,

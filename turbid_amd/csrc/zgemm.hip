@@ -353,13 +353,71 @@ __device__ __forceinline__ C2<T> ld(const TX* p) {
   return mk<T>((T)v.x, (T)v.y);
 }
 
+// Runs load(idx) then store(idx, value) for idx = threadIdx.x + j*kNT < n,
+// NB indices at a time: all NB loads of a batch are issued before its first
+// store, so a thread keeps NB global reads in flight instead of one.
+template <int NB, typename V, class Load, class Store>
+__device__ __forceinline__ void batched(int n, Load load, Store store) {
+  for (int base = threadIdx.x; base < n; base += NB * kNT) {
+    V v[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int idx = base + j * kNT;
+      if (idx < n) v[j] = load(idx);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int idx = base + j * kNT;
+      if (idx < n) store(idx, v[j]);
+    }
+  }
+}
+
+// The global inputs an epilogue mode reads besides the contraction result.
+template <typename T> struct EpiIn { C2<T> x, e0, e1, e2; };
+
+template <typename T, typename TX>
+__device__ __forceinline__ EpiIn<T> epi_load(const ZArgs& a, int f, long long off) {
+  EpiIn<T> in;
+  in.x = in.e0 = in.e1 = in.e2 = mk<T>((T)0, (T)0);
+  const TX* x = (const TX*)a.x + f * a.xs;
+  switch (a.mode) {
+    case Z_GRAD:
+      in.x = ld<T, TX>(x + off);
+      break;
+    case Z_RHS:
+      in.x = ld<T, TX>(x + off);
+      in.e0 = ld<T, TX>((const TX*)a.h + f * a.hs + off);
+      break;
+    case Z_DIV:
+      in.e0 = ld<T, TX>((const TX*)a.e0 + off);
+      in.e1 = ld<T, TX>((const TX*)a.e1 + off);
+      break;
+    case Z_CORRECT:
+      in.x = ld<T, TX>(x + off);
+      in.e0 = ld<T, TX>((const TX*)a.e0 + off);
+      in.e1 = ld<T, TX>((const TX*)a.e0 + a.es + off);
+      in.e2 = ld<T, TX>((const TX*)a.e0 + 2 * a.es + off);
+      break;
+    case Z_GRADTMP:
+      in.e0 = ld<T, TX>((const TX*)a.e0 + f * a.es + off);
+      in.e1 = ld<T, TX>((const TX*)a.e1 + f * a.es + off);
+      break;
+    default:
+      break;
+  }
+  return in;
+}
+
 template <typename T, typename TX, typename TY>
-__device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, int row, C2<T> v, C2<T> xv) {
+__device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, int row, C2<T> v,
+                                          const EpiIn<T>& in) {
   const int nz = a.nz;
   const long long off = col * nz + row;
   TY* y = (TY*)a.y;
   const T kx = (T)a.kx[col / a.nyc];
   const T ky = (T)a.ky[col % a.nyc];
+  const C2<T> xv = in.x;
   switch (a.mode) {
     case Z_PLAIN:
       st<TY, T>(y + f * a.ys + off, v);
@@ -370,24 +428,24 @@ __device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, 
       st<TY, T>(y + (3 * f + 2) * a.ys + off, v);
       break;
     case Z_RHS: {
-      const C2<T> h = ld<T, TX>((const TX*)a.h + f * a.hs + off);
+      const C2<T> h = in.e0;
       const T k2x = -(kx * kx), k2y = -(ky * ky);
       const C2<T> lap = cadd<T>(cadd<T>(v, cscale<T>(k2x, xv)), cscale<T>(k2y, xv));
       const C2<T> r = csub<T>(cadd<T>(cscale<T>((T)a.s2, h), cscale<T>((T)a.s1, xv)), lap);
       st<TY, T>(y + f * a.ys + off, r);
     } break;
     case Z_DIV: {
-      const C2<T> su = ld<T, TX>((const TX*)a.e0 + off);
-      const C2<T> sv = ld<T, TX>((const TX*)a.e1 + off);
+      const C2<T> su = in.e0;
+      const C2<T> sv = in.e1;
       const C2<T> dv = cadd<T>(cadd<T>(v, cmuli<T>(kx, su)), cmuli<T>(ky, sv));
       st<TY, T>(y + off, cscale<T>((T)a.q, dv));
     } break;
     case Z_CORRECT: {
       const C2<T> gx = cmuli<T>(kx, xv), gy = cmuli<T>(ky, xv);
       const T cf = (T)a.cf;
-      const C2<T> su = ld<T, TX>((const TX*)a.e0 + off);
-      const C2<T> sv = ld<T, TX>((const TX*)a.e0 + a.es + off);
-      const C2<T> sw = ld<T, TX>((const TX*)a.e0 + 2 * a.es + off);
+      const C2<T> su = in.e0;
+      const C2<T> sv = in.e1;
+      const C2<T> sw = in.e2;
       st<TY, T>(y + off, csub<T>(su, cscale<T>(cf, gx)));
       st<TY, T>(y + a.ys + off, csub<T>(sv, cscale<T>(cf, gy)));
       C2<T> w = csub<T>(sw, cscale<T>(cf, v));
@@ -408,8 +466,8 @@ __device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, 
       }
     } break;
     case Z_GRADTMP: {
-      const C2<T> f0 = ld<T, TX>((const TX*)a.e0 + f * a.es + off);
-      const C2<T> f1 = ld<T, TX>((const TX*)a.e1 + f * a.es + off);
+      const C2<T> f0 = in.e0;
+      const C2<T> f1 = in.e1;
       st<TY, T>(y + f * a.ys + off, cadd<T>(cadd<T>(cmuli<T>(kx, f0), cmuli<T>(ky, f1)), v));
     } break;
     default:
@@ -432,40 +490,46 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_zgemm(ZArgs a, Pane
   const long long c0 = (long long)blockIdx.x * pn.ncol;
   const int ncol = (int)min((long long)pn.ncol, a.P - c0);
   const TX* x = (const TX*)a.x + f * a.xs + c0 * nz;
+  const C2<T> zero = mk<T>((T)0, (T)0);
   if (pn.parity == 0) {
-    for (int idx = threadIdx.x; idx < pn.ncol * nz; idx += kNT) {
+    batched<8, C2<T>>(pn.ncol * nz, [&](int idx) {
+      const int c = idx / nz;
+      return (c < ncol) ? ld<T, TX>(x + idx) : zero;
+    }, [&](int idx, C2<T> v) {
       const int c = idx / nz, k = idx - c * nz;
-      xs[c * ldx + k] = (c < ncol) ? ld<T, TX>(x + (long long)c * nz + k) : mk<T>((T)0, (T)0);
-    }
+      xs[c * ldx + k] = v;
+    });
   } else {
     // even part e_j = (x_j + x_{N-1-j})/2 at j <= M (e_M = x_M), odd part
     // o_j = (x_j - x_{N-1-j})/2 at M+1+j, j < M
     const int M = (nz - 1) / 2;
-    for (int idx = threadIdx.x; idx < pn.ncol * (M + 1); idx += kNT) {
+    struct LoHi { C2<T> lo, hi; };
+    batched<4, LoHi>(pn.ncol * (M + 1), [&](int idx) {
       const int c = idx / (M + 1), j = idx - c * (M + 1);
-      C2<T> e = mk<T>((T)0, (T)0), o = e;
+      LoHi r = {zero, zero};
       if (c < ncol) {
-        const C2<T> lo = ld<T, TX>(x + (long long)c * nz + j);
-        const C2<T> hi = ld<T, TX>(x + (long long)c * nz + nz - 1 - j);
-        if (j < M) {
-          e = mk<T>((T)0.5 * (lo.x + hi.x), (T)0.5 * (lo.y + hi.y));
-          o = mk<T>((T)0.5 * (lo.x - hi.x), (T)0.5 * (lo.y - hi.y));
-        } else {
-          e = lo;
-        }
+        r.lo = ld<T, TX>(x + (long long)c * nz + j);
+        r.hi = ld<T, TX>(x + (long long)c * nz + nz - 1 - j);
       }
-      xs[c * ldx + j] = e;
-      if (j < M) xs[c * ldx + M + 1 + j] = o;
-    }
+      return r;
+    }, [&](int idx, LoHi r) {
+      const int c = idx / (M + 1), j = idx - c * (M + 1);
+      if (j < M) {
+        xs[c * ldx + j] = mk<T>((T)0.5 * (r.lo.x + r.hi.x), (T)0.5 * (r.lo.y + r.hi.y));
+        xs[c * ldx + M + 1 + j] = mk<T>((T)0.5 * (r.lo.x - r.hi.x), (T)0.5 * (r.lo.y - r.hi.y));
+      } else {
+        xs[c * ldx + j] = r.lo;
+      }
+    });
   }
   __syncthreads();
   gemm_panel<T, DEPTH>((const T*)a.op, a.lda, pn, xs, [&](int row, int c, C2<T> v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  const bool needx = a.mode == Z_GRAD || a.mode == Z_RHS || a.mode == Z_CORRECT;
   const int M = (nz - 1) / 2;
-  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
+  batched<4, EpiIn<T>>(ncol * nz, [&](int idx) {
+    return epi_load<T, TX>(a, f, c0 * nz + idx);
+  }, [&](int idx, const EpiIn<T>& in) {
     const int c = idx / nz, row = idx - c * nz;
-    const C2<T> xv = needx ? ld<T, TX>(x + idx) : mk<T>((T)0, (T)0);
     C2<T> v;
     if (pn.parity == 0) {
       v = xs[c * ldx + row];
@@ -476,8 +540,8 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_zgemm(ZArgs a, Pane
       const C2<T> sv = xs[c * ldx + i], dv = xs[c * ldx + M + 1 + i];
       v = top ? (pn.parity > 0 ? csub<T>(sv, dv) : csub<T>(dv, sv)) : cadd<T>(sv, dv);
     }
-    zepilogue<T, TX, TY>(a, f, c0 + c, row, v, xv);
-  }
+    zepilogue<T, TX, TY>(a, f, c0 + c, row, v, in);
+  });
 }
 
 template <typename T, typename TX, typename TY>
@@ -522,10 +586,12 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HArgs a, 
   double2* y = fi == 0 ? a.f[0].y : (fi == 1 ? a.f[1].y : a.f[2].y);
   const long long c0 = (long long)blockIdx.x * pn.ncol;
   const int ncol = (int)min((long long)pn.ncol, (long long)a.P - c0);
-  for (int idx = threadIdx.x; idx < pn.ncol * nz; idx += kNT) {
+  batched<8, double2>(pn.ncol * nz, [&](int idx) {
+    return (idx / nz < ncol) ? x[c0 * nz + idx] : make_double2(0.0, 0.0);
+  }, [&](int idx, double2 v) {
     const int c = idx / nz, k = idx - c * nz;
-    xs[c * ldx + k] = (c < ncol) ? x[(c0 + c) * nz + k] : make_double2(0.0, 0.0);
-  }
+    xs[c * ldx + k] = v;
+  });
   __syncthreads();
   if (a.mode == 0) {
     // forward fix: X_i += fbf_i X_0 + ftf_i X_{N-1}, i = 1..N-2
@@ -646,33 +712,41 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
   const long long c0 = (long long)blockIdx.x * p1.ncol;
   const int ncol = (int)min((long long)p1.ncol, (long long)a.P - c0);
   const double2* x = a.f[0].x + c0 * nz;
-  for (int idx = threadIdx.x; idx < p1.ncol * (M + 1); idx += kNT) {
+  struct Quad { double2 lo, hi, x0, xn; };
+  const double2 zero = make_double2(0.0, 0.0);
+  batched<4, Quad>(p1.ncol * (M + 1), [&](int idx) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
-    double2 sv = make_double2(0.0, 0.0), dv = sv;
+    Quad q = {zero, zero, zero, zero};
     if (c < ncol) {
       const double2* xc = x + (long long)c * nz;
-      double2 lo = xc[j], hi = xc[nz - 1 - j];
-      if (a.mode == 0) {
-        if (j > 0) {   // X_i += fbf_i X_0 + ftf_i X_{N-1} on interior rows
-          const double2 x0 = xc[0], xn = xc[nz - 1];
-          lo = cadd<double>(lo, cadd<double>(cscale<double>(fix[j], x0), cscale<double>(fix[nz + j], xn)));
-          hi = cadd<double>(hi, cadd<double>(cscale<double>(fix[nz - 1 - j], x0),
-                                             cscale<double>(fix[2 * nz - 1 - j], xn)));
-        }
-      } else if (j == 0) {
-        lo = make_double2(0.0, 0.0);
-        hi = lo;
-      }
-      if (j < M) {
-        sv = cadd<double>(lo, hi);
-        dv = csub<double>(lo, hi);
-      } else {
-        sv = lo;
+      q.lo = xc[j];
+      q.hi = xc[nz - 1 - j];
+      if (a.mode == 0 && j > 0) {
+        q.x0 = xc[0];
+        q.xn = xc[nz - 1];
       }
     }
-    xs[c * ldx + j] = sv;
-    if (j < M) xs[c * ldx + M + 1 + j] = dv;
-  }
+    return q;
+  }, [&](int idx, Quad q) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    double2 lo = q.lo, hi = q.hi;
+    if (a.mode == 0) {
+      if (j > 0) {   // X_i += fbf_i X_0 + ftf_i X_{N-1} on interior rows
+        lo = cadd<double>(lo, cadd<double>(cscale<double>(fix[j], q.x0), cscale<double>(fix[nz + j], q.xn)));
+        hi = cadd<double>(hi, cadd<double>(cscale<double>(fix[nz - 1 - j], q.x0),
+                                           cscale<double>(fix[2 * nz - 1 - j], q.xn)));
+      }
+    } else if (j == 0) {
+      lo = zero;
+      hi = zero;
+    }
+    if (j < M) {
+      xs[c * ldx + j] = cadd<double>(lo, hi);
+      xs[c * ldx + M + 1 + j] = csub<double>(lo, hi);
+    } else {
+      xs[c * ldx + j] = lo;
+    }
+  });
   __syncthreads();
   gemm_panel<double, DEPTH>(a.f[0].vinv, a.lda, p1, xs, [&](int row, int c, double2 v) {
     double r = 1.0;
@@ -755,6 +829,256 @@ void helmholtz_split(const HArgs& a, hipStream_t st) {
   else
     hipLaunchKernelGGL(k_helmholtz_split<2>, grid, dim3(kNT), lds, st, a, p1, p2);
   TBD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// rhs assembly + boundary rows + Helmholtz in one kernel
+// ---------------------------------------------------------------------------
+__host__ inline Panel with_ncg(Panel pn, int ncg, int ldx) {
+  pn.ncg = ncg;
+  pn.ncol = 16 * ncg;
+  pn.ldx = ldx;
+  int pairs = 0, tailrows = 0;
+  for (int i = 0; i < pn.nseg; ++i) {
+    pairs += pn.sg[i].npair;
+    tailrows += pn.sg[i].rows - pn.sg[i].RM;
+  }
+  pn.nunit = pairs * ncg;
+  pn.ntail = tailrows * pn.ncol;
+  return pn;
+}
+
+template <int DEPTH, bool SPLIT>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArgs a, Panel pd, Panel p1, Panel p2) {
+  extern __shared__ char smem_raw[];
+  __shared__ double2 bcv[2][kSplitMaxCol];
+  __shared__ double2 tfix[2][kSplitMaxCol];
+  const int nz = a.nz;
+  const int M = (nz - 1) / 2;
+  const int ldx = pd.ldx;
+  double2* xs = reinterpret_cast<double2*>(smem_raw);
+  const int fi = blockIdx.y;
+  const RHField& F = fi == 0 ? a.f[0] : (fi == 1 ? a.f[1] : a.f[2]);
+  const double2* x = F.x;
+  const double2* h = F.h;
+  const double* fix = F.fix;
+  const double* lam = F.lambda;
+  const double sigma = F.sigma;
+  const long long c0 = (long long)blockIdx.x * pd.ncol;
+  const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
+  const double2 zero = make_double2(0.0, 0.0);
+  // boundary rows of every column of the panel
+  if ((int)threadIdx.x < pd.ncol) {
+    const int c = threadIdx.x;
+    double2 bot = zero, top = zero;
+    if (c < ncol) {
+      const long long col = c0 + c;
+      if (a.bc == 0) {
+        const long long j = (long long)F.comp * a.P + col;
+        const double sk = a.s, k = a.k;
+        const double2 m1t = a.m1t[j], m2t = a.m2t[j], m1b = a.m1b[j], m2b = a.m2b[j];
+        top = make_double2(k * ((1 + sk) * m1t.x + (0 - sk) * m2t.x), k * ((1 + sk) * m1t.y + (0 - sk) * m2t.y));
+        bot = make_double2(k * ((1 + sk) * m1b.x + (0 - sk) * m2b.x), k * ((1 + sk) * m1b.y + (0 - sk) * m2b.y));
+        if (a.open_top && F.comp < 2) {
+          const double2 n1 = a.n1t[j], n2 = a.n2t[j];
+          top = make_double2(k * ((1 + sk) * n1.x + (0 - sk) * n2.x), k * ((1 + sk) * n1.y + (0 - sk) * n2.y));
+        }
+        if (col == 0) top = make_double2(a.top[F.comp], 0.0);
+      } else if (a.erosion) {
+        const double2 e = a.ehat[col], cv = x[col * nz];
+        double2 v = make_double2(e.x - a.ws * cv.x * a.cbar, e.y - a.ws * cv.y * a.cbar);
+        const double scale = 10E+6;
+        bot = make_double2(-1.0 * v.x * a.re * a.sc / a.cbar * scale, -1.0 * v.y * a.re * a.sc / a.cbar * scale);
+      }
+    }
+    bcv[0][c] = bot;
+    bcv[1][c] = top;
+  }
+  // even/odd halves of x for the D2 contraction
+  struct LoHi { double2 lo, hi; };
+  batched<4, LoHi>(pd.ncol * (M + 1), [&](int idx) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    LoHi r = {zero, zero};
+    if (c < ncol) {
+      r.lo = x[(c0 + c) * nz + j];
+      r.hi = x[(c0 + c) * nz + nz - 1 - j];
+    }
+    return r;
+  }, [&](int idx, LoHi r) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    if (j < M) {
+      xs[c * ldx + j] = make_double2(0.5 * (r.lo.x + r.hi.x), 0.5 * (r.lo.y + r.hi.y));
+      xs[c * ldx + M + 1 + j] = make_double2(0.5 * (r.lo.x - r.hi.x), 0.5 * (r.lo.y - r.hi.y));
+    } else {
+      xs[c * ldx + j] = r.lo;
+    }
+  });
+  __syncthreads();
+  gemm_panel<double, DEPTH>(a.d2p, a.ldp, pd, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
+  __syncthreads();
+  // rhs per row pair (j, nz-1-j); rows 0 and nz-1 are the boundary rows, the
+  // interior ones get the forward fix; written back to the two slots read
+  struct In { double2 xl, xh, hl, hh; };
+  batched<2, In>(pd.ncol * (M + 1), [&](int idx) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    In r = {zero, zero, zero, zero};
+    if (c < ncol && j > 0) {
+      const long long o = (c0 + c) * nz;
+      r.xl = x[o + j];
+      r.hl = h[o + j];
+      if (j < M) {
+        r.xh = x[o + nz - 1 - j];
+        r.hh = h[o + nz - 1 - j];
+      }
+    }
+    return r;
+  }, [&](int idx, In r) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    const double2 bot = bcv[0][c], top = bcv[1][c];
+    double2 lo = bot, hi = top;
+    if (j > 0) {
+      const long long col = c0 + c;
+      const long long cc = col < a.P ? col : 0;
+      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double k2x = -(kx * kx), k2y = -(ky * ky);
+      const double2 sv = xs[c * ldx + j], dv = xs[c * ldx + M + 1 + j];
+      auto rhs = [&](double2 v, double2 xv, double2 hv) {
+        const double2 lap = cadd<double>(cadd<double>(v, cscale<double>(k2x, xv)), cscale<double>(k2y, xv));
+        const double2 t = csub<double>(cadd<double>(cscale<double>(a.s2, hv), cscale<double>(a.s1, xv)), lap);
+        return t;
+      };
+      auto fixed = [&](int i, double2 v) {
+        return cadd<double>(v, cadd<double>(cscale<double>(fix[i], bot), cscale<double>(fix[nz + i], top)));
+      };
+      lo = fixed(j, rhs(cadd<double>(sv, dv), r.xl, r.hl));
+      if (j < M) hi = fixed(nz - 1 - j, rhs(csub<double>(sv, dv), r.xh, r.hh));
+    }
+    if (j == M) {
+      xs[c * ldx + M] = lo;
+    } else if (SPLIT) {
+      xs[c * ldx + j] = cadd<double>(lo, hi);
+      xs[c * ldx + M + 1 + j] = csub<double>(lo, hi);
+    } else {
+      xs[c * ldx + j] = lo;
+      xs[c * ldx + M + 1 + j] = hi;
+    }
+  });
+  __syncthreads();
+  // Z = R o (V~^-1 X)
+  const int brow = SPLIT ? M + 1 : nz - 1;
+  gemm_panel<double, DEPTH>(F.vinv, a.lda, p1, xs, [&](int row, int c, double2 v) {
+    double r = 1.0;
+    if (row != 0 && row != brow) {
+      const long long col = c0 + c;
+      const long long cc = col < a.P ? col : 0;
+      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double k2 = (-(kx * kx)) + (-(ky * ky));
+      r = 1.0 / (((k2 + lam[row]) + sigma));
+    }
+    xs[c * ldx + row] = cscale<double>(r, v);
+  });
+  __syncthreads();
+  gemm_panel<double, DEPTH>(F.vnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
+  __syncthreads();
+  // row 0 = fbb . Y, then row N-1 = fbt . Y with row 0 replaced
+  const int part = threadIdx.x & 15;
+  for (int cb = 0; cb < pd.ncol; cb += kNT / 16) {
+    const int c = cb + (threadIdx.x >> 4);
+    double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
+    const double* wb = SPLIT ? F.pfix : fix + 2 * nz;
+    const double* wt = SPLIT ? F.pfix + nz : fix + 3 * nz;
+    if (c < pd.ncol) {
+      for (int j = part; j < nz; j += 16) {
+        const double2 v = xs[c * ldx + j];
+        br += wb[j] * v.x;
+        bi += wb[j] * v.y;
+        tr += wt[j] * v.x;
+        ti += wt[j] * v.y;
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      br += __shfl_xor(br, o, 16);
+      bi += __shfl_xor(bi, o, 16);
+      tr += __shfl_xor(tr, o, 16);
+      ti += __shfl_xor(ti, o, 16);
+    }
+    if (part == 0 && c < pd.ncol) {
+      const double fbt0 = fix[3 * nz];
+      double2 y0 = xs[c * ldx];
+      if (SPLIT) y0 = cadd<double>(y0, xs[c * ldx + M + 1]);
+      tfix[0][c] = make_double2(br, bi);
+      tfix[1][c] = make_double2(tr + fbt0 * (br - y0.x), ti + fbt0 * (bi - y0.y));
+    }
+  }
+  __syncthreads();
+  double2* y = F.y + c0 * nz;
+  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
+    const int c = idx / nz, k = idx - c * nz;
+    double2 v;
+    if (k == 0) {
+      v = tfix[0][c];
+    } else if (k == nz - 1) {
+      v = tfix[1][c];
+    } else if (!SPLIT) {
+      v = xs[c * ldx + k];
+    } else if (k <= M) {
+      v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
+    } else {
+      const int i = nz - 1 - k;
+      v = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
+    }
+    y[idx] = v;
+  }
+}
+
+void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st) {
+  TBD_CHECK(a.nz >= 7 && a.nz % 2 == 1 && a.P > 0 && nfield >= 1 && nfield <= 3 && a.d2p &&
+                a.ldp >= (a.nz + 1) / 2 && a.ldp % 4 == 0 && a.lda % 4 == 0 &&
+                a.lda >= (a.split ? (a.nz + 1) / 2 : a.nz),
+            "rhs_helmholtz > bad shape");
+  const Panel pd0 = make_panel(a.nz, sizeof(double2), kLdsLimit, 1);
+  Panel q1, q2;
+  if (a.split) {
+    make_split_panels(a.nz, kLdsLimit, q1, q2);
+  } else {
+    q1 = make_panel(a.nz, sizeof(double2), kLdsLimit, 0);
+    q2 = q1;
+  }
+  const int ncg = std::min(std::min(pd0.ncg, q1.ncg), q2.ncg);
+  const int ldx = pd0.ldx;
+  const Panel pd = with_ncg(pd0, ncg, ldx), p1 = with_ncg(q1, ncg, ldx), p2 = with_ncg(q2, ncg, ldx);
+  TBD_CHECK(pd.ncol <= kSplitMaxCol, "rhs_helmholtz > too many columns per panel");
+  const int depth = std::max(std::max(buffered_depth(pd), buffered_depth(p1)), buffered_depth(p2));
+  TBD_CHECK(depth <= kMaxU, "rhs_helmholtz > nz too large for the register-buffered contraction");
+  const size_t lds = sizeof(double2) * pd.ncol * ldx;
+  TBD_CHECK(lds + sizeof(double2) * 4 * kSplitMaxCol <= 160 * 1024, "rhs_helmholtz > nz too large for one LDS panel");
+  dim3 grid(ceil_div(a.P, pd.ncol), nfield);
+  if (a.split) {
+    if (depth <= 1) hipLaunchKernelGGL((k_rhs_helmholtz<1, true>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
+    else hipLaunchKernelGGL((k_rhs_helmholtz<2, true>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
+  } else {
+    if (depth <= 1) hipLaunchKernelGGL((k_rhs_helmholtz<1, false>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
+    else hipLaunchKernelGGL((k_rhs_helmholtz<2, false>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
+  }
+  TBD_HIP(hipGetLastError());
+}
+
+std::vector<double> permute_columns(const std::vector<double>& rm, int nz, int lda) {
+  const int M = (nz - 1) / 2;
+  std::vector<double> out(rm.size(), 0.0);
+  for (int r = 0; r < nz; ++r)
+    for (int p = 0; p < nz; ++p) {
+      const int nat = p <= M ? p : 3 * M + 1 - p;
+      out[(size_t)r * lda + p] = rm[(size_t)r * lda + nat];
+    }
+  return out;
+}
+
+double rhs_helmholtz_flops(int nz, long long P, bool split) {
+  const double M = (nz - 1) / 2;
+  const double d2 = 4.0 * ((M + 1) * (M + 1) + (M + 1) * M) * (double)P;
+  return d2 + (split ? helmholtz_split_flops(nz, P) : 8.0 * nz * (double)nz * P);
 }
 
 double helmholtz_split_flops(int nz, long long P) {

@@ -48,7 +48,7 @@ KCLASSES = {
     "ypass_bwd32": (4, "bytes"), "xpass_bwd32": (5, "bytes"), "xpass_fwd32": (6, "bytes"), "ypass_fwd32": (7, "bytes"),
     "xfused64": (8, "bytes"), "xfused32": (9, "bytes"), "zgemm64": (10, "flops"), "zgemm32": (11, "flops"),
     "helmholtz": (12, "flops"), "products64": (13, "bytes"), "products32": (14, "bytes"), "other": (15, None),
-    "helmholtz_split": (16, "flops"),
+    "helmholtz_split": (16, "flops"), "rhs_helmholtz": (17, "flops"), "rhs_helmholtz_split": (18, "flops"),
 }
 KCLASS = {k: v[0] for k, v in KCLASSES.items()}
 
