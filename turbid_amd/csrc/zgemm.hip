@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_zgemm(ZArgs a, Pane
     // o_j = (x_j - x_{N-1-j})/2 at M+1+j, j < M
     const int M = (nz - 1) / 2;
     struct LoHi { C2<T> lo, hi; };
-    batched<4, LoHi>(pn.ncol * (M + 1), [&](int idx) {
+    batched<5, LoHi>(pn.ncol * (M + 1), [&](int idx) {
       const int c = idx / (M + 1), j = idx - c * (M + 1);
       LoHi r = {zero, zero};
       if (c < ncol) {
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
   const double2* x = a.f[0].x + c0 * nz;
   struct Quad { double2 lo, hi, x0, xn; };
   const double2 zero = make_double2(0.0, 0.0);
-  batched<4, Quad>(p1.ncol * (M + 1), [&](int idx) {
+  batched<5, Quad>(p1.ncol * (M + 1), [&](int idx) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
     Quad q = {zero, zero, zero, zero};
     if (c < ncol) {
@@ -896,7 +896,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   }
   // even/odd halves of x for the D2 contraction
   struct LoHi { double2 lo, hi; };
-  batched<4, LoHi>(pd.ncol * (M + 1), [&](int idx) {
+  batched<5, LoHi>(pd.ncol * (M + 1), [&](int idx) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
     LoHi r = {zero, zero};
     if (c < ncol) {
@@ -919,7 +919,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   // rhs per row pair (j, nz-1-j); rows 0 and nz-1 are the boundary rows, the
   // interior ones get the forward fix; written back to the two slots read
   struct In { double2 xl, xh, hl, hh; };
-  batched<2, In>(pd.ncol * (M + 1), [&](int idx) {
+  batched<3, In>(pd.ncol * (M + 1), [&](int idx) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
     In r = {zero, zero, zero, zero};
     if (c < ncol && j > 0) {
