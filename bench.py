@@ -57,6 +57,7 @@ KERNELS = {
     "helmholtz_split": ("k_helmholtz_split<1>", "flops", MFMA_PEAK["fp64"]),
     "rhs_helmholtz": ("k_rhs_helmholtz<1, false>", "flops", MFMA_PEAK["fp64"]),
     "rhs_helmholtz_split": ("k_rhs_helmholtz<1, true>", "flops", MFMA_PEAK["fp64"]),
+    "pressure_chain": ("k_pressure_chain<1>", "flops", MFMA_PEAK["fp64"]),
     "products64": ("k_products_*<double>", "bytes", HBM_PEAK),
     "products32": ("k_products_*<float>", "bytes", HBM_PEAK),
     "other": ("(boundary rows, H update, filter, diagnostics, copies)", None, None),

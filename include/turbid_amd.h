@@ -105,12 +105,14 @@ int tbd_sync(tbd_ctx* ctx);
  *         factorisation (FP64; fields with the same boundary type at both walls)
  *   17, 18 right-hand side + boundary rows + Helmholtz solve in one kernel
  *         (full / even/odd factorisation; FP64; D2 through its even/odd split)
+ *   19    divergence + pressure solve + projection in one kernel (FP64)
  * work: algorithmic work of the timed launches -- bytes read plus written
  * (classes 0..9, 13, 14) or the flops executed: 4*nz^2*P per contraction of
  * one component with the full operator, 4*((M+1)^2 + (M+1)*M)*P (nz = 2M+1)
  * through the even/odd split (classes 10, 11); a Helmholtz solve is two
  * contractions (class 12; class 16: 4*(2(M+1)^2 + M^2 + (M+1)*M)*P); classes
- * 17, 18 add the split D2 contraction to their solve. */
+ * 17, 18 add the split D2 contraction to their solve, class 19 two split D1
+ * contractions to an even/odd solve. */
 int tbd_profile_enable(tbd_ctx* ctx, int enable);
 int tbd_profile_read(tbd_ctx* ctx, int kclass, long long* launches, double* total_ms, double* work);
 int tbd_profile_reset(tbd_ctx* ctx);

@@ -90,7 +90,7 @@ double Context::zwork() const {
 }
 
 void Context::rhs_solve(RHArgs& base, int n, const int* k, const int* comp, const double2* const* x,
-                        const double2* const* h, double2* const* y, double sigma) {
+                        double2* const* h, const void* const* adv, double2* const* y, double sigma) {
   base.nz = d.nz; base.P = (int)d.P; base.nyc = d.nyc; base.kx = kx_; base.ky = ky_;
   base.d2p = d2p_; base.ldp = ldp_;
   for (int pass = 0; pass < 2; ++pass) {
@@ -102,7 +102,7 @@ void Context::rhs_solve(RHArgs& base, int n, const int* k, const int* comp, cons
       const int f = k[i];
       if ((int)hsplit_[f] != pass) continue;
       RHField& q = a.f[nf++];
-      q.x = x[i]; q.h = h[i]; q.y = y[i]; q.comp = comp[i]; q.sigma = sigma; q.fix = fix_[f];
+      q.x = x[i]; q.h = h[i]; q.adv = adv[i]; q.y = y[i]; q.comp = comp[i]; q.sigma = sigma; q.fix = fix_[f];
       if (pass) {
         a.lda = pld_[f];
         q.vinv = pinv_[f]; q.vnorm = pnorm_[f]; q.lambda = plam_[f]; q.pfix = pfix_[f];
@@ -114,6 +114,20 @@ void Context::rhs_solve(RHArgs& base, int n, const int* k, const int* comp, cons
     launch(pass ? KC_RHELMS : KC_RHELM, nf * rhs_helmholtz_flops(d.nz, d.P, pass == 1),
            [&] { rhs_helmholtz(a, nf, st_); });
   }
+}
+
+void Context::pchain(double q, double cf, const double2* s, double2* p, double2* y, int slot, bool count) {
+  PArgs a;
+  a.nz = d.nz; a.P = (int)d.P; a.nyc = d.nyc; a.kx = kx_; a.ky = ky_;
+  a.d1p = d1p_; a.ldp = ldp_;
+  a.pinv = pinv_[3]; a.pnorm = pnorm_[3]; a.plam = plam_[3]; a.pfix = pfix_[3]; a.fix = fix_[3];
+  a.lda = pld_[3]; a.nullrow = fac[3].pnull;
+  a.q = q; a.cf = cf; a.s = s; a.p = p; a.y = y; a.S = d.S;
+  a.wall_bot = wall_[slot][0]; a.wall_top = wall_[slot][1];
+  a.wall2_bot = wall2_[slot][0]; a.wall2_top = wall2_[slot][1];
+  a.open_top = opentop_ ? 1 : 0;
+  if (count) launch(KC_PCHAIN, pressure_chain_flops(d.nz, d.P), [&] { pressure_chain(a, st_); });
+  else pressure_chain(a, st_);
 }
 
 void Context::solve(int mode, int n, const int* k, const double* sigma, const double2* const* x,
@@ -272,6 +286,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   {
     const char* e = std::getenv("TBD_RHS_UNFUSED");
     fuse_rhs_ = parity_ && nz % 2 == 1 && nz >= 7 && !(e && e[0] == '1');
+    fuse_p_ = fuse_rhs_ && hsplit_[3];
   }
   // Fourier multipliers with the reference's sign at iy = nyc/2 (stepper_space.cc:181-213)
   std::vector<double> kx(d.mx), ky(d.nyc);
@@ -601,7 +616,8 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   u.sn = std::sin(cfg.slope); u.cs = std::cos(cfg.slope); u.slope_dir = cfg.slope_direction;
   forces(time, u.f1, u.f2);
   if (capturing_) u.fdev = force_;
-  launch(KC_OTHER, 0.0, [&] { h_update<CT>(huvw_, adv, c_, u, 3, S, st_); });
+  // on the fused path the update happens inside the right-hand-side kernel
+  if (!fuse_rhs_) launch(KC_OTHER, 0.0, [&] { h_update<CT>(huvw_, adv, c_, u, 3, S, st_); });
   // lap = D2 u - kx^2 u - ky^2 u ; rhs = sigma (c2 H + u) - lap (:517-536)
   const double redtc3 = -cfg.reynolds / dt_ / c3;
   if (!fuse_rhs_) {
@@ -632,11 +648,15 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       r.n1t = wall2_[wslot_][1]; r.n2t = wall2_[1 - wslot_][1];
       r.open_top = opentop_ ? 1 : 0;
       r.top[0] = cfg.vt_u1[4]; r.top[1] = cfg.vt_u2[4]; r.top[2] = cfg.vt_u3[4];
+      r.hupd = 1; r.adv32 = is32<T>() ? 1 : 0; r.c1 = u.c1; r.dt = u.dt;
+      r.buoy = u.buoy; r.K = u.K; r.sn = u.sn; r.cs = u.cs; r.slope_dir = u.slope_dir; r.cb = c_;
+      r.force = 1; r.f1 = u.f1; r.f2 = u.f2; r.fdev = u.fdev;
       const int k[3] = {0, 1, 2};
       const double2* x[3] = {uvw_, uvw_ + S, uvw_ + 2 * S};
-      const double2* h[3] = {huvw_, huvw_ + S, huvw_ + 2 * S};
+      double2* h[3] = {huvw_, huvw_ + S, huvw_ + 2 * S};
+      const void* av[3] = {adv, adv + S, adv + 2 * S};
       double2* y[3] = {suvw_, suvw_ + S, suvw_ + 2 * S};
-      rhs_solve(r, 3, k, k, x, h, y, sg);
+      rhs_solve(r, 3, k, k, x, h, av, y, sg);
       return;
     }
     BcArgs b;
@@ -665,35 +685,43 @@ template <typename T>
 void Context::carrier_correct(int stage) {
   const double c3 = coef_[2][stage];
   const long long S = d.S, R = d.R;
-  // div s, rhs_p = div / (2 dt c3) (:586-589)
-  {
-    ZArgs a;
-    a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
-    a.x = suvw_ + 2 * S; a.e0 = suvw_; a.e1 = suvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
-    a.q = 0.5 / dt_ / c3;
-    launch(KC_Z64, zwork(), [&] { zgemm<double, double2, double2>(a, 1, st_); });
-  }
-  // pressure solve (:594, :955-987)
-  {
-    const int k = 3;
-    const double sigma = 0.0;
-    const double2* x = rhsp_;
-    double2* y = p_;
-    solve(1, 1, &k, &sigma, &x, &y);
-  }
-  // grad p, boundary Update (m2 <- m1, m1 <- walls), u = s - 2 dt c3 grad p,
-  // mean w := 0 (:597-620, boundary_carrier_flat.cc:131-170)
-  if (stage == 2) olddt_ = dt_;
-  wslot_ = 1 - wslot_;
-  {
-    ZArgs a;
-    a.mode = Z_CORRECT; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
-    a.x = p_; a.e0 = suvw_; a.es = S; a.y = uvw_; a.ys = S; a.kx = kx_; a.ky = ky_;
-    a.cf = 2 * dt_ * c3;
-    a.wall_bot = wall_[wslot_][0]; a.wall_top = wall_[wslot_][1];
-    a.wall2_bot = wall2_[wslot_][0]; a.wall2_top = wall2_[wslot_][1];
-    a.open_top = opentop_ ? 1 : 0;
-    launch(KC_Z64, zwork(), [&] { zgemm<double, double2, double2>(a, 1, st_); });
+  if (fuse_p_) {
+    // divergence, pressure solve, grad p, boundary Update and projection in
+    // one pass over the spectral columns (:586-620)
+    if (stage == 2) olddt_ = dt_;
+    wslot_ = 1 - wslot_;
+    pchain(0.5 / dt_ / c3, 2 * dt_ * c3, suvw_, p_, uvw_, wslot_, true);
+  } else {
+    // div s, rhs_p = div / (2 dt c3) (:586-589)
+    {
+      ZArgs a;
+      a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
+      a.x = suvw_ + 2 * S; a.e0 = suvw_; a.e1 = suvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
+      a.q = 0.5 / dt_ / c3;
+      launch(KC_Z64, zwork(), [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    }
+    // pressure solve (:594, :955-987)
+    {
+      const int k = 3;
+      const double sigma = 0.0;
+      const double2* x = rhsp_;
+      double2* y = p_;
+      solve(1, 1, &k, &sigma, &x, &y);
+    }
+    // grad p, boundary Update (m2 <- m1, m1 <- walls), u = s - 2 dt c3 grad p,
+    // mean w := 0 (:597-620, boundary_carrier_flat.cc:131-170)
+    if (stage == 2) olddt_ = dt_;
+    wslot_ = 1 - wslot_;
+    {
+      ZArgs a;
+      a.mode = Z_CORRECT; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
+      a.x = p_; a.e0 = suvw_; a.es = S; a.y = uvw_; a.ys = S; a.kx = kx_; a.ky = ky_;
+      a.cf = 2 * dt_ * c3;
+      a.wall_bot = wall_[wslot_][0]; a.wall_top = wall_[wslot_][1];
+      a.wall2_bot = wall2_[wslot_][0]; a.wall2_top = wall2_[wslot_][1];
+      a.open_top = opentop_ ? 1 : 0;
+      launch(KC_Z64, zwork(), [&] { zgemm<double, double2, double2>(a, 1, st_); });
+    }
   }
   // uvwnonlinear = DftBackward(uvw) (:625-626)
   bwd<T, double2>(uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_);
@@ -762,8 +790,8 @@ void Context::scalar_correct(int stage, int alternation) {
       fwd<T, CT>(aphys, R, advc, S, 1, 0, d.nz, tmp);
     }
   }
-  // H = c1 H - dt adv (:313-315)
-  {
+  // H = c1 H - dt adv (:313-315); on the fused path inside the right-hand-side kernel
+  if (!fuse_rhs_) {
     HUpd u;
     u.c1 = c1; u.dt = dt_; u.nz = 0;
     launch(KC_OTHER, 0.0, [&] { h_update<CT>(hc_, advc, c_, u, 1, S, st_); });
@@ -807,11 +835,13 @@ void Context::scalar_correct(int stage, int alternation) {
       r.s1 = sigma; r.s2 = sigma * coef_[1][stage];
       r.bc = 1; r.ehat = ehat_; r.ws = wsc; r.cbar = cfg.averaged_concentration; r.re = cfg.reynolds;
       r.sc = cfg.schmidt; r.erosion = erosion;
+      r.hupd = 1; r.adv32 = is32<T>() ? 1 : 0; r.c1 = coef_[0][stage]; r.dt = dt_;
       const int comp = 0;
       const double2* x = c_;
-      const double2* h = hc_;
+      double2* h = hc_;
+      const void* av = advc;
       double2* y = c_;
-      rhs_solve(r, 1, &k, &comp, &x, &h, &y, sigma);
+      rhs_solve(r, 1, &k, &comp, &x, &h, &av, &y, sigma);
     } else {
       const double2* x = rhsc_;
       double2* y = c_;
@@ -872,29 +902,33 @@ void Context::stats_reset() {
 // 2 dt c3 = 1, phi in place of p.
 void Context::project() {
   const long long S = d.S;
-  {
-    ZArgs a;
-    a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
-    a.x = uvw_ + 2 * S; a.e0 = uvw_; a.e1 = uvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
-    a.q = 1.0;
-    zgemm<double, double2, double2>(a, 1, st_);
-  }
-  {
-    const int k = 3;
-    const double sigma = 0.0;
-    const double2* x = rhsp_;
-    double2* y = rhsp_;
-    solve(1, 1, &k, &sigma, &x, &y);
-  }
-  {
-    ZArgs a;
-    a.mode = Z_CORRECT; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
-    a.x = rhsp_; a.e0 = uvw_; a.es = S; a.y = uvw_; a.ys = S; a.kx = kx_; a.ky = ky_;
-    a.cf = 1.0;
-    a.wall_bot = wall_[0][0]; a.wall_top = wall_[0][1];
-    a.wall2_bot = wall2_[0][0]; a.wall2_top = wall2_[0][1];
-    a.open_top = opentop_ ? 1 : 0;
-    zgemm<double, double2, double2>(a, 1, st_);
+  if (fuse_p_) {
+    pchain(1.0, 1.0, uvw_, rhsp_, uvw_, 0, false);
+  } else {
+    {
+      ZArgs a;
+      a.mode = Z_DIV; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
+      a.x = uvw_ + 2 * S; a.e0 = uvw_; a.e1 = uvw_ + S; a.y = rhsp_; a.kx = kx_; a.ky = ky_;
+      a.q = 1.0;
+      zgemm<double, double2, double2>(a, 1, st_);
+    }
+    {
+      const int k = 3;
+      const double sigma = 0.0;
+      const double2* x = rhsp_;
+      double2* y = rhsp_;
+      solve(1, 1, &k, &sigma, &x, &y);
+    }
+    {
+      ZArgs a;
+      a.mode = Z_CORRECT; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
+      a.x = rhsp_; a.e0 = uvw_; a.es = S; a.y = uvw_; a.ys = S; a.kx = kx_; a.ky = ky_;
+      a.cf = 1.0;
+      a.wall_bot = wall_[0][0]; a.wall_top = wall_[0][1];
+      a.wall2_bot = wall2_[0][0]; a.wall2_top = wall2_[0][1];
+      a.open_top = opentop_ ? 1 : 0;
+      zgemm<double, double2, double2>(a, 1, st_);
+    }
   }
   for (int s = 0; s < 2; ++s)
     for (int b = 0; b < 2; ++b) {

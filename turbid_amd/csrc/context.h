@@ -26,6 +26,7 @@ enum KClass {
   KC_OTHER,
   KC_HELMS,                                   // Helmholtz / pressure solves through the even/odd form
   KC_RHELM, KC_RHELMS,                        // rhs + boundary rows + Helmholtz (full / even/odd form)
+  KC_PCHAIN,                                  // divergence + pressure solve + projection
   KC_N
 };
 
@@ -81,11 +82,11 @@ class Context {
   void forces(double time, double& f1, double& f2) const;
   template <class F> void launch(int kclass, double work, F&& f);
   double zwork() const;
-  // rhs + boundary rows + Helmholtz of n fields (factorisations k[i],
-  // carrier components comp[i]) through rhs_helmholtz: one launch for the
+  // explicit-term update + rhs + boundary rows + Helmholtz of n fields
+  // (factorisations k[i], carrier components comp[i]) through rhs_helmholtz: one launch for the
   // fields with a full factorisation, one for those with an even/odd form
   void rhs_solve(RHArgs& base, int n, const int* k, const int* comp, const double2* const* x,
-                 const double2* const* h, double2* const* y, double sigma);
+                 double2* const* h, const void* const* adv, double2* const* y, double sigma);
   // Helmholtz solves of n <= 3 fields, field i with factorisation k[i] and
   // stage coefficient sigma[i]: fields whose factorisation has an even/odd
   // form go through helmholtz_split (one launch each), the rest share one
@@ -146,6 +147,8 @@ class Context {
   double* pfix_[5] = {nullptr};
   double* vinvp_[5] = {nullptr};   // V~^-1 with permuted columns (rhs_helmholtz)
   bool fuse_rhs_ = false;          // rhs + boundary rows + Helmholtz fused (TBD_RHS_UNFUSED=1 disables)
+  bool fuse_p_ = false;            // divergence + pressure solve + projection fused (same switch)
+  void pchain(double q, double cf, const double2* s, double2* p, double2* y, int slot, bool count);
   double* kx_ = nullptr;
   double* ky_ = nullptr;
   double* gridz_ = nullptr;

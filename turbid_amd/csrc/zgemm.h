@@ -108,7 +108,8 @@ double helmholtz_split_flops(int nz, long long P);
 // (permute_columns); split ones take their even/odd form.  y may alias x.
 struct RHField {
   const double2* x = nullptr;     // field (D2 operand, s1 term)
-  const double2* h = nullptr;     // explicit-term accumulator
+  double2* h = nullptr;           // explicit-term accumulator (updated in place when RHArgs::hupd)
+  const void* adv = nullptr;      // nonlinear term (RHArgs::hupd): float2 if RHArgs::adv32, else double2
   double2* y = nullptr;           // solution
   int comp = 0;                   // carrier component (0..2) for the wall-row arrays
   const double* vinv = nullptr;   // permuted V~^-1, or pinv
@@ -140,6 +141,16 @@ struct RHArgs {
   const double2* ehat = nullptr;
   double ws = 0, cbar = 1, re = 0, sc = 0;
   int erosion = 0;
+  // explicit-term update folded in (hupd != 0), the arithmetic of h_update:
+  // h <- c1 h - dt adv; buoyancy from the scalar (w: -K c cs; the slope
+  // component: +K c sn); mean column: + dt f1 (u), + dt f2 (v), the force read
+  // from fdev when set.  h is written back before it enters the rhs.
+  int hupd = 0, adv32 = 0;
+  double c1 = 0, dt = 0;
+  int buoy = 0, slope_dir = 0, force = 0;
+  double K = 0, sn = 0, cs = 0, f1 = 0, f2 = 0;
+  const double* fdev = nullptr;
+  const double2* cb = nullptr;    // scalar state (buoyancy source)
   RHField f[3];
 };
 void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st);
@@ -147,6 +158,40 @@ void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st);
 double rhs_helmholtz_flops(int nz, long long P, bool split);
 // V~^-1 (row-major, nz x lda) with its columns in the order 0..M, nz-1..M+1
 std::vector<double> permute_columns(const std::vector<double>& rowmajor, int nz, int lda);
+
+// Divergence, pressure solve and projection in one kernel (FlatCarrierPhase::
+// Correct, phase_carrier_flat.cc:586-620, boundary_carrier_flat.cc:131-170):
+//   rhs_p = q ((D1 sw + i kx su) + i ky sv), rows 0 and nz-1 zeroed;
+//   p     = the pressure solve of rhs_p through the even/odd form of its
+//           factorisation (mean column gated at nullrow), written to p;
+//   u = su - cf i kx p, v = sv - cf i ky p, w = sw - cf D1 p (mean w := 0),
+//   the wall rows of (i kx p, i ky p, D1 p) (and of (i kx, i ky) D1 p when
+//   open_top) saved as Z_CORRECT does.
+// D1 through its even/odd split (pack_parity, centro-antisymmetric).  s and y
+// may alias (the projection in place).
+struct PArgs {
+  int nz = 0, P = 0, nyc = 0;
+  const double* kx = nullptr;
+  const double* ky = nullptr;
+  const double* d1p = nullptr;    // D1 split, leading dimension ldp
+  int ldp = 0;
+  const double* pinv = nullptr;   // pressure factorisation, even/odd form (leading dimension lda)
+  const double* pnorm = nullptr;
+  const double* plam = nullptr;
+  const double* pfix = nullptr;
+  const double* fix = nullptr;    // natural 4*nz fix array
+  int lda = 0, nullrow = -1;
+  double q = 0, cf = 0;
+  const double2* s = nullptr;     // su, sv, sw at s + f*S
+  double2* p = nullptr;
+  double2* y = nullptr;           // u, v, w at y + f*S
+  long long S = 0;
+  double2* wall_bot = nullptr; double2* wall_top = nullptr;
+  double2* wall2_bot = nullptr; double2* wall2_top = nullptr;
+  int open_top = 0;
+};
+void pressure_chain(const PArgs& a, hipStream_t st);
+double pressure_chain_flops(int nz, long long P);
 
 // Rows 0 and nz-1 of M x for every column (thin contraction: wall values of
 // dU/dz, dV/dz used by the erosion map and the log features).

@@ -860,7 +860,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   const int fi = blockIdx.y;
   const RHField& F = fi == 0 ? a.f[0] : (fi == 1 ? a.f[1] : a.f[2]);
   const double2* x = F.x;
-  const double2* h = F.h;
+  double2* h = F.h;
   const double* fix = F.fix;
   const double* lam = F.lambda;
   const double sigma = F.sigma;
@@ -918,17 +918,55 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   __syncthreads();
   // rhs per row pair (j, nz-1-j); rows 0 and nz-1 are the boundary rows, the
   // interior ones get the forward fix; written back to the two slots read
-  struct In { double2 xl, xh, hl, hh; };
-  batched<3, In>(pd.ncol * (M + 1), [&](int idx) {
+  struct In { double2 xl, xh, hl, hh, al, ah, cl, ch; };
+  auto adv_at = [&](long long e) -> double2 {
+    if (a.adv32) {
+      const float2 v = reinterpret_cast<const float2*>(F.adv)[e];
+      return make_double2((double)v.x, (double)v.y);
+    }
+    return reinterpret_cast<const double2*>(F.adv)[e];
+  };
+  // h <- c1 h - dt adv (+ buoyancy, + mean-column force), as h_update does
+  auto hnew = [&](double2 hv, double2 av, double2 cv, long long col) {
+    hv.x = a.c1 * hv.x - a.dt * av.x;
+    hv.y = a.c1 * hv.y - a.dt * av.y;
+    if (a.buoy) {
+      if (F.comp == 2) {
+        hv.x = hv.x - a.K * cv.x * a.cs;
+        hv.y = hv.y - a.K * cv.y * a.cs;
+      } else if ((F.comp == 0 && a.slope_dir == 1) || (F.comp == 1 && a.slope_dir == 2)) {
+        hv.x = hv.x + a.K * cv.x * a.sn;
+        hv.y = hv.y + a.K * cv.y * a.sn;
+      }
+    }
+    if (a.force && col == 0) {
+      const double f1 = a.fdev ? a.fdev[0] : a.f1, f2 = a.fdev ? a.fdev[1] : a.f2;
+      if (F.comp == 0) hv.x += a.dt * f1;
+      else if (F.comp == 1) hv.x += a.dt * f2;
+    }
+    return hv;
+  };
+  batched<2, In>(pd.ncol * (M + 1), [&](int idx) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
-    In r = {zero, zero, zero, zero};
-    if (c < ncol && j > 0) {
+    In r = {zero, zero, zero, zero, zero, zero, zero, zero};
+    if (c < ncol) {
       const long long o = (c0 + c) * nz;
-      r.xl = x[o + j];
-      r.hl = h[o + j];
-      if (j < M) {
-        r.xh = x[o + nz - 1 - j];
-        r.hh = h[o + nz - 1 - j];
+      const bool upd = a.hupd != 0;
+      if (j > 0) {
+        r.xl = x[o + j];
+        if (j < M) r.xh = x[o + nz - 1 - j];
+      }
+      if (j > 0 || upd) {
+        r.hl = h[o + j];
+        if (j < M) r.hh = h[o + nz - 1 - j];
+      }
+      if (upd) {
+        r.al = adv_at(o + j);
+        if (j < M) r.ah = adv_at(o + nz - 1 - j);
+        if (a.buoy) {
+          r.cl = a.cb[o + j];
+          if (j < M) r.ch = a.cb[o + nz - 1 - j];
+        }
       }
     }
     return r;
@@ -936,6 +974,16 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
     const int c = idx / (M + 1), j = idx - c * (M + 1);
     const double2 bot = bcv[0][c], top = bcv[1][c];
     double2 lo = bot, hi = top;
+    if (a.hupd && c < ncol) {   // every row of h, boundary rows included
+      const long long col = c0 + c;
+      const long long o = col * nz;
+      r.hl = hnew(r.hl, r.al, r.cl, col);
+      h[o + j] = r.hl;
+      if (j < M) {
+        r.hh = hnew(r.hh, r.ah, r.ch, col);
+        h[o + nz - 1 - j] = r.hh;
+      }
+    }
     if (j > 0) {
       const long long col = c0 + c;
       const long long cc = col < a.P ? col : 0;
@@ -1062,6 +1110,225 @@ void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st) {
     else hipLaunchKernelGGL((k_rhs_helmholtz<2, false>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
   }
   TBD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// divergence + pressure solve + projection in one kernel
+// ---------------------------------------------------------------------------
+template <int DEPTH>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArgs a, Panel pd, Panel p1, Panel p2) {
+  extern __shared__ char smem_raw[];
+  __shared__ double2 tfix[2][kSplitMaxCol];
+  const int nz = a.nz;
+  const int M = (nz - 1) / 2;
+  const int ldx = pd.ldx;
+  const long long S = a.S;
+  double2* xs = reinterpret_cast<double2*>(smem_raw);
+  const long long c0 = (long long)blockIdx.x * pd.ncol;
+  const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
+  const double2 zero = make_double2(0.0, 0.0);
+  const double2* su = a.s;
+  const double2* sv = a.s + S;
+  const double2* sw = a.s + 2 * S;
+  struct LoHi { double2 lo, hi; };
+  // even/odd halves of sw for D1
+  batched<5, LoHi>(pd.ncol * (M + 1), [&](int idx) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    LoHi r = {zero, zero};
+    if (c < ncol) {
+      r.lo = sw[(c0 + c) * nz + j];
+      r.hi = sw[(c0 + c) * nz + nz - 1 - j];
+    }
+    return r;
+  }, [&](int idx, LoHi r) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    if (j < M) {
+      xs[c * ldx + j] = make_double2(0.5 * (r.lo.x + r.hi.x), 0.5 * (r.lo.y + r.hi.y));
+      xs[c * ldx + M + 1 + j] = make_double2(0.5 * (r.lo.x - r.hi.x), 0.5 * (r.lo.y - r.hi.y));
+    } else {
+      xs[c * ldx + j] = r.lo;
+    }
+  });
+  __syncthreads();
+  gemm_panel<double, DEPTH>(a.d1p, a.ldp, pd, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
+  __syncthreads();
+  // pressure right-hand side per row pair, boundary rows zeroed, split (s, d)
+  struct UV { double2 ul, uh, vl, vh; };
+  batched<3, UV>(pd.ncol * (M + 1), [&](int idx) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    UV r = {zero, zero, zero, zero};
+    if (c < ncol && j > 0) {
+      const long long o = (c0 + c) * nz;
+      r.ul = su[o + j];
+      r.vl = sv[o + j];
+      if (j < M) {
+        r.uh = su[o + nz - 1 - j];
+        r.vh = sv[o + nz - 1 - j];
+      }
+    }
+    return r;
+  }, [&](int idx, UV r) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    double2 lo = zero, hi = zero;
+    if (j > 0) {
+      const long long col = c0 + c;
+      const long long cc = col < a.P ? col : 0;
+      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double2 sv1 = xs[c * ldx + j], dv1 = xs[c * ldx + M + 1 + j];
+      auto div = [&](double2 v, double2 u0, double2 v0) {
+        const double2 t = cadd<double>(cadd<double>(v, cmuli<double>(kx, u0)), cmuli<double>(ky, v0));
+        return cscale<double>(a.q, t);
+      };
+      lo = div(cadd<double>(sv1, dv1), r.ul, r.vl);
+      if (j < M) hi = div(csub<double>(dv1, sv1), r.uh, r.vh);
+    }
+    if (j == M) {
+      xs[c * ldx + M] = lo;
+    } else {
+      xs[c * ldx + j] = cadd<double>(lo, hi);
+      xs[c * ldx + M + 1 + j] = csub<double>(lo, hi);
+    }
+  });
+  __syncthreads();
+  gemm_panel<double, DEPTH>(a.pinv, a.lda, p1, xs, [&](int row, int c, double2 v) {
+    double r = 1.0;
+    if (row != 0 && row != M + 1) {
+      const long long col = c0 + c;
+      const long long cc = col < a.P ? col : 0;
+      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double k2 = (-(kx * kx)) + (-(ky * ky));
+      r = 1.0 / (((k2 + a.plam[row]) + 0.0));
+      if (row == a.nullrow && col == 0) r = 0.0;
+    }
+    xs[c * ldx + row] = cscale<double>(r, v);
+  });
+  __syncthreads();
+  gemm_panel<double, DEPTH>(a.pnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
+  __syncthreads();
+  // backward fixes: row 0 = fbb . Y, then row N-1 = fbt . Y with row 0 replaced
+  const int part = threadIdx.x & 15;
+  for (int cb = 0; cb < pd.ncol; cb += kNT / 16) {
+    const int c = cb + (threadIdx.x >> 4);
+    double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
+    if (c < pd.ncol) {
+      for (int j = part; j < nz; j += 16) {
+        const double2 v = xs[c * ldx + j];
+        br += a.pfix[j] * v.x;
+        bi += a.pfix[j] * v.y;
+        tr += a.pfix[nz + j] * v.x;
+        ti += a.pfix[nz + j] * v.y;
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      br += __shfl_xor(br, o, 16);
+      bi += __shfl_xor(bi, o, 16);
+      tr += __shfl_xor(tr, o, 16);
+      ti += __shfl_xor(ti, o, 16);
+    }
+    if (part == 0 && c < pd.ncol) {
+      const double fbt0 = a.fix[3 * nz];
+      const double2 y0 = cadd<double>(xs[c * ldx], xs[c * ldx + M + 1]);
+      tfix[0][c] = make_double2(br, bi);
+      tfix[1][c] = make_double2(tr + fbt0 * (br - y0.x), ti + fbt0 * (bi - y0.y));
+    }
+  }
+  __syncthreads();
+  // p to HBM; the panel's (a, b) are p's even/odd halves except at the wall
+  // pair, which is replaced by ((p_0 + p_N-1)/2, (p_0 - p_N-1)/2)
+  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
+    const int c = idx / nz, k = idx - c * nz;
+    double2 v;
+    if (k == 0) {
+      v = tfix[0][c];
+    } else if (k == nz - 1) {
+      v = tfix[1][c];
+    } else if (k <= M) {
+      v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
+    } else {
+      const int i = nz - 1 - k;
+      v = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
+    }
+    a.p[c0 * nz + idx] = v;
+  }
+  if ((int)threadIdx.x < pd.ncol) {
+    const int c = threadIdx.x;
+    const double2 t0 = tfix[0][c], tn = tfix[1][c];
+    xs[c * ldx] = make_double2(0.5 * (t0.x + tn.x), 0.5 * (t0.y + tn.y));
+    xs[c * ldx + M + 1] = make_double2(0.5 * (t0.x - tn.x), 0.5 * (t0.y - tn.y));
+  }
+  __syncthreads();
+  gemm_panel<double, DEPTH>(a.d1p, a.ldp, pd, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
+  __syncthreads();
+  // projection per row pair
+  struct In { double2 pl, ph, ul, uh, vl, vh, wl, wh; };
+  batched<2, In>(pd.ncol * (M + 1), [&](int idx) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    In r = {zero, zero, zero, zero, zero, zero, zero, zero};
+    if (c < ncol) {
+      const long long o = (c0 + c) * nz;
+      const long long lo = o + j, hi = o + nz - 1 - j;
+      r.pl = a.p[lo]; r.ul = su[lo]; r.vl = sv[lo]; r.wl = sw[lo];
+      if (j < M) { r.ph = a.p[hi]; r.uh = su[hi]; r.vh = sv[hi]; r.wh = sw[hi]; }
+    }
+    return r;
+  }, [&](int idx, In r) {
+    const int c = idx / (M + 1), j = idx - c * (M + 1);
+    if (c >= ncol) return;
+    const long long col = c0 + c;
+    const double kx = a.kx[col / a.nyc], ky = a.ky[col % a.nyc];
+    const double2 s2 = xs[c * ldx + j], d2 = xs[c * ldx + M + 1 + j];
+    auto put = [&](int row, double2 v, double2 pv, double2 u0, double2 v0, double2 w0) {
+      const long long off = col * nz + row;
+      const double2 gx = cmuli<double>(kx, pv), gy = cmuli<double>(ky, pv);
+      a.y[off] = csub<double>(u0, cscale<double>(a.cf, gx));
+      a.y[S + off] = csub<double>(v0, cscale<double>(a.cf, gy));
+      double2 w = csub<double>(w0, cscale<double>(a.cf, v));
+      if (col == 0) w = make_double2(0.0, 0.0);
+      a.y[2 * S + off] = w;
+      if (row == 0 || row == nz - 1) {
+        double2* wb = (row == 0) ? a.wall_bot : a.wall_top;
+        const long long P = a.P;
+        wb[col] = gx;
+        wb[P + col] = gy;
+        wb[2 * P + col] = v;
+        if (a.open_top) {
+          double2* w2 = (row == 0) ? a.wall2_bot : a.wall2_top;
+          w2[col] = cmuli<double>(kx, v);
+          w2[P + col] = cmuli<double>(ky, v);
+        }
+      }
+    };
+    put(j, cadd<double>(s2, d2), r.pl, r.ul, r.vl, r.wl);
+    if (j < M) put(nz - 1 - j, csub<double>(d2, s2), r.ph, r.uh, r.vh, r.wh);
+  });
+}
+
+void pressure_chain(const PArgs& a, hipStream_t st) {
+  TBD_CHECK(a.nz >= 7 && a.nz % 2 == 1 && a.P > 0 && a.d1p && a.pinv && a.pnorm && a.pfix && a.fix &&
+                a.ldp >= (a.nz + 1) / 2 && a.ldp % 4 == 0 && a.lda >= (a.nz + 1) / 2 && a.lda % 4 == 0,
+            "pressure_chain > bad shape");
+  const Panel pd0 = make_panel(a.nz, sizeof(double2), kLdsLimit, -1);
+  Panel q1, q2;
+  make_split_panels(a.nz, kLdsLimit, q1, q2);
+  const int ncg = std::min(std::min(pd0.ncg, q1.ncg), q2.ncg);
+  const int ldx = pd0.ldx;
+  const Panel pd = with_ncg(pd0, ncg, ldx), p1 = with_ncg(q1, ncg, ldx), p2 = with_ncg(q2, ncg, ldx);
+  TBD_CHECK(pd.ncol <= kSplitMaxCol, "pressure_chain > too many columns per panel");
+  const int depth = std::max(std::max(buffered_depth(pd), buffered_depth(p1)), buffered_depth(p2));
+  TBD_CHECK(depth <= kMaxU, "pressure_chain > nz too large for the register-buffered contraction");
+  const size_t lds = sizeof(double2) * pd.ncol * ldx;
+  TBD_CHECK(lds + sizeof(double2) * 2 * kSplitMaxCol <= 160 * 1024, "pressure_chain > nz too large for one LDS panel");
+  dim3 grid(ceil_div(a.P, pd.ncol), 1);
+  if (depth <= 1) hipLaunchKernelGGL(k_pressure_chain<1>, grid, dim3(kNT), lds, st, a, pd, p1, p2);
+  else hipLaunchKernelGGL(k_pressure_chain<2>, grid, dim3(kNT), lds, st, a, pd, p1, p2);
+  TBD_HIP(hipGetLastError());
+}
+
+double pressure_chain_flops(int nz, long long P) {
+  const double M = (nz - 1) / 2;
+  const double d1 = 4.0 * ((M + 1) * (M + 1) + (M + 1) * M) * (double)P;
+  return 2 * d1 + helmholtz_split_flops(nz, P);
 }
 
 std::vector<double> permute_columns(const std::vector<double>& rm, int nz, int lda) {

@@ -49,6 +49,7 @@ KCLASSES = {
     "xfused64": (8, "bytes"), "xfused32": (9, "bytes"), "zgemm64": (10, "flops"), "zgemm32": (11, "flops"),
     "helmholtz": (12, "flops"), "products64": (13, "bytes"), "products32": (14, "bytes"), "other": (15, None),
     "helmholtz_split": (16, "flops"), "rhs_helmholtz": (17, "flops"), "rhs_helmholtz_split": (18, "flops"),
+    "pressure_chain": (19, "flops"),
 }
 KCLASS = {k: v[0] for k, v in KCLASSES.items()}
 
