@@ -239,20 +239,39 @@ __device__ __forceinline__ void xback_regs(const C2<T>* __restrict__ in, int ny,
   constexpr int N = N1 * N2;
   constexpr int S = xf_seqs<N1, N2, NT>();
   const int t = threadIdx.x;
-  __syncthreads();
-  for (int idx = t; idx < S * N; idx += NT) {
+  // all of a thread's spectra are loaded before any is packed into LDS
+  constexpr int NI = (S * N + NT - 1) / NT;
+  C2<T> ra[NI], rb[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int idx = t + i * NT;
     const int s = idx % S, kk = idx / S;
     const int ya = y0 + 2 * s;
     const bool direct = kk < mx;
     const int kx = direct ? kk : N - kk;
-    C2<T> z = mk<T>((T)0, (T)0);
-    if (direct || kx < mx) {
-      C2<T> A = (ya < ny) ? in[(long long)kx * ny + ya] : mk<T>((T)0, (T)0);
-      C2<T> B = (ya + 1 < ny) ? in[(long long)kx * ny + ya + 1] : mk<T>((T)0, (T)0);
-      if (kx == 0 || 2 * kx == N) { A.y = (T)0; B.y = (T)0; }
-      z = direct ? mk<T>(A.x - B.y, A.y + B.x) : mk<T>(A.x + B.y, B.x - A.y);
+    ra[i] = mk<T>((T)0, (T)0);
+    rb[i] = ra[i];
+    if (idx < S * N && (direct || kx < mx)) {
+      if (ya < ny) ra[i] = in[(long long)kx * ny + ya];
+      if (ya + 1 < ny) rb[i] = in[(long long)kx * ny + ya + 1];
     }
-    L[s * (N + 1) + kk] = z;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int idx = t + i * NT;
+    if (idx < S * N) {
+      const int s = idx % S, kk = idx / S;
+      const bool direct = kk < mx;
+      const int kx = direct ? kk : N - kk;
+      C2<T> z = mk<T>((T)0, (T)0);
+      if (direct || kx < mx) {
+        C2<T> A = ra[i], B = rb[i];
+        if (kx == 0 || 2 * kx == N) { A.y = (T)0; B.y = (T)0; }
+        z = direct ? mk<T>(A.x - B.y, A.y + B.x) : mk<T>(A.x + B.y, B.x - A.y);
+      }
+      L[s * (N + 1) + kk] = z;
+    }
   }
   __syncthreads();
   C2<T> v[N1];
