@@ -249,6 +249,22 @@ def test_graph_replay_matches_eager(precision, monkeypatch):
     assert np.array_equal(fe, fg)
 
 
+@pytest.mark.parametrize("precision,graph", [(0, "1"), (1, "1"), (1, "0")])
+def test_overlapped_scalar_matches_serial(precision, graph, monkeypatch):
+    # the scalar of stage s on a second stream beside the carrier Predict of
+    # s + 1 gives the serial result bit for bit (captured and eager steps)
+    cfg = small_config(precision=precision)
+    monkeypatch.setenv("TBD_GRAPH", graph)
+    monkeypatch.setenv("TBD_NO_OVERLAP", "1")
+    (us, ps, cs), fs = _run_solver_only(cfg, 7)
+    monkeypatch.setenv("TBD_NO_OVERLAP", "0")
+    (uo, po, co), fo = _run_solver_only(cfg, 7)
+    assert np.array_equal(us, uo)
+    assert np.array_equal(ps, po)
+    assert np.array_equal(cs, co)
+    assert np.array_equal(fs, fo)
+
+
 def test_phase_stats_match_host():
     # on-device phase statistics against the same sums on the downloaded state
     cfg = small_config()

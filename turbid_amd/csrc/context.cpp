@@ -206,6 +206,11 @@ Context::Context(const tbd_config& c) : cfg(c) {
   xpb_ = (double)d.nz * d.mx * d.ny;
   TBD_CHECK(d.nyc >= 2 && d.mx >= 1, "Context > grid too small for the 3/2 rule");
   TBD_HIP(hipStreamCreate(&st_));
+  TBD_HIP(hipStreamCreate(&st2_));
+  TBD_HIP(hipEventCreateWithFlags(&evf_, hipEventDisableTiming));
+  TBD_HIP(hipEventCreateWithFlags(&evj_, hipEventDisableTiming));
+  if (const char* e = std::getenv("TBD_NO_OVERLAP")) overlap_ = std::string(e) != "1";
+  else overlap_ = true;
   fft_.create(d);
   if (const char* e = std::getenv("TBD_FUSED")) fused_ = std::string(e) == "1";
   dt_ = olddt_ = cfg.dt;
@@ -343,8 +348,15 @@ Context::Context(const tbd_config& c) : cfg(c) {
     advc_ = alloc(Sc);
     cphys_ = (double*)alloc(sizeof(double) * d.R);
     const size_t plane = (size_t)d.nx * d.ny;
-    dudz_bot_ = alloc(ts * plane);
-    dvdz_bot_ = alloc(ts * plane);
+    for (int k = 0; k < 2; ++k) {
+      dudz_bot_[k] = alloc(ts * plane);
+      dvdz_bot_[k] = alloc(ts * plane);
+    }
+    sgrad_ = alloc(3 * Sc);
+    sprod_ = alloc(3 * Sc);
+    sgphys_ = alloc(3 * Rt);
+    saphys_ = alloc(Rt);
+    stmp_ = alloc(std::max((size_t)3 * 2 * ts, sizeof(double2)) * fft_tmp_elems(d, 1));
     ephys_ = (double*)alloc(sizeof(double) * plane);
     ehat_ = (double2*)alloc(sizeof(double2) * d.P);
     wrows_ = (double2*)alloc(sizeof(double2) * 4 * d.P);
@@ -362,9 +374,13 @@ Context::Context(const tbd_config& c) : cfg(c) {
 
 Context::~Context() {
   if (st_) (void)hipStreamSynchronize(st_);
+  if (st2_) (void)hipStreamSynchronize(st2_);
   drop_graphs();
   for (void* p : allocs_) (void)hipFree(p);
   fft_.destroy();
+  if (evf_) (void)hipEventDestroy(evf_);
+  if (evj_) (void)hipEventDestroy(evj_);
+  if (st2_) (void)hipStreamDestroy(st2_);
   if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -391,7 +407,7 @@ void Context::drop_graphs() {
 // previous dt) is advanced the same way as by the eager substages.
 void Context::step(double time, int alternation) {
   if (!use_graphs_ || prof.on) {
-    for (int s = 0; s < 3; ++s) substage(s, time, alternation);
+    stages(time, alternation);
     return;
   }
   double f1 = 0, f2 = 0;
@@ -414,7 +430,7 @@ void Context::step(double time, int alternation) {
   capturing_ = true;
   TBD_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
   try {
-    for (int s = 0; s < 3; ++s) substage(s, time, alternation);
+    stages(time, alternation);
   } catch (...) {
     capturing_ = false;
     hipGraph_t broken = nullptr;
@@ -564,8 +580,9 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       // tile are written by different workgroups while the inputs are read
       CT* xout = (CT*)gphys_;
       launch(is32<T>() ? KC_XFU32 : KC_XFU64, 12 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] {
-        xfused_conv<T>(fft_, tmp, uvwnl, R, (T)0, (T)0, (T)0, 3, xout, erosion_planes ? (T*)dudz_bot_ : nullptr,
-                       erosion_planes ? (T*)dvdz_bot_ : nullptr, st_);
+        xfused_conv<T>(fft_, tmp, uvwnl, R, (T)0, (T)0, (T)0, 3, xout,
+                       erosion_planes ? (T*)dudz_bot_[stage & 1] : nullptr,
+                       erosion_planes ? (T*)dvdz_bot_[stage & 1] : nullptr, st_);
       });
       ypf<T, CT>(xout, adv, S, 3, 0, d.nz);
     } else {
@@ -574,8 +591,8 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       fwd<T, CT>(aphys, R, adv, S, 3, 0, d.nz, tmp);
       if (erosion_planes) {
         const size_t pl = (size_t)d.nx * d.ny * sizeof(T);
-        TBD_HIP(hipMemcpyAsync(dudz_bot_, gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
-        TBD_HIP(hipMemcpyAsync(dvdz_bot_, gphys + 5 * R, pl, hipMemcpyDeviceToDevice, st_));
+        TBD_HIP(hipMemcpyAsync(dudz_bot_[stage & 1], gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
+        TBD_HIP(hipMemcpyAsync(dvdz_bot_[stage & 1], gphys + 5 * R, pl, hipMemcpyDeviceToDevice, st_));
       }
     }
   } else {
@@ -604,8 +621,8 @@ void Context::carrier_predict(int stage, double time, int alternation) {
         zrows_wall(d1_, lda_, d.nz, uvw_ + S, d.P, wrows_ + 2 * d.P, wrows_ + 3 * d.P, st_);
       });
       const long long plane = (long long)d.nx * d.ny;
-      bwd<T, double2>(wrows_, 2 * d.P, (T*)dudz_bot_, plane, 1, 0, 1, tmp, 1);
-      bwd<T, double2>(wrows_ + 2 * d.P, 2 * d.P, (T*)dvdz_bot_, plane, 1, 0, 1, tmp, 1);
+      bwd<T, double2>(wrows_, 2 * d.P, (T*)dudz_bot_[stage & 1], plane, 1, 0, 1, tmp, 1);
+      bwd<T, double2>(wrows_ + 2 * d.P, 2 * d.P, (T*)dvdz_bot_[stage & 1], plane, 1, 0, 1, tmp, 1);
     }
   }
   // H = c1 H - dt adv, buoyancy, mean-mode force (:294-504)
@@ -747,12 +764,13 @@ void Context::scalar_correct(int stage, int alternation) {
   const double c1 = coef_[0][stage], c2 = coef_[1][stage], c3 = coef_[2][stage];
   const long long S = d.S, R = d.R;
   T* uvwnl = (T*)uvwnl_;
-  T* gphys = (T*)gphys_;
-  T* aphys = (T*)aphys_;
-  CT* grad = (CT*)grad_;
-  CT* prod = (CT*)prod_;
+  // own work buffers (not the carrier's): see Context::overlap_
+  T* gphys = (T*)sgphys_;
+  T* aphys = (T*)saphys_;
+  CT* grad = (CT*)sgrad_;
+  CT* prod = (CT*)sprod_;
   CT* advc = (CT*)advc_;
-  CT* tmp = (CT*)tmp_;
+  CT* tmp = (CT*)stmp_;
   // advecting velocity = carrier velocity + settling shift on the mean mode
   // (:256-279); its physical form is the carrier's plus constants
   const double ws = cfg.settling_velocity;
@@ -766,8 +784,8 @@ void Context::scalar_correct(int stage, int alternation) {
       ypf<T, CT>(tmp, prod, S, 3, 0, d.nz);
     } else {
       bwd<T, double2>(c_, S, aphys, R, 1, 0, d.nz, tmp);
-      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 7.0 * R * sizeof(T), [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys + 6 * R, R, st_); });
-      fwd<T, CT>(gphys + 6 * R, R, prod, S, 3, 0, d.nz, tmp);
+      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 7.0 * R * sizeof(T), [&] { scalar_products_div<T, T>(uvwnl, su, sv, sw, aphys, gphys, R, st_); });
+      fwd<T, CT>(gphys, R, prod, S, 3, 0, d.nz, tmp);
     }
     ZArgs a;
     a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
@@ -776,17 +794,17 @@ void Context::scalar_correct(int stage, int alternation) {
   } else {             // convective form (:584-592)
     ZArgs a;
     a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
-    a.x = c_; a.xs = S; a.y = grad + 6 * S; a.ys = S; a.kx = kx_; a.ky = ky_;
+    a.x = c_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
     launch(is32<T>() ? KC_Z32 : KC_Z64, zwork(), [&] { zgemm<T, double2, CT>(a, 1, st_); });
     if (fused_) {
-      ypb<T, CT>(grad + 6 * S, S, tmp, 3, 0, d.nz);
+      ypb<T, CT>(grad, S, tmp, 3, 0, d.nz);
       launch(is32<T>() ? KC_XFU32 : KC_XFU64, 4 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] {
         xfused_conv<T>(fft_, tmp, uvwnl, R, su, sv, sw, 1, tmp, nullptr, nullptr, st_);
       });
       ypf<T, CT>(tmp, advc, S, 1, 0, d.nz);
     } else {
-      bwd<T, CT>(grad + 6 * S, S, gphys + 6 * R, R, 3, 0, d.nz, tmp);
-      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 7.0 * R * sizeof(T), [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys + 6 * R, aphys, R, st_); });
+      bwd<T, CT>(grad, S, gphys, R, 3, 0, d.nz, tmp);
+      launch(is32<T>() ? KC_PROD32 : KC_PROD64, 7.0 * R * sizeof(T), [&] { scalar_products_conv<T>(uvwnl, su, sv, sw, gphys, aphys, R, st_); });
       fwd<T, CT>(aphys, R, advc, S, 1, 0, d.nz, tmp);
     }
   }
@@ -810,10 +828,10 @@ void Context::scalar_correct(int stage, int alternation) {
   if (erosion) {
     const long long plane = (long long)d.nx * d.ny;
     launch(KC_OTHER, 0.0, [&] {
-      erosion_map<T>((const T*)dudz_bot_, (const T*)dvdz_bot_, ephys_, plane, cfg.reynolds,
+      erosion_map<T>((const T*)dudz_bot_[stage & 1], (const T*)dvdz_bot_[stage & 1], ephys_, plane, cfg.reynolds,
                      cfg.critical_shear_stress, cfg.erosion_rate, st_);
     });
-    fwd<double, double2>(ephys_, plane, ehat_, d.P, 1, 0, 1, (double2*)tmp_, 1);
+    fwd<double, double2>(ephys_, plane, ehat_, d.P, 1, 0, 1, (double2*)stmp_, 1);
   }
   const double wsc = cfg.settling_velocity * std::cos(cfg.slope);
   if (!fuse_rhs_)
@@ -849,15 +867,45 @@ void Context::scalar_correct(int stage, int alternation) {
     }
   }
   // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
-  bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)tmp_);
+  bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)stmp_);
   // the filter raises filt_ when it changes a plane; otherwise the transform
   // back would reproduce c except for the kx = 0 plane's projection, which is
   // applied directly
   fill_zero(filt_, sizeof(int) * 2, st_);
   launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, filt_, st_); });
-  fwd<double, double2>(cphys_, R, c_, S, 1, 0, d.nz, (double2*)tmp_, -1, filt_);
+  fwd<double, double2>(cphys_, R, c_, S, 1, 0, d.nz, (double2*)stmp_, -1, filt_);
   launch(KC_OTHER, 0.0, [&] { kx0_round_trip(c_, d.nyc, d.nz, filt_, st_); });
   launch(KC_OTHER, 0.0, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });
+}
+
+void Context::stages(double time, int alternation) {
+  const bool buoy = cfg.nphase > 1 && cfg.affected_by_scalar_phase;
+  if (!overlap_ || cfg.nphase < 2 || buoy || prof.on) {
+    for (int s = 0; s < 3; ++s) substage(s, time, alternation);
+    return;
+  }
+  const hipStream_t main = st_;
+  for (int s = 0; s < 3; ++s) {
+    if (cfg.precision == 1) carrier_predict<float>(s, time, alternation);
+    else carrier_predict<double>(s, time, alternation);
+    // the scalar of stage s - 1 has read the physical velocity Correct overwrites
+    if (s > 0) TBD_HIP(hipStreamWaitEvent(main, evj_, 0));
+    if (cfg.precision == 1) carrier_correct<float>(s);
+    else carrier_correct<double>(s);
+    TBD_HIP(hipEventRecord(evf_, main));
+    TBD_HIP(hipStreamWaitEvent(st2_, evf_, 0));
+    st_ = st2_;
+    try {
+      if (cfg.precision == 1) scalar_correct<float>(s, alternation);
+      else scalar_correct<double>(s, alternation);
+    } catch (...) {
+      st_ = main;
+      throw;
+    }
+    st_ = main;
+    TBD_HIP(hipEventRecord(evj_, st2_));
+  }
+  TBD_HIP(hipStreamWaitEvent(main, evj_, 0));
 }
 
 void Context::substage(int stage, double time, int alternation) {

@@ -148,6 +148,17 @@ class Context {
   double* vinvp_[5] = {nullptr};   // V~^-1 with permuted columns (rhs_helmholtz)
   bool fuse_rhs_ = false;          // rhs + boundary rows + Helmholtz fused (TBD_RHS_UNFUSED=1 disables)
   bool fuse_p_ = false;            // divergence + pressure solve + projection fused (same switch)
+  // The scalar's Correct of stage s reads the carrier's physical velocity of
+  // stage s and writes only scalar state; without buoyancy the carrier never
+  // reads the scalar.  So in step() the scalar of stage s runs on a second
+  // stream beside the carrier Predict of stage s + 1, joined before the
+  // carrier Correct that overwrites the physical velocity (TBD_NO_OVERLAP=1
+  // disables; off while profiling).  The scalar has its own work buffers.
+  bool overlap_ = false;
+  hipStream_t st2_ = nullptr;
+  hipEvent_t evf_ = nullptr, evj_ = nullptr;
+  void *sgrad_ = nullptr, *sprod_ = nullptr, *sgphys_ = nullptr, *saphys_ = nullptr, *stmp_ = nullptr;
+  void stages(double time, int alternation);
   void pchain(double q, double cf, const double2* s, double2* p, double2* y, int slot, bool count);
   double* kx_ = nullptr;
   double* ky_ = nullptr;
@@ -175,8 +186,11 @@ class Context {
   double2* wall2_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // 2P each
   int wslot_ = 0;          // slot holding m1
   // erosion
-  void* dudz_bot_ = nullptr;   // T, plane
-  void* dvdz_bot_ = nullptr;
+  // bottom planes of du/dz, dv/dz (T) written by the carrier's Predict and
+  // read by the scalar's Correct of the same stage; one pair per stage parity,
+  // so the scalar of stage s may run beside the carrier Predict of s + 1
+  void* dudz_bot_[2] = {nullptr, nullptr};
+  void* dvdz_bot_[2] = {nullptr, nullptr};
   double* ephys_ = nullptr;
   double2* ehat_ = nullptr;
   double2* wrows_ = nullptr;   // 4P: bottom/top rows of D1 u, D1 v
