@@ -481,7 +481,7 @@ static constexpr size_t kLdsLimit = 80 * 1024;   // two workgroups per CU
 // as one coalesced pass (row fastest) that re-reads the input column where the
 // mode needs it.
 template <typename T, typename TX, typename TY, int DEPTH>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_zgemm(ZArgs a, Panel pn) {
+__global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH == 1 ? 4 : 2)) void k_zgemm(ZArgs a, Panel pn) {
   extern __shared__ char smem_raw[];
   C2<T>* xs = reinterpret_cast<C2<T>*>(smem_raw);
   const int nz = a.nz;
@@ -526,7 +526,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_zgemm(ZArgs a, Pane
   gemm_panel<T, DEPTH>((const T*)a.op, a.lda, pn, xs, [&](int row, int c, C2<T> v) { xs[c * ldx + row] = v; });
   __syncthreads();
   const int M = (nz - 1) / 2;
-  batched<4, EpiIn<T>>(ncol * nz, [&](int idx) {
+  batched<sizeof(T) == 4 ? 2 : 4, EpiIn<T>>(ncol * nz, [&](int idx) {
     return epi_load<T, TX>(a, f, c0 * nz + idx);
   }, [&](int idx, const EpiIn<T>& in) {
     const int c = idx / nz, row = idx - c * nz;
