@@ -606,12 +606,14 @@ void Context::carrier_predict(int stage, double time, int alternation) {
     }
     // adv_i = (i kx F_i0 + i ky F_i1) + D1 F_i2 ; rows (uu,uv,uw) (uv,vv,vw) (uw,vw,ww)
     const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    for (int f = 0; f < 3; ++f) {
+    {
       ZArgs a;
       a.mode = Z_GRADTMP; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
-      a.x = prod + rows[f][2] * S; a.e0 = prod + rows[f][0] * S; a.e1 = prod + rows[f][1] * S;
-      a.y = adv + f * S; a.kx = kx_; a.ky = ky_;
-      launch(is32<T>() ? KC_Z32 : KC_Z64, zwork(), [&] { zgemm<T, CT, CT>(a, 1, st_); });
+      for (int f = 0; f < 3; ++f) {
+        a.xf[f] = prod + rows[f][2] * S; a.e0f[f] = prod + rows[f][0] * S; a.e1f[f] = prod + rows[f][1] * S;
+      }
+      a.y = adv; a.ys = S; a.kx = kx_; a.ky = ky_;
+      launch(is32<T>() ? KC_Z32 : KC_Z64, 3 * zwork(), [&] { zgemm<T, CT, CT>(a, 3, st_); });
     }
     // Gradient(u), Gradient(v) + DftBackward for the erosion map: only the
     // bottom plane of du/dz, dv/dz is read (phase_scalar_flat.cc:349-351)

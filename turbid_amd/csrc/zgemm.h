@@ -33,6 +33,11 @@ struct ZArgs {
   int nyc = 0;           // spectral y size (column -> (ix, iy))
   const void* op = nullptr;       // operator (T*), shared by all fields of the launch
   const void* x = nullptr;  long long xs = 0;   // field f at x + f*xs (complex elements)
+  // when xf[0] is set: field f reads x, e0, e1 at xf[f], e0f[f], e1f[f]
+  // (fields whose operands are not evenly strided)
+  const void* xf[3] = {nullptr, nullptr, nullptr};
+  const void* e0f[3] = {nullptr, nullptr, nullptr};
+  const void* e1f[3] = {nullptr, nullptr, nullptr};
   void* y = nullptr;        long long ys = 0;
   const void* e0 = nullptr; const void* e1 = nullptr; const void* e2 = nullptr;
   long long es = 0;                             // stride between fields of e0/e1

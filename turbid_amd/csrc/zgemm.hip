@@ -373,6 +373,23 @@ __device__ __forceinline__ void batched(int n, Load load, Store store) {
   }
 }
 
+// Operand bases of field f (evenly strided, or per field when xf[0] is set).
+__device__ __forceinline__ const void* pick3(const void* const* p, int f) {
+  return f == 0 ? p[0] : (f == 1 ? p[1] : p[2]);
+}
+template <typename TX>
+__device__ __forceinline__ const TX* zx(const ZArgs& a, int f) {
+  return a.xf[0] ? (const TX*)pick3(a.xf, f) : (const TX*)a.x + f * a.xs;
+}
+template <typename TX>
+__device__ __forceinline__ const TX* ze0(const ZArgs& a, int f) {
+  return a.xf[0] ? (const TX*)pick3(a.e0f, f) : (const TX*)a.e0 + f * a.es;
+}
+template <typename TX>
+__device__ __forceinline__ const TX* ze1(const ZArgs& a, int f) {
+  return a.xf[0] ? (const TX*)pick3(a.e1f, f) : (const TX*)a.e1 + f * a.es;
+}
+
 // The global inputs an epilogue mode reads besides the contraction result.
 template <typename T> struct EpiIn { C2<T> x, e0, e1, e2; };
 
@@ -380,7 +397,7 @@ template <typename T, typename TX>
 __device__ __forceinline__ EpiIn<T> epi_load(const ZArgs& a, int f, long long off) {
   EpiIn<T> in;
   in.x = in.e0 = in.e1 = in.e2 = mk<T>((T)0, (T)0);
-  const TX* x = (const TX*)a.x + f * a.xs;
+  const TX* x = zx<TX>(a, f);
   switch (a.mode) {
     case Z_GRAD:
       in.x = ld<T, TX>(x + off);
@@ -400,8 +417,8 @@ __device__ __forceinline__ EpiIn<T> epi_load(const ZArgs& a, int f, long long of
       in.e2 = ld<T, TX>((const TX*)a.e0 + 2 * a.es + off);
       break;
     case Z_GRADTMP:
-      in.e0 = ld<T, TX>((const TX*)a.e0 + f * a.es + off);
-      in.e1 = ld<T, TX>((const TX*)a.e1 + f * a.es + off);
+      in.e0 = ld<T, TX>(ze0<TX>(a, f) + off);
+      in.e1 = ld<T, TX>(ze1<TX>(a, f) + off);
       break;
     default:
       break;
@@ -489,7 +506,7 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
   const int f = blockIdx.y;
   const long long c0 = (long long)blockIdx.x * pn.ncol;
   const int ncol = (int)min((long long)pn.ncol, a.P - c0);
-  const TX* x = (const TX*)a.x + f * a.xs + c0 * nz;
+  const TX* x = zx<TX>(a, f) + c0 * nz;
   const C2<T> zero = mk<T>((T)0, (T)0);
   if (pn.parity == 0) {
     batched<8, C2<T>>(pn.ncol * nz, [&](int idx) {
