@@ -714,6 +714,68 @@ __host__ inline void make_split_panels(int nz, size_t lds_limit, Panel& p1, Pane
   p1.ntail = tailrows * p1.ncol;
 }
 
+// Backward fixes of every panel column, 16 lanes per column: t0 = wb . Y and
+// tN = wt . Y + fbt0 (t0 - Y_0) (the top fix sees row 0 already replaced);
+// Y is the panel column in natural order, or its split (a, b) when SPLIT, in
+// which case Y_0 = a_0 + b_0 and wb, wt are the folded weights.
+template <bool SPLIT>
+__device__ __forceinline__ void backward_fixes(const double2* xs, int ldx, int ncolp, int nz, int M,
+                                               const double* wb, const double* wt, double fbt0,
+                                               double2 (*tfix)[kSplitMaxCol]) {
+  const int part = threadIdx.x & 15;
+  for (int cb = 0; cb < ncolp; cb += kNT / 16) {
+    const int c = cb + (threadIdx.x >> 4);
+    double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
+    if (c < ncolp) {
+      for (int j = part; j < nz; j += 16) {
+        const double2 v = xs[c * ldx + j];
+        br += wb[j] * v.x;
+        bi += wb[j] * v.y;
+        tr += wt[j] * v.x;
+        ti += wt[j] * v.y;
+      }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+      br += __shfl_xor(br, o, 16);
+      bi += __shfl_xor(bi, o, 16);
+      tr += __shfl_xor(tr, o, 16);
+      ti += __shfl_xor(ti, o, 16);
+    }
+    if (part == 0 && c < ncolp) {
+      double2 y0 = xs[c * ldx];
+      if (SPLIT) y0 = cadd<double>(y0, xs[c * ldx + M + 1]);
+      tfix[0][c] = make_double2(br, bi);
+      tfix[1][c] = make_double2(tr + fbt0 * (br - y0.x), ti + fbt0 * (bi - y0.y));
+    }
+  }
+}
+
+// Writes the panel's ncol columns to y (column stride nz): rows 0 and nz-1
+// from tfix, the others from the panel (natural order, or y_i = a_i + b_i,
+// y_{nz-1-i} = a_i - b_i when SPLIT).
+template <bool SPLIT>
+__device__ __forceinline__ void store_columns(const double2* xs, int ldx, int ncol, int nz, int M,
+                                              const double2 (*tfix)[kSplitMaxCol], double2* y) {
+  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
+    const int c = idx / nz, k = idx - c * nz;
+    double2 v;
+    if (k == 0) {
+      v = tfix[0][c];
+    } else if (k == nz - 1) {
+      v = tfix[1][c];
+    } else if (!SPLIT) {
+      v = xs[c * ldx + k];
+    } else if (k <= M) {
+      v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
+    } else {
+      const int i = nz - 1 - k;
+      v = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
+    }
+    y[idx] = v;
+  }
+}
+
 template <int DEPTH>
 __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HArgs a, Panel p1, Panel p2) {
   extern __shared__ char smem_raw[];
@@ -780,54 +842,9 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
   __syncthreads();
   gemm_panel<double, DEPTH>(a.f[0].vnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  // row 0 = fbb . Y, then row N-1 = fbt . Y with row 0 replaced
-  const double* wb = a.f[0].pfix;
-  const double* wt = a.f[0].pfix + nz;
-  const double fbt0 = fix[3 * nz];
-  const int part = threadIdx.x & 15;
-  for (int cb = 0; cb < p1.ncol; cb += kNT / 16) {
-    const int c = cb + (threadIdx.x >> 4);
-    double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
-    if (c < p1.ncol) {
-      for (int j = part; j < nz; j += 16) {
-        const double2 v = xs[c * ldx + j];
-        br += wb[j] * v.x;
-        bi += wb[j] * v.y;
-        tr += wt[j] * v.x;
-        ti += wt[j] * v.y;
-      }
-    }
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-      br += __shfl_xor(br, o, 16);
-      bi += __shfl_xor(bi, o, 16);
-      tr += __shfl_xor(tr, o, 16);
-      ti += __shfl_xor(ti, o, 16);
-    }
-    if (part == 0 && c < p1.ncol) {
-      const double2 a0 = xs[c * ldx], b0 = xs[c * ldx + M + 1];
-      const double y0r = a0.x + b0.x, y0i = a0.y + b0.y;
-      tfix[0][c] = make_double2(br, bi);
-      tfix[1][c] = make_double2(tr + fbt0 * (br - y0r), ti + fbt0 * (bi - y0i));
-    }
-  }
+  backward_fixes<true>(xs, ldx, p1.ncol, nz, M, a.f[0].pfix, a.f[0].pfix + nz, fix[3 * nz], tfix);
   __syncthreads();
-  double2* y = a.f[0].y + c0 * nz;
-  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
-    const int c = idx / nz, k = idx - c * nz;
-    double2 v;
-    if (k == 0) {
-      v = tfix[0][c];
-    } else if (k == nz - 1) {
-      v = tfix[1][c];
-    } else if (k <= M) {
-      v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
-    } else {
-      const int i = nz - 1 - k;
-      v = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
-    }
-    y[idx] = v;
-  }
+  store_columns<true>(xs, ldx, ncol, nz, M, tfix, a.f[0].y + c0 * nz);
 }
 
 void helmholtz_split(const HArgs& a, hipStream_t st) {
@@ -1045,56 +1062,10 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   __syncthreads();
   gemm_panel<double, DEPTH>(F.vnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  // row 0 = fbb . Y, then row N-1 = fbt . Y with row 0 replaced
-  const int part = threadIdx.x & 15;
-  for (int cb = 0; cb < pd.ncol; cb += kNT / 16) {
-    const int c = cb + (threadIdx.x >> 4);
-    double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
-    const double* wb = SPLIT ? F.pfix : fix + 2 * nz;
-    const double* wt = SPLIT ? F.pfix + nz : fix + 3 * nz;
-    if (c < pd.ncol) {
-      for (int j = part; j < nz; j += 16) {
-        const double2 v = xs[c * ldx + j];
-        br += wb[j] * v.x;
-        bi += wb[j] * v.y;
-        tr += wt[j] * v.x;
-        ti += wt[j] * v.y;
-      }
-    }
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-      br += __shfl_xor(br, o, 16);
-      bi += __shfl_xor(bi, o, 16);
-      tr += __shfl_xor(tr, o, 16);
-      ti += __shfl_xor(ti, o, 16);
-    }
-    if (part == 0 && c < pd.ncol) {
-      const double fbt0 = fix[3 * nz];
-      double2 y0 = xs[c * ldx];
-      if (SPLIT) y0 = cadd<double>(y0, xs[c * ldx + M + 1]);
-      tfix[0][c] = make_double2(br, bi);
-      tfix[1][c] = make_double2(tr + fbt0 * (br - y0.x), ti + fbt0 * (bi - y0.y));
-    }
-  }
+  backward_fixes<SPLIT>(xs, ldx, pd.ncol, nz, M, SPLIT ? F.pfix : fix + 2 * nz, SPLIT ? F.pfix + nz : fix + 3 * nz,
+                        fix[3 * nz], tfix);
   __syncthreads();
-  double2* y = F.y + c0 * nz;
-  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
-    const int c = idx / nz, k = idx - c * nz;
-    double2 v;
-    if (k == 0) {
-      v = tfix[0][c];
-    } else if (k == nz - 1) {
-      v = tfix[1][c];
-    } else if (!SPLIT) {
-      v = xs[c * ldx + k];
-    } else if (k <= M) {
-      v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
-    } else {
-      const int i = nz - 1 - k;
-      v = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
-    }
-    y[idx] = v;
-  }
+  store_columns<SPLIT>(xs, ldx, ncol, nz, M, tfix, F.y + c0 * nz);
 }
 
 void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st) {
@@ -1222,52 +1193,11 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
   __syncthreads();
   gemm_panel<double, DEPTH>(a.pnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  // backward fixes: row 0 = fbb . Y, then row N-1 = fbt . Y with row 0 replaced
-  const int part = threadIdx.x & 15;
-  for (int cb = 0; cb < pd.ncol; cb += kNT / 16) {
-    const int c = cb + (threadIdx.x >> 4);
-    double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
-    if (c < pd.ncol) {
-      for (int j = part; j < nz; j += 16) {
-        const double2 v = xs[c * ldx + j];
-        br += a.pfix[j] * v.x;
-        bi += a.pfix[j] * v.y;
-        tr += a.pfix[nz + j] * v.x;
-        ti += a.pfix[nz + j] * v.y;
-      }
-    }
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-      br += __shfl_xor(br, o, 16);
-      bi += __shfl_xor(bi, o, 16);
-      tr += __shfl_xor(tr, o, 16);
-      ti += __shfl_xor(ti, o, 16);
-    }
-    if (part == 0 && c < pd.ncol) {
-      const double fbt0 = a.fix[3 * nz];
-      const double2 y0 = cadd<double>(xs[c * ldx], xs[c * ldx + M + 1]);
-      tfix[0][c] = make_double2(br, bi);
-      tfix[1][c] = make_double2(tr + fbt0 * (br - y0.x), ti + fbt0 * (bi - y0.y));
-    }
-  }
+  backward_fixes<true>(xs, ldx, pd.ncol, nz, M, a.pfix, a.pfix + nz, a.fix[3 * nz], tfix);
   __syncthreads();
   // p to HBM; the panel's (a, b) are p's even/odd halves except at the wall
   // pair, which is replaced by ((p_0 + p_N-1)/2, (p_0 - p_N-1)/2)
-  for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
-    const int c = idx / nz, k = idx - c * nz;
-    double2 v;
-    if (k == 0) {
-      v = tfix[0][c];
-    } else if (k == nz - 1) {
-      v = tfix[1][c];
-    } else if (k <= M) {
-      v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
-    } else {
-      const int i = nz - 1 - k;
-      v = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
-    }
-    a.p[c0 * nz + idx] = v;
-  }
+  store_columns<true>(xs, ldx, ncol, nz, M, tfix, a.p + c0 * nz);
   if ((int)threadIdx.x < pd.ncol) {
     const int c = threadIdx.x;
     const double2 t0 = tfix[0][c], tn = tfix[1][c];
