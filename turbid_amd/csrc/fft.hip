@@ -260,39 +260,45 @@ __global__ __launch_bounds__(NT) void k_ypf2(const C2<T>* __restrict__ tmp, Spec
   if (gate && *gate == 0) return;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
-  __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(false)];
-  C2<T>* E = reinterpret_cast<C2<T>*>(raw);
+  // level-A -> level-B exchange through one real plane: real parts, then
+  // imaginary parts (half the LDS of a complex exchange, twice the occupancy)
+  __shared__ __align__(16) char raw[sizeof(T) * lds2_elems<T, N1, N2, NT>(false)];
+  T* E = reinterpret_cast<T*>(raw);
   const int z0 = blockIdx.x * S, k = blockIdx.y, d = blockIdx.z;
   const int ns = min(S, nplanes - z0);
   const int t = threadIdx.x;
-  if (t < S * N2) {
-    const int n2 = t % N2, s = t / N2;
-    if (s < ns) {
-      const C2<T>* src = tmp + (((long long)d * nplanes + z0 + s) * mx + k) * N + n2;
-      C2<T> v[N1];
+  const int n2 = t % N2, sa = t / N2;
+  const bool act_a = t < S * N2 && sa < ns;
+  const int sb = t % S, k1 = t / S;
+  const bool act_b = t < S * N1 && sb < ns;
+  C2<T> v[N1];
+  if (act_a) {
+    const C2<T>* src = tmp + (((long long)d * nplanes + z0 + sa) * mx + k) * N + n2;
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[N2 * n1];
-      level_a_store<T, N1, N2, -1>(v, E, s, n2, tw);
-    }
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[N2 * n1];
+    level_a_regs<T, N1, N2, -1>(v, n2, tw);
+    level_a_put<T, N1, N2>(v, E, sa, n2, 0);
   }
   __syncthreads();
-  if (t < S * N1) {
-    const int s = t % S, k1 = t / S;
-    if (s < ns) {
-      C2<T> w[N2];
-      level_b_load<T, N1, N2, -1>(w, E, s, k1);
-      const int half = nyc / 2;
-      SpecT* out = spec + (long long)d * sstride + (long long)k * nyc * scol + zoff + z0 + s;
+  C2<T> w[N2];
+  if (act_b) level_b_get<T, N1, N2>(w, E, sb, k1, 0);
+  __syncthreads();
+  if (act_a) level_a_put<T, N1, N2>(v, E, sa, n2, 1);
+  __syncthreads();
+  if (act_b) {
+    level_b_get<T, N1, N2>(w, E, sb, k1, 1);
+    level_b_dft<T, N2, -1>(w);
+    const int half = nyc / 2;
+    SpecT* out = spec + (long long)d * sstride + (long long)k * nyc * scol + zoff + z0 + sb;
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) {
-        const int y = k1 + N1 * k2;
-        const int iy = y < half ? y : (y >= N - half ? y - N + nyc : -1);
-        if (iy >= 0) {
-          SpecT o;
-          o.x = w[k2].x * scale;
-          o.y = w[k2].y * scale;
-          out[(long long)iy * scol] = o;
-        }
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int y = k1 + N1 * k2;
+      const int iy = y < half ? y : (y >= N - half ? y - N + nyc : -1);
+      if (iy >= 0) {
+        SpecT o;
+        o.x = w[k2].x * scale;
+        o.y = w[k2].y * scale;
+        out[(long long)iy * scol] = o;
       }
     }
   }
@@ -305,41 +311,46 @@ __global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, lon
                                              int nyc, int nplanes, int zoff) {
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
-  __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(false)];
-  C2<T>* E = reinterpret_cast<C2<T>*>(raw);
+  // exchange through one real plane, real then imaginary parts (see k_ypf2)
+  __shared__ __align__(16) char raw[sizeof(T) * lds2_elems<T, N1, N2, NT>(false)];
+  T* E = reinterpret_cast<T*>(raw);
   const int z0 = blockIdx.x * S, k = blockIdx.y, d = blockIdx.z;
   const int ns = min(S, nplanes - z0);
   const int t = threadIdx.x;
-  if (t < S * N2) {
-    const int s = t % S, n2 = t / S;
-    if (s < ns) {
-      const int half = nyc / 2;
-      const SpecT* in = spec + (long long)d * sstride + (long long)k * nyc * scol + zoff + z0 + s;
-      C2<T> v[N1];
+  const int sa = t % S, n2 = t / S;
+  const bool act_a = t < S * N2 && sa < ns;
+  const int k1 = t % N1, sb = t / N1;
+  const bool act_b = t < S * N1 && sb < ns;
+  C2<T> v[N1];
+  if (act_a) {
+    const int half = nyc / 2;
+    const SpecT* in = spec + (long long)d * sstride + (long long)k * nyc * scol + zoff + z0 + sa;
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) {
-        const int y = N2 * n1 + n2;
-        const int iy = y < half ? y : (y >= N - half ? y - N + nyc : -1);
-        if (iy >= 0) {
-          const SpecT u = in[(long long)iy * scol];
-          v[n1] = mk<T>((T)u.x, (T)u.y);
-        } else {
-          v[n1] = mk<T>((T)0, (T)0);
-        }
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const int y = N2 * n1 + n2;
+      const int iy = y < half ? y : (y >= N - half ? y - N + nyc : -1);
+      if (iy >= 0) {
+        const SpecT u = in[(long long)iy * scol];
+        v[n1] = mk<T>((T)u.x, (T)u.y);
+      } else {
+        v[n1] = mk<T>((T)0, (T)0);
       }
-      level_a_store<T, N1, N2, +1>(v, E, s, n2, tw);
     }
+    level_a_regs<T, N1, N2, +1>(v, n2, tw);
+    level_a_put<T, N1, N2>(v, E, sa, n2, 0);
   }
   __syncthreads();
-  if (t < S * N1) {
-    const int k1 = t % N1, s = t / N1;
-    if (s < ns) {
-      C2<T> w[N2];
-      level_b_load<T, N1, N2, +1>(w, E, s, k1);
-      C2<T>* out = tmp + (((long long)d * nplanes + z0 + s) * mx + k) * N + k1;
+  C2<T> w[N2];
+  if (act_b) level_b_get<T, N1, N2>(w, E, sb, k1, 0);
+  __syncthreads();
+  if (act_a) level_a_put<T, N1, N2>(v, E, sa, n2, 1);
+  __syncthreads();
+  if (act_b) {
+    level_b_get<T, N1, N2>(w, E, sb, k1, 1);
+    level_b_dft<T, N2, +1>(w);
+    C2<T>* out = tmp + (((long long)d * nplanes + z0 + sb) * mx + k) * N + k1;
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) out[N1 * k2] = w[k2];
-    }
+    for (int k2 = 0; k2 < N2; ++k2) out[N1 * k2] = w[k2];
   }
 }
 

@@ -158,4 +158,36 @@ __device__ __forceinline__ void level_b_load(C2<T>* w, const C2<T>* E, int s, in
   RegDft<T, N2, SIGN>::run(w);
 }
 
+// The same exchange through half the LDS: one real plane E[s][k1][n2]
+// carries first the real parts, then (after a barrier on both sides) the
+// imaginary parts.  level_a_regs leaves the twiddled level-A outputs in v;
+// level_a_put writes part 0 (real) or 1 (imaginary); level_b_get reads it
+// into w; level_b_dft finishes level B.
+template <typename T, int N1, int N2, int SIGN>
+__device__ __forceinline__ void level_a_regs(C2<T>* v, int n2, const C2<T>* __restrict__ tw) {
+  RegDft<T, N1, SIGN>::run(v);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1)
+    if (n2 != 0) v[k1] = cmul<T>(v[k1], twiddle<T, SIGN>(tw, n2 * k1));
+}
+template <typename T, int N1, int N2>
+__device__ __forceinline__ void level_a_put(const C2<T>* v, T* E, int s, int n2, int part) {
+  T* e = E + s * xseq<N1, N2>() + n2;
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) e[k1 * xrow<N1, N2>()] = part == 0 ? v[k1].x : v[k1].y;
+}
+template <typename T, int N1, int N2>
+__device__ __forceinline__ void level_b_get(C2<T>* w, const T* E, int s, int k1, int part) {
+  const T* e = E + s * xseq<N1, N2>() + k1 * xrow<N1, N2>();
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) {
+    if (part == 0) w[n2].x = e[n2];
+    else w[n2].y = e[n2];
+  }
+}
+template <typename T, int N2, int SIGN>
+__device__ __forceinline__ void level_b_dft(C2<T>* w) {
+  RegDft<T, N2, SIGN>::run(w);
+}
+
 }  // namespace tbd
