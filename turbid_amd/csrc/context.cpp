@@ -93,6 +93,16 @@ void Context::rhs_solve(RHArgs& base, int n, const int* k, const int* comp, cons
                         double2* const* h, const void* const* adv, double2* const* y, double sigma) {
   base.nz = d.nz; base.P = (int)d.P; base.nyc = d.nyc; base.kx = kx_; base.ky = ky_;
   base.d2p = d2p_; base.ldp = ldp_;
+  // with fields of both kinds, the even/odd-form launch runs on a stream of
+  // its own beside the full-form one (joined before returning)
+  int kinds = 0;
+  for (int i = 0; i < n; ++i) kinds |= hsplit_[k[i]] ? 2 : 1;
+  const bool fork = overlap_ && !prof.on && kinds == 3;
+  const hipStream_t main = st_;
+  if (fork) {
+    TBD_HIP(hipEventRecord(evf3_, main));
+    TBD_HIP(hipStreamWaitEvent(st3_, evf3_, 0));
+  }
   for (int pass = 0; pass < 2; ++pass) {
     RHArgs a = base;
     a.split = pass;
@@ -111,8 +121,13 @@ void Context::rhs_solve(RHArgs& base, int n, const int* k, const int* comp, cons
       }
     }
     if (nf == 0) continue;
+    const hipStream_t sx = (fork && pass == 1) ? st3_ : main;
     launch(pass ? KC_RHELMS : KC_RHELM, nf * rhs_helmholtz_flops(d.nz, d.P, pass == 1),
-           [&] { rhs_helmholtz(a, nf, st_); });
+           [&] { rhs_helmholtz(a, nf, sx); });
+  }
+  if (fork) {
+    TBD_HIP(hipEventRecord(evj3_, st3_));
+    TBD_HIP(hipStreamWaitEvent(main, evj3_, 0));
   }
 }
 
@@ -209,6 +224,9 @@ Context::Context(const tbd_config& c) : cfg(c) {
   TBD_HIP(hipStreamCreate(&st2_));
   TBD_HIP(hipEventCreateWithFlags(&evf_, hipEventDisableTiming));
   TBD_HIP(hipEventCreateWithFlags(&evj_, hipEventDisableTiming));
+  TBD_HIP(hipStreamCreate(&st3_));
+  TBD_HIP(hipEventCreateWithFlags(&evf3_, hipEventDisableTiming));
+  TBD_HIP(hipEventCreateWithFlags(&evj3_, hipEventDisableTiming));
   if (const char* e = std::getenv("TBD_NO_OVERLAP")) overlap_ = std::string(e) != "1";
   else overlap_ = true;
   fft_.create(d);
@@ -375,12 +393,16 @@ Context::Context(const tbd_config& c) : cfg(c) {
 Context::~Context() {
   if (st_) (void)hipStreamSynchronize(st_);
   if (st2_) (void)hipStreamSynchronize(st2_);
+  if (st3_) (void)hipStreamSynchronize(st3_);
   drop_graphs();
   for (void* p : allocs_) (void)hipFree(p);
   fft_.destroy();
   if (evf_) (void)hipEventDestroy(evf_);
   if (evj_) (void)hipEventDestroy(evj_);
+  if (evf3_) (void)hipEventDestroy(evf3_);
+  if (evj3_) (void)hipEventDestroy(evj3_);
   if (st2_) (void)hipStreamDestroy(st2_);
+  if (st3_) (void)hipStreamDestroy(st3_);
   if (st_) (void)hipStreamDestroy(st_);
 }
 

@@ -157,6 +157,9 @@ class Context {
   bool overlap_ = false;
   hipStream_t st2_ = nullptr;
   hipEvent_t evf_ = nullptr, evj_ = nullptr;
+  // the carrier's even/odd-form solve (w) beside its full-form solves (u, v)
+  hipStream_t st3_ = nullptr;
+  hipEvent_t evf3_ = nullptr, evj3_ = nullptr;
   void *sgrad_ = nullptr, *sprod_ = nullptr, *sgphys_ = nullptr, *saphys_ = nullptr, *stmp_ = nullptr;
   void stages(double time, int alternation);
   void pchain(double q, double cf, const double2* s, double2* p, double2* y, int slot, bool count);
