@@ -137,6 +137,13 @@ int tbd_zcontract(int nz, long long ncol, int fp32, const double* m, const doubl
  * fix, 1 pressure (src/phase_carrier_flat.cc:861-994). */
 int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x, double* y);
 
+/* Swirling strength of the context's current velocity
+ * (FlatChannelPostprocessor::Execute, src/postprocessor_channel_flat.cc:
+ * 203-236): Gradient of u, v, w, DftBackward, and per physical point
+ * max |Im lambda| over the eigenvalues of the 3x3 velocity-gradient tensor.
+ * out: nx*ny*nz doubles, x fastest (the real cube phase_00_vortex.dat holds). */
+int tbd_swirling_strength(tbd_ctx* ctx, double* out);
+
 /* Phase-binned statistics on the device (reference log and postprocessing:
  * solver_channel_flat.cc:372-414 writes the mean-mode wall gradients each
  * step; the MATLAB companions average profiles by phase).  accumulate adds

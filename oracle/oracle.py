@@ -147,6 +147,48 @@ def eig(a):
     return lam, v
 
 
+def wavenumbers(cfg):
+    """kx[mx], ky[nyc] with the reference's +ky sign at iy = nyc/2
+    (src/stepper_space.cc:181-213)."""
+    _, nyc, mx = dims(cfg["nx"], cfg["ny"], cfg["nz"])
+    kx = 2 * np.pi / cfg["lx"] * np.arange(mx)
+    ky1 = 2 * np.pi / cfg["ly"]
+    ky = np.array([ky1 * iy if iy <= nyc // 2 else -ky1 * (nyc - iy) for iy in range(nyc)])
+    return kx, ky
+
+
+def velocity_gradient(uvw, cfg):
+    """Physical G[i, j] = d u_i / d x_j, shape (3, 3, nz, ny, nx), from the
+    spectral velocity (3, mx, nyc, nz): SpaceStepper::Gradient (i kx, i ky,
+    D1 along z; src/stepper_space.cc:373-389) then DftBackward
+    (postprocessor_channel_flat.cc:203-208)."""
+    kx, ky = wavenumbers(cfg)
+    _, d1, _ = chebyshev(cfg["nz"], cfg.get("mapping", 0), cfg.get("mapping_coefficient", 0.0))
+    d1 = np.asarray(d1)
+    g = np.empty((9,) + uvw.shape[1:], dtype=np.complex128)
+    for i in range(3):
+        u = uvw[i]
+        g[3 * i + 0] = 1j * kx[:, None, None] * u
+        g[3 * i + 1] = 1j * ky[None, :, None] * u
+        g[3 * i + 2] = u @ d1.T
+    phys = dft_backward(g, cfg["nx"], cfg["ny"])
+    return phys.reshape(3, 3, *phys.shape[1:])
+
+
+def swirling_strength_tensors(G):
+    """max |Im lambda| over the eigenvalues of each 3x3 tensor G[..., 3, 3]
+    (arma::eig_gen, i.e. LAPACK geev, then max(unique(abs(imag(.)))),
+    postprocessor_channel_flat.cc:229-232)."""
+    lam = np.linalg.eigvals(G)
+    return np.abs(lam.imag).max(axis=-1)
+
+
+def swirling_strength(uvw, cfg):
+    """Swirling strength on the physical grid, (nz, ny, nx)."""
+    G = velocity_gradient(uvw, cfg)
+    return swirling_strength_tensors(np.moveaxis(G, (0, 1), (-2, -1)))
+
+
 class Oracle:
     """One restated solver instance (carrier + optional scalar phase)."""
 

@@ -278,6 +278,13 @@ int tbd_helmholtz(tbd_ctx* ctx, int which, int stage, int mode, const double* x,
   });
 }
 
+int tbd_swirling_strength(tbd_ctx* ctx, double* out) {
+  return guard([&] {
+    TBD_CHECK(ctx && out, "tbd_swirling_strength > bad argument");
+    ctx->c->swirling(out);
+  });
+}
+
 int tbd_stats_accumulate(tbd_ctx* ctx, int bin) {
   return guard([&] {
     TBD_CHECK(ctx, "tbd_stats_accumulate > null context");

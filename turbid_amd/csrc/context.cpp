@@ -1015,6 +1015,26 @@ void Context::project() {
   sync();
 }
 
+void Context::swirling(double* out) {
+  const long long S = d.S, R = d.R;
+  if (!pgrad_) {
+    pgrad_ = (double2*)alloc(sizeof(double2) * 9 * S);
+    pphys_ = (double*)alloc(sizeof(double) * 10 * R);
+  }
+  // Gradient(uvw, dudxyz / dvdxyz / dwdxyz) (postprocessor_channel_flat.cc:203-205)
+  ZArgs a;
+  a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<double>(a);
+  a.x = uvw_; a.xs = S; a.y = pgrad_; a.ys = S; a.kx = kx_; a.ky = ky_;
+  zgemm<double, double2, double2>(a, 3, st_);
+  // DftBackward of the nine components (:206-208), one at a time: the
+  // transform scratch holds one FP64 component in either precision
+  for (int k = 0; k < 9; ++k)
+    fft_backward<double, double2>(fft_, pgrad_ + k * S, S, pphys_ + k * R, R, 1, 0, d.nz, (double2*)tmp_, st_);
+  swirling_strength(pphys_, R, pphys_ + 9 * R, st_);
+  TBD_HIP(hipMemcpyAsync(out, pphys_ + 9 * R, sizeof(double) * R, hipMemcpyDeviceToHost, st_));
+  sync();
+}
+
 void Context::helmholtz_host(int which, int stage, int mode, const double* x, double* y) {
   TBD_CHECK(which >= 0 && which < (cfg.nphase > 1 ? 5 : 4), "Context::helmholtz > bad factorisation index");
   const size_t S16 = sizeof(double2) * d.S;

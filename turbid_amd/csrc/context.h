@@ -66,6 +66,10 @@ class Context {
   void features(double* out50);
   void sync();
   void helmholtz_host(int which, int stage, int mode, const double* x, double* y);
+  // swirling strength of the current velocity on the physical grid (R
+  // values, x fastest): Gradient of u, v, w, DftBackward, per-point
+  // eigenvalues (FlatChannelPostprocessor::Execute), all in FP64
+  void swirling(double* out);
 
   tbd_config cfg;
   Dims d;
@@ -145,6 +149,8 @@ class Context {
   double* pnorm_[5] = {nullptr};
   double* plam_[5] = {nullptr};
   double* pfix_[5] = {nullptr};
+  double2* pgrad_ = nullptr;   // postprocessor work: 9 spectral gradients (FP64), allocated on first use
+  double* pphys_ = nullptr;    // 9 physical gradients + 1 output
   double* vinvp_[5] = {nullptr};   // V~^-1 with permuted columns (rhs_helmholtz)
   bool fuse_rhs_ = false;          // rhs + boundary rows + Helmholtz fused (TBD_RHS_UNFUSED=1 disables)
   bool fuse_p_ = false;            // divergence + pressure solve + projection fused (same switch)

@@ -14,6 +14,12 @@ void WriteCubeCplx(std::ostream& out, const cx* data, long long rows, long long 
   if (!out) throw std::runtime_error("RealCplxCube::SaveCplx(ofstream,ftype_t) > Write Failed!");
 }
 
+void WriteCubeReal(std::ostream& out, const double* data, long long rows, long long cols, long long slices) {
+  out << "ARMA_CUB_BIN_FN008" << '\n' << rows << ' ' << cols << ' ' << slices << '\n';
+  out.write(reinterpret_cast<const char*>(data), (std::streamsize)(sizeof(double) * rows * cols * slices));
+  if (!out) throw std::runtime_error("RealCplxCube::SaveReal(ofstream,ftype_t) > Write Failed!");
+}
+
 void ReadCubeCplx(std::istream& in, std::vector<cx>& data, long long rows, long long cols, long long slices) {
   std::string header;
   std::getline(in, header);

@@ -18,6 +18,9 @@ namespace tbdhost {
 typedef std::complex<double> cx;
 
 void WriteCubeCplx(std::ostream& out, const cx* data, long long rows, long long cols, long long slices);
+// real cube (header "ARMA_CUB_BIN_FN008"; RealCplxCube::SaveReal,
+// src/cube_real_cplx.cc:797-809)
+void WriteCubeReal(std::ostream& out, const double* data, long long rows, long long cols, long long slices);
 // Reads one cube; throws if the header or the expected shape does not match.
 void ReadCubeCplx(std::istream& in, std::vector<cx>& data, long long rows, long long cols, long long slices);
 

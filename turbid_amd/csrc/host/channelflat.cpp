@@ -2,7 +2,10 @@
 // MI355X.  Usage: channelflat [configuration.ini] [logcomputation]
 //   --parse-only <ini>   print the configuration the path reads, then exit
 //                        (no GPU needed)
+//   --postprocess <ini>  swirling strength of the saved indices (the
+//                        reference's FlatChannelPostprocessor)
 #include <cstring>
+#include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <stdexcept>
@@ -62,9 +65,48 @@ static int CopyState(const char* inipath, const char* from, const char* to) {
   return 0;
 }
 
+// --postprocess <ini>: FlatChannelPostprocessor::Execute
+// (postprocessor_channel_flat.cc:183-296): for every index in
+// [PostProcessor] work_index_range, load <index>/phase_00_variable.dat and
+// write the swirling strength of its velocity to <index>/phase_00_vortex.dat
+// (an Armadillo real cube Nx x Ny x Nz).
+static int Postprocess(const char* inipath) {
+  IniFile ini;
+  ini.Load(inipath);
+  const int nphase = ini.GetValueInt("System", "total_phase_number");
+  tbd_config c = LoadConfig(ini, nphase, 0);
+  c.nphase = 1;   // the postprocessor reads the carrier only
+  const std::vector<int> range = ini.GetVectorInt("PostProcessor", "work_index_range");
+  if (range.size() != 2) throw std::runtime_error("FlatChannelPostprocessor > work_index_range needs two values");
+  const int nyc = c.ny / 3 * 2, mx = (c.nx / 3 * 2) / 2 + 1;
+  tbd_ctx* ctx = nullptr;
+  Check(tbd_create(&c, &ctx));
+  try {
+    std::vector<cx> uvw, p;
+    std::vector<double> out((size_t)c.nx * c.ny * c.nz);
+    for (int index = range[0]; index <= range[1]; ++index) {
+      const std::string path = "./" + std::to_string((long long)index) + "/";
+      LoadCarrierVariable(path + "phase_00_variable.dat", uvw, p, c.nz, nyc, mx);
+      Check(tbd_upload_state(ctx, reinterpret_cast<const double*>(uvw.data()),
+                             reinterpret_cast<const double*>(p.data()), nullptr));
+      Check(tbd_swirling_strength(ctx, out.data()));
+      std::ofstream f(path + "phase_00_vortex.dat", std::ios::out | std::ios::binary);
+      if (!f) throw std::runtime_error("FlatChannelPostprocessor::Execute > Failed to Open " + path);
+      WriteCubeReal(f, out.data(), c.nx, c.ny, c.nz);
+      std::cout << "postprocess: " << path << "phase_00_vortex.dat" << std::endl;
+    }
+  } catch (...) {
+    tbd_destroy(ctx);
+    throw;
+  }
+  tbd_destroy(ctx);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
     if (argc > 2 && std::strcmp(argv[1], "--parse-only") == 0) return ParseOnly(argv[2]);
+    if (argc > 2 && std::strcmp(argv[1], "--postprocess") == 0) return Postprocess(argv[2]);
     if (argc > 4 && std::strcmp(argv[1], "--copy-state") == 0) return CopyState(argv[2], argv[3], argv[4]);
     FlatChannelSolver solver(argc, argv);
     solver.Configure();

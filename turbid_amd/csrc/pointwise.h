@@ -93,6 +93,15 @@ static constexpr int kPhaseStats = 8;
 void phase_stats(const double2* uvw, const double2* c, long long S, int nz, int nyc, long long P, double* acc,
                  hipStream_t st);
 
+// Swirling strength per point (FlatChannelPostprocessor::Execute,
+// postprocessor_channel_flat.cc:212-236): g holds the nine physical gradient
+// fields at g + (3i + j) R, G_ij = d u_i / d x_j; out = max |Im lambda| over
+// the eigenvalues of G, i.e. |Im| of the complex pair when there is one, 0
+// when all three are real.  The eigenvalues come from the characteristic
+// cubic in closed form (Cardano, the root of larger magnitude taken without
+// cancellation).
+void swirling_strength(const double* g, long long R, double* out, hipStream_t st);
+
 // small copies: dst[i] = Re(src[idx[i]]) for the log features
 void gather_real(const double2* src, const long long* idx, int n, double* dst, hipStream_t st);
 void fill_zero(void* p, size_t bytes, hipStream_t st);

@@ -110,6 +110,40 @@ void h_update(double2* h, const TA* adv, const double2* c, const HUpd& u, int nf
   TBD_HIP(hipGetLastError());
 }
 
+// lambda^3 - I1 lambda^2 + I2 lambda - I3 = 0; with lambda = t + I1/3:
+// t^3 + p t + q = 0, p = I2 - I1^2/3, q = -2 I1^3/27 + I1 I2/3 - I3.  When
+// D = (q/2)^2 + (p/3)^3 > 0 the roots are t1 = A + B and -(A + B)/2 +- i
+// (sqrt 3 / 2)(A - B) with A = cbrt(-q/2 + sqrt D), B = -p / (3 A).
+__device__ __forceinline__ double swirl3(const double a[9]) {
+  const double i1 = a[0] + a[4] + a[8];
+  const double i2 = a[0] * a[4] - a[1] * a[3] + a[0] * a[8] - a[2] * a[6] + a[4] * a[8] - a[5] * a[7];
+  const double i3 = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                    a[2] * (a[3] * a[7] - a[4] * a[6]);
+  const double p = i2 - i1 * i1 / 3.0;
+  const double q = -2.0 * i1 * i1 * i1 / 27.0 + i1 * i2 / 3.0 - i3;
+  const double disc = 0.25 * q * q + p * p * p / 27.0;
+  if (!(disc > 0.0)) return 0.0;
+  const double sq = sqrt(disc);
+  // the term without cancellation: sign(-q) (|q|/2 + sqrt D)
+  const double m = q > 0.0 ? -(0.5 * q + sq) : (-0.5 * q + sq);
+  const double A = cbrt(m);
+  const double B = A != 0.0 ? -p / (3.0 * A) : 0.0;
+  return 0.8660254037844386 * fabs(A - B);
+}
+
+__global__ void k_swirl(const double* __restrict__ g, long long R, double* __restrict__ out) {
+  GRID_STRIDE(i, R) {
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = g[k * R + i];
+    out[i] = swirl3(a);
+  }
+}
+void swirling_strength(const double* g, long long R, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_swirl, grid1d(R), dim3(kNT), 0, st, g, R, out);
+  TBD_HIP(hipGetLastError());
+}
+
 __global__ void k_set_pair(double* dst, double a, double b) {
   if (threadIdx.x == 0) {
     dst[0] = a;

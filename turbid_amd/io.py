@@ -87,7 +87,8 @@ def _fmt(v):
     return str(v)
 
 
-def write_ini(path, cfg, max_step=100, write_interval=100, start_step=0, max_cfl=0.5, print_log=0, write_log=1):
+def write_ini(path, cfg, max_step=100, write_interval=100, start_step=0, max_cfl=0.5, print_log=0, write_log=1,
+              work_index_range=None):
     """configuration.ini in the reference's layout from a workload dict."""
     scalar = "ErosionDeposition" if cfg.get("scalar_bc", 1) == 1 else "ZeroFlux"
     lines = [
@@ -145,6 +146,8 @@ def write_ini(path, cfg, max_step=100, write_interval=100, start_step=0, max_cfl
         "erosion_rate              = %s" % _fmt(float(cfg.get("erosion_rate", 0.0))),
         "critical_shear_stress     = %s" % _fmt(float(cfg.get("critical_shear_stress", 1.0))),
     ]
+    if work_index_range is not None:   # [PostProcessor] (configure.h:60-61)
+        lines += ["[PostProcessor]", "work_index_range          = %s" % _fmt(list(work_index_range))]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -156,3 +159,19 @@ def write_run(run_dir, cfg, uvw, p, c=None, **ini_kw):
     write_ini(ini, cfg, **ini_kw)
     write_state(run_dir, 0, cfg, uvw, p, c if cfg.get("nphase", 1) > 1 else None)
     return ini
+
+
+REAL_HEADER = b"ARMA_CUB_BIN_FN008"
+
+
+def read_real_cube(path):
+    """An Armadillo binary real cube (RealCplxCube::SaveReal): array of shape
+    (slices, cols, rows), i.e. (Nz, Ny, Nx) for a physical field."""
+    with open(path, "rb") as f:
+        h = f.readline().rstrip(b"\n")
+        if h != REAL_HEADER:
+            raise ValueError("unexpected cube header %r" % h)
+        rows, cols, slices = (int(x) for x in f.readline().split())
+        n = rows * cols * slices
+        a = np.frombuffer(f.read(8 * n), dtype=np.float64, count=n).copy()
+    return a.reshape(slices, cols, rows)

@@ -37,7 +37,7 @@ EXPORTS = [
     "tbd_download_physical_velocity", "tbd_substage", "tbd_step", "tbd_set_dt", "tbd_features",
     "tbd_sync", "tbd_profile_enable", "tbd_profile_read", "tbd_profile_reset", "tbd_get_operators",
     "tbd_get_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
-    "tbd_project", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
+    "tbd_project", "tbd_swirling_strength", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
     "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
     "tbd_bf16_gemm",
 ]
@@ -112,6 +112,7 @@ def lib():
         L.tbd_helmholtz.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.tbd_project.argtypes = [vp]
         L.tbd_stats_accumulate.argtypes = [vp, ctypes.c_int]
+        L.tbd_swirling_strength.argtypes = [vp, dp]
         L.tbd_stats_read.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.tbd_stats_reset.argtypes = [vp]
         fp = ctypes.POINTER(ctypes.c_float)
@@ -235,6 +236,12 @@ class Solver:
 
     def set_dt(self, dt):
         _check(lib().tbd_set_dt(self._h, dt))
+
+    def swirling_strength(self):
+        """Swirling strength of the current velocity, (nz, ny, nx) physical."""
+        out = np.zeros(self.nz * self.ny * self.nx)
+        _check(lib().tbd_swirling_strength(self._h, _p(out)))
+        return out.reshape(self.nz, self.ny, self.nx)
 
     def stats_accumulate(self, phase_bin):
         _check(lib().tbd_stats_accumulate(self._h, int(phase_bin)))
