@@ -150,6 +150,30 @@ def test_steps_mixed():
     assert rel_l2(cg, co) < 1e-5
 
 
+@pytest.mark.parametrize("precision,tol", [(0, 1e-9), (1, 1e-5)])
+def test_fields_after_50_steps(precision, tol):
+    # every field of the captured steps (scalar stream overlapped) against the
+    # FP64 oracle after 50 steps: u, v, w, p, c separately
+    cfg = small_config(precision=precision)
+    o, g = _pair(cfg)
+    st = RT.Stepper(cfg["dt"])
+    feat = np.zeros(50)
+    for _ in range(50):
+        st.update(feat[19])
+        for s in range(3):
+            o.substage(s, st.time, st.alternation)
+        g.step(st.time, st.alternation)
+        st.toggle()
+        feat = g.features()
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    uo, ug = uo.reshape(3, -1), ug.reshape(3, -1)
+    errs = {"u": rel_l2(ug[0], uo[0]), "v": rel_l2(ug[1], uo[1]), "w": rel_l2(ug[2], uo[2]),
+            "p": rel_l2(pg, po), "c": rel_l2(cg, co)}
+    print("precision %d, 50 steps, relative L2 per field: %s" % (precision, errs))
+    assert max(errs.values()) < tol, errs
+
+
 def test_steps_fused_path(monkeypatch):
     # the fused transform path (x transforms and products in one kernel) stays
     # interchangeable with the default one
