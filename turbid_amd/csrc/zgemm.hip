@@ -154,46 +154,52 @@ __device__ __forceinline__ void unit_mfma(const T* __restrict__ op, int lda, con
     i0 = mfma16(p.w, b3.y, i0);
     if (TWO) { r1 = mfma16(q.w, b3.x, r1); i1 = mfma16(q.w, b3.y, i1); }
   };
+  // operand of the first k step past the full blocks (k = KM + g; zero beyond
+  // K), fetched before the main loop so that its latency is hidden
+  const bool kt = sg.KM + g < sg.K;
+  T at0 = kt ? ob[(long long)(mt0 * 16 + cl) * lda + sg.KM + g] : T(0);
+  T at1 = (TWO && kt) ? ob[(long long)(mt0 * 16 + 16 + cl) * lda + sg.KM + g] : T(0);
   // two operand register sets used alternately: the loads of one set are in
   // flight while the other set's MFMAs issue (no register copies)
+  // Blocks in pairs with no branch between a prefetch and its use (the load
+  // of the next pair's first block clamped to the last block: a redundant
+  // in-bounds load at the end); the scheduling barriers keep each prefetch
+  // ahead of the block before its use.  An odd last block runs on its own.
+  const int kl = sg.KM - 16;
   if (sg.KM > 0) {
     V4<T> pa = ld4(A0), qa = pa, pb = pa, qb = pa;
     if (TWO) qa = ld4(A1);
-    for (int kb = 0; kb < sg.KM; kb += 32) {
-      const bool more = kb + 16 < sg.KM;
-      if (more) {
-        pb = ld4(A0 + kb + 16);
-        if (TWO) qb = ld4(A1 + kb + 16);
-      }
+    int kb = 0;
+    for (; kb + 32 <= sg.KM; kb += 32) {
+      pb = ld4(A0 + kb + 16);
+      if (TWO) qb = ld4(A1 + kb + 16);
+      __builtin_amdgcn_sched_barrier(0);
       block(pa, qa, kb);
-      if (!more) break;
-      if (kb + 32 < sg.KM) {
-        pa = ld4(A0 + kb + 32);
-        if (TWO) qa = ld4(A1 + kb + 32);
-      }
+      const int k2 = min(kb + 32, kl);
+      pa = ld4(A0 + k2);
+      if (TWO) qa = ld4(A1 + k2);
+      __builtin_amdgcn_sched_barrier(0);
       block(pb, qb, kb + 16);
     }
+    if (kb < sg.KM) block(pa, qa, kb);
+  }
+  // k beyond the last full block (one k for the CGL sizes), four k per MFMA
+  // step with the operand zero past K
+  for (int kk = sg.KM; kk < sg.K; kk += 4) {
+    const bool in = kk + g < sg.K;
+    if (kk > sg.KM) {
+      at0 = in ? ob[(long long)(mt0 * 16 + cl) * lda + kk + g] : T(0);
+      if (TWO) at1 = in ? ob[(long long)(mt0 * 16 + 16 + cl) * lda + kk + g] : T(0);
+    }
+    const C2<T> bt = in ? xc[kk + g] : mk<T>(0, 0);
+    r0 = mfma16(at0, bt.x, r0);
+    i0 = mfma16(at0, bt.y, i0);
+    if (TWO) { r1 = mfma16(at1, bt.x, r1); i1 = mfma16(at1, bt.y, i1); }
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     out[0][r] = mk<T>(r0[r], i0[r]);
     out[1][r] = mk<T>(r1[r], i1[r]);
-  }
-  // k beyond the last full block (one k for the CGL sizes)
-  for (int k = sg.KM; k < sg.K; ++k) {
-    const C2<T> xv = xc[k];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = mt0 * 16 + AccT<T>::row(g, r);
-      const T m0 = ob[(long long)row * lda + k];
-      out[0][r].x += m0 * xv.x;
-      out[0][r].y += m0 * xv.y;
-      if (TWO) {
-        const T m1 = ob[(long long)(row + 16) * lda + k];
-        out[1][r].x += m1 * xv.x;
-        out[1][r].y += m1 * xv.y;
-      }
-    }
   }
 }
 
@@ -224,7 +230,17 @@ __device__ __forceinline__ C2<T> tail_dot(const T* __restrict__ op, int lda, con
     const Seg& sg = pn.sg[si];
     const T* orow = op + (long long)(sg.opofs + row) * lda;
     const C2<T>* xc = xs + c * pn.ldx + sg.kofs;
-    for (int k = lig; k < sg.K; k += G) {
+    int k = lig;
+    // four operand loads in flight at a time
+    for (; k + 3 * G < sg.K; k += 4 * G) {
+      const T m0 = orow[k], m1 = orow[k + G], m2 = orow[k + 2 * G], m3 = orow[k + 3 * G];
+      const C2<T> x0 = xc[k], x1 = xc[k + G], x2 = xc[k + 2 * G], x3 = xc[k + 3 * G];
+      vr += m0 * x0.x; vi += m0 * x0.y;
+      vr += m1 * x1.x; vi += m1 * x1.y;
+      vr += m2 * x2.x; vi += m2 * x2.y;
+      vr += m3 * x3.x; vi += m3 * x3.y;
+    }
+    for (; k < sg.K; k += G) {
       const T m = orow[k];
       const C2<T> xv = xc[k];
       vr += m * xv.x;
@@ -256,7 +272,8 @@ __device__ __forceinline__ void unit_index(const Panel& pn, int u, int& si, int&
 template <typename T, int BUFFERED, class Emit>
 __device__ __forceinline__ void gemm_panel(const T* __restrict__ op, int lda, const Panel& pn, const C2<T>* xs,
                                            Emit emit) {
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform (a scalar register): the unit loops branch on the scalar unit
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4;
   const int cl = lane & 15;
@@ -300,6 +317,17 @@ __device__ __forceinline__ void gemm_panel(const T* __restrict__ op, int lda, co
   } else {
     C2<T> buf[BUFFERED][2][4];
     C2<T> tl[BUFFERED];
+    // the tail rows wait on operand loads: waves 4..7 compute them before
+    // their units, waves 0..3 after (wave w runs on SIMD w % 4), so that the
+    // MFMAs of the other wave of the workgroup on the SIMD hide the wait
+    auto tails = [&] {
+#pragma unroll
+      for (int t = 0; t < BUFFERED; ++t) {
+        const int idx = t * per + threadIdx.x / G;
+        if (t * per < ntail) tl[t] = tail_dot<T>(op, lda, pn, xs, idx, G, lig, idx < ntail);
+      }
+    };
+    if (wave & 4) tails();
 #pragma unroll
     for (int j = 0; j < BUFFERED; ++j) {
       const int u = wave + j * kWaves;
@@ -309,11 +337,7 @@ __device__ __forceinline__ void gemm_panel(const T* __restrict__ op, int lda, co
         run_unit(si, mt0, cg, buf[j]);
       }
     }
-#pragma unroll
-    for (int t = 0; t < BUFFERED; ++t) {
-      const int idx = t * per + threadIdx.x / G;
-      if (t * per < ntail) tl[t] = tail_dot<T>(op, lda, pn, xs, idx, G, lig, idx < ntail);
-    }
+    if (!(wave & 4)) tails();
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < BUFFERED; ++j) {
