@@ -456,8 +456,9 @@ __device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, 
   const int nz = a.nz;
   const long long off = col * nz + row;
   TY* y = (TY*)a.y;
-  const T kx = (T)a.kx[col / a.nyc];
-  const T ky = (T)a.ky[col % a.nyc];
+  const int ci = (int)col, ix = ci / a.nyc;
+  const T kx = (T)a.kx[ix];
+  const T ky = (T)a.ky[ci - ix * a.nyc];
   const C2<T> xv = in.x;
   switch (a.mode) {
     case Z_PLAIN:
@@ -517,6 +518,15 @@ __device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, 
 }
 
 static constexpr size_t kLdsLimit = 80 * 1024;   // two workgroups per CU
+static constexpr int kSplitMaxCol = 64;
+
+// (kx, ky) of spectral column col (columns past P, the padding of the last
+// panel, read column 0); 32-bit index arithmetic (P < 2^31)
+__device__ __forceinline__ double2 kcol(const double* kx, const double* ky, long long col, long long P, int nyc) {
+  const int ci = col < P ? (int)col : 0;
+  const int ix = ci / nyc;
+  return make_double2(kx[ix], ky[ci - ix * nyc]);
+}
 
 // Panel load, buffered contraction written over the panel, then the epilogue
 // as one coalesced pass (row fastest) that re-reads the input column where the
@@ -627,6 +637,8 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HArgs a, 
   double2* y = fi == 0 ? a.f[0].y : (fi == 1 ? a.f[1].y : a.f[2].y);
   const long long c0 = (long long)blockIdx.x * pn.ncol;
   const int ncol = (int)min((long long)pn.ncol, (long long)a.P - c0);
+  __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
+  for (int c = threadIdx.x; c < pn.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
   batched<8, double2>(pn.ncol * nz, [&](int idx) {
     return (idx / nz < ncol) ? x[c0 * nz + idx] : make_double2(0.0, 0.0);
   }, [&](int idx, double2 v) {
@@ -657,8 +669,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HArgs a, 
     double r = 1.0;
     if (row > 0 && row < nz - 1) {
       const long long col = c0 + c;
-      const long long cc = col < a.P ? col : 0;
-      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double kx = kv[c].x, ky = kv[c].y;
       const double k2 = (-(kx * kx)) + (-(ky * ky));
       r = 1.0 / (((k2 + lam[row]) + sigma));
       if (row == nullrow && col == 0) r = 0.0;
@@ -708,7 +719,8 @@ void helmholtz(const HArgs& a, int nfield, hipStream_t st) {
   const int depth = buffered_depth(pn);
   TBD_CHECK(depth <= kMaxU, "helmholtz > nz too large for the register-buffered contraction");
   const size_t lds = sizeof(double2) * pn.ncol * pn.ldx;
-  TBD_CHECK(lds <= 160 * 1024, "helmholtz > nz too large for one LDS panel");
+  TBD_CHECK(pn.ncol <= kSplitMaxCol, "helmholtz > too many columns per panel");
+  TBD_CHECK(lds + sizeof(double2) * kSplitMaxCol <= 160 * 1024, "helmholtz > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, pn.ncol), nfield);
   if (depth <= 1)
     hipLaunchKernelGGL(k_helmholtz<1>, grid, dim3(kNT), lds, st, a, pn);
@@ -725,7 +737,6 @@ void helmholtz(const HArgs& a, int nfield, hipStream_t st) {
 // products over (a, b) and the column is written as y_i = a_i + b_i,
 // y_{nz-1-i} = a_i - b_i.
 // ---------------------------------------------------------------------------
-static constexpr int kSplitMaxCol = 64;
 
 __host__ inline void make_split_panels(int nz, size_t lds_limit, Panel& p1, Panel& p2) {
   const int M = (nz - 1) / 2;
@@ -746,25 +757,47 @@ template <bool SPLIT>
 __device__ __forceinline__ void backward_fixes(const double2* xs, int ldx, int ncolp, int nz, int M,
                                                const double* wb, const double* wt, double fbt0,
                                                double2 (*tfix)[kSplitMaxCol]) {
-  const int part = threadIdx.x & 15;
-  for (int cb = 0; cb < ncolp; cb += kNT / 16) {
-    const int c = cb + (threadIdx.x >> 4);
+  // L lanes per column (every thread busy for ncolp <= kNT / 16), four
+  // weight pairs in flight per lane
+  int L = 64;
+  while (L > 16 && L * ncolp > kNT) L >>= 1;
+  const int part = threadIdx.x & (L - 1);
+  for (int cb = 0; cb < ncolp; cb += kNT / L) {
+    const int c = cb + threadIdx.x / L;
     double br = 0.0, bi = 0.0, tr = 0.0, ti = 0.0;
     if (c < ncolp) {
-      for (int j = part; j < nz; j += 16) {
-        const double2 v = xs[c * ldx + j];
+      const double2* xc = xs + c * ldx;
+      int j = part;
+      for (; j + 3 * L < nz; j += 4 * L) {
+        double b[4], t[4];
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          b[u] = wb[j + u * L];
+          t[u] = wt[j + u * L];
+          v[u] = xc[j + u * L];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          br += b[u] * v[u].x;
+          bi += b[u] * v[u].y;
+          tr += t[u] * v[u].x;
+          ti += t[u] * v[u].y;
+        }
+      }
+      for (; j < nz; j += L) {
+        const double2 v = xc[j];
         br += wb[j] * v.x;
         bi += wb[j] * v.y;
         tr += wt[j] * v.x;
         ti += wt[j] * v.y;
       }
     }
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-      br += __shfl_xor(br, o, 16);
-      bi += __shfl_xor(bi, o, 16);
-      tr += __shfl_xor(tr, o, 16);
-      ti += __shfl_xor(ti, o, 16);
+    for (int o = L >> 1; o > 0; o >>= 1) {
+      br += __shfl_xor(br, o, L);
+      bi += __shfl_xor(bi, o, L);
+      tr += __shfl_xor(tr, o, L);
+      ti += __shfl_xor(ti, o, L);
     }
     if (part == 0 && c < ncolp) {
       double2 y0 = xs[c * ldx];
@@ -814,6 +847,8 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
   const int nullrow = a.f[0].nullrow;
   const long long c0 = (long long)blockIdx.x * p1.ncol;
   const int ncol = (int)min((long long)p1.ncol, (long long)a.P - c0);
+  __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
+  for (int c = threadIdx.x; c < p1.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
   const double2* x = a.f[0].x + c0 * nz;
   struct Quad { double2 lo, hi, x0, xn; };
   const double2 zero = make_double2(0.0, 0.0);
@@ -855,8 +890,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
     double r = 1.0;
     if (row != 0 && row != M + 1) {
       const long long col = c0 + c;
-      const long long cc = col < a.P ? col : 0;
-      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double kx = kv[c].x, ky = kv[c].y;
       const double k2 = (-(kx * kx)) + (-(ky * ky));
       r = 1.0 / (((k2 + lam[row]) + sigma));
       if (row == nullrow && col == 0) r = 0.0;
@@ -880,7 +914,7 @@ void helmholtz_split(const HArgs& a, hipStream_t st) {
   const int depth = std::max(buffered_depth(p1), buffered_depth(p2));
   TBD_CHECK(depth <= kMaxU, "helmholtz_split > nz too large for the register-buffered contraction");
   const size_t lds = sizeof(double2) * p1.ncol * p1.ldx;
-  TBD_CHECK(lds + sizeof(double2) * 2 * kSplitMaxCol <= 160 * 1024, "helmholtz_split > nz too large for one LDS panel");
+  TBD_CHECK(lds + sizeof(double2) * 3 * kSplitMaxCol <= 160 * 1024, "helmholtz_split > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, p1.ncol), 1);
   if (depth <= 1)
     hipLaunchKernelGGL(k_helmholtz_split<1>, grid, dim3(kNT), lds, st, a, p1, p2);
@@ -924,6 +958,8 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   const double sigma = F.sigma;
   const long long c0 = (long long)blockIdx.x * pd.ncol;
   const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
+  __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
+  for (int c = threadIdx.x; c < pd.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
   const double2 zero = make_double2(0.0, 0.0);
   // boundary rows of every column of the panel
   if ((int)threadIdx.x < pd.ncol) {
@@ -1044,8 +1080,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
     }
     if (j > 0) {
       const long long col = c0 + c;
-      const long long cc = col < a.P ? col : 0;
-      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double kx = kv[c].x, ky = kv[c].y;
       const double k2x = -(kx * kx), k2y = -(ky * ky);
       const double2 sv = xs[c * ldx + j], dv = xs[c * ldx + M + 1 + j];
       auto rhs = [&](double2 v, double2 xv, double2 hv) {
@@ -1076,8 +1111,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
     double r = 1.0;
     if (row != 0 && row != brow) {
       const long long col = c0 + c;
-      const long long cc = col < a.P ? col : 0;
-      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double kx = kv[c].x, ky = kv[c].y;
       const double k2 = (-(kx * kx)) + (-(ky * ky));
       r = 1.0 / (((k2 + lam[row]) + sigma));
     }
@@ -1112,7 +1146,7 @@ void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st) {
   const int depth = std::max(std::max(buffered_depth(pd), buffered_depth(p1)), buffered_depth(p2));
   TBD_CHECK(depth <= kMaxU, "rhs_helmholtz > nz too large for the register-buffered contraction");
   const size_t lds = sizeof(double2) * pd.ncol * ldx;
-  TBD_CHECK(lds + sizeof(double2) * 4 * kSplitMaxCol <= 160 * 1024, "rhs_helmholtz > nz too large for one LDS panel");
+  TBD_CHECK(lds + sizeof(double2) * 5 * kSplitMaxCol <= 160 * 1024, "rhs_helmholtz > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, pd.ncol), nfield);
   if (a.split) {
     if (depth <= 1) hipLaunchKernelGGL((k_rhs_helmholtz<1, true>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
@@ -1138,6 +1172,8 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
   double2* xs = reinterpret_cast<double2*>(smem_raw);
   const long long c0 = (long long)blockIdx.x * pd.ncol;
   const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
+  __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
+  for (int c = threadIdx.x; c < pd.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
   const double2 zero = make_double2(0.0, 0.0);
   const double2* su = a.s;
   const double2* sv = a.s + S;
@@ -1184,8 +1220,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
     double2 lo = zero, hi = zero;
     if (j > 0) {
       const long long col = c0 + c;
-      const long long cc = col < a.P ? col : 0;
-      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double kx = kv[c].x, ky = kv[c].y;
       const double2 sv1 = xs[c * ldx + j], dv1 = xs[c * ldx + M + 1 + j];
       auto div = [&](double2 v, double2 u0, double2 v0) {
         const double2 t = cadd<double>(cadd<double>(v, cmuli<double>(kx, u0)), cmuli<double>(ky, v0));
@@ -1206,8 +1241,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
     double r = 1.0;
     if (row != 0 && row != M + 1) {
       const long long col = c0 + c;
-      const long long cc = col < a.P ? col : 0;
-      const double kx = a.kx[cc / a.nyc], ky = a.ky[cc % a.nyc];
+      const double kx = kv[c].x, ky = kv[c].y;
       const double k2 = (-(kx * kx)) + (-(ky * ky));
       r = 1.0 / (((k2 + a.plam[row]) + 0.0));
       if (row == a.nullrow && col == 0) r = 0.0;
@@ -1247,7 +1281,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
     const int c = idx / (M + 1), j = idx - c * (M + 1);
     if (c >= ncol) return;
     const long long col = c0 + c;
-    const double kx = a.kx[col / a.nyc], ky = a.ky[col % a.nyc];
+    const double kx = kv[c].x, ky = kv[c].y;
     const double2 s2 = xs[c * ldx + j], d2 = xs[c * ldx + M + 1 + j];
     auto put = [&](int row, double2 v, double2 pv, double2 u0, double2 v0, double2 w0) {
       const long long off = col * nz + row;
@@ -1289,7 +1323,7 @@ void pressure_chain(const PArgs& a, hipStream_t st) {
   const int depth = std::max(std::max(buffered_depth(pd), buffered_depth(p1)), buffered_depth(p2));
   TBD_CHECK(depth <= kMaxU, "pressure_chain > nz too large for the register-buffered contraction");
   const size_t lds = sizeof(double2) * pd.ncol * ldx;
-  TBD_CHECK(lds + sizeof(double2) * 2 * kSplitMaxCol <= 160 * 1024, "pressure_chain > nz too large for one LDS panel");
+  TBD_CHECK(lds + sizeof(double2) * 3 * kSplitMaxCol <= 160 * 1024, "pressure_chain > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, pd.ncol), 1);
   if (depth <= 1) hipLaunchKernelGGL(k_pressure_chain<1>, grid, dim3(kNT), lds, st, a, pd, p1, p2);
   else hipLaunchKernelGGL(k_pressure_chain<2>, grid, dim3(kNT), lds, st, a, pd, p1, p2);
