@@ -888,7 +888,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
   __syncthreads();
   gemm_panel<double, DEPTH>(a.f[0].vinv, a.lda, p1, xs, [&](int row, int c, double2 v) {
     double r = 1.0;
-    if (row != 0 && row != M + 1) {
+    if (row != 0 && row != M + 1 && row < nz) {   // row nz: zero row, no eigenvalue
       const long long col = c0 + c;
       const double kx = kv[c].x, ky = kv[c].y;
       const double k2 = (-(kx * kx)) + (-(ky * ky));
@@ -953,8 +953,15 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   const RHField& F = fi == 0 ? a.f[0] : (fi == 1 ? a.f[1] : a.f[2]);
   const double2* x = F.x;
   double2* h = F.h;
+  // eigenvalues and the two forward-fix rows staged in LDS after the panel
+  // (read per element in the epilogues)
+  double* lam = reinterpret_cast<double*>(xs + pd.ncol * ldx);
+  double* fixs = lam + nz + 1;
   const double* fix = F.fix;
-  const double* lam = F.lambda;
+  // (nz eigenvalues; the even/odd form's last panel index, nz, is a zero row of
+  // its V~^-1 that the V~ contraction does not read)
+  for (int i = threadIdx.x; i < nz + 1; i += kNT) lam[i] = i < nz ? F.lambda[i] : 0.0;
+  for (int i = threadIdx.x; i < 2 * nz; i += kNT) fixs[i] = fix[i];
   const double sigma = F.sigma;
   const long long c0 = (long long)blockIdx.x * pd.ncol;
   const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
@@ -1089,7 +1096,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
         return t;
       };
       auto fixed = [&](int i, double2 v) {
-        return cadd<double>(v, cadd<double>(cscale<double>(fix[i], bot), cscale<double>(fix[nz + i], top)));
+        return cadd<double>(v, cadd<double>(cscale<double>(fixs[i], bot), cscale<double>(fixs[nz + i], top)));
       };
       lo = fixed(j, rhs(cadd<double>(sv, dv), r.xl, r.hl));
       if (j < M) hi = fixed(nz - 1 - j, rhs(csub<double>(sv, dv), r.xh, r.hh));
@@ -1145,7 +1152,7 @@ void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st) {
   TBD_CHECK(pd.ncol <= kSplitMaxCol, "rhs_helmholtz > too many columns per panel");
   const int depth = std::max(std::max(buffered_depth(pd), buffered_depth(p1)), buffered_depth(p2));
   TBD_CHECK(depth <= kMaxU, "rhs_helmholtz > nz too large for the register-buffered contraction");
-  const size_t lds = sizeof(double2) * pd.ncol * ldx;
+  const size_t lds = sizeof(double2) * pd.ncol * ldx + sizeof(double) * (3 * a.nz + 1);
   TBD_CHECK(lds + sizeof(double2) * 5 * kSplitMaxCol <= 160 * 1024, "rhs_helmholtz > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, pd.ncol), nfield);
   if (a.split) {
@@ -1239,7 +1246,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
   __syncthreads();
   gemm_panel<double, DEPTH>(a.pinv, a.lda, p1, xs, [&](int row, int c, double2 v) {
     double r = 1.0;
-    if (row != 0 && row != M + 1) {
+    if (row != 0 && row != M + 1 && row < nz) {   // row nz: zero row, no eigenvalue
       const long long col = c0 + c;
       const double kx = kv[c].x, ky = kv[c].y;
       const double k2 = (-(kx * kx)) + (-(ky * ky));
