@@ -542,6 +542,15 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
   const int ncol = (int)min((long long)pn.ncol, a.P - c0);
   const TX* x = zx<TX>(a, f) + c0 * nz;
   const C2<T> zero = mk<T>((T)0, (T)0);
+  // Z_GRAD on the even/odd split: the two pointwise outputs (i kx x, i ky x)
+  // are written from the load pass, so the epilogue neither re-reads x nor
+  // writes them
+  const bool grad_early = a.mode == Z_GRAD && pn.parity != 0;
+  __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
+  if (grad_early) {
+    for (int c = threadIdx.x; c < pn.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
+    __syncthreads();
+  }
   if (pn.parity == 0) {
     batched<8, C2<T>>(pn.ncol * nz, [&](int idx) {
       const int c = idx / nz;
@@ -565,6 +574,18 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
       return r;
     }, [&](int idx, LoHi r) {
       const int c = idx / (M + 1), j = idx - c * (M + 1);
+      if (grad_early && c < ncol) {
+        TY* y = (TY*)a.y;
+        const T kx = (T)kv[c].x, ky = (T)kv[c].y;
+        const long long off = (c0 + c) * nz + j;
+        st<TY, T>(y + (3 * f + 0) * a.ys + off, cmuli<T>(kx, r.lo));
+        st<TY, T>(y + (3 * f + 1) * a.ys + off, cmuli<T>(ky, r.lo));
+        if (j < M) {
+          const long long offh = (c0 + c) * nz + nz - 1 - j;
+          st<TY, T>(y + (3 * f + 0) * a.ys + offh, cmuli<T>(kx, r.hi));
+          st<TY, T>(y + (3 * f + 1) * a.ys + offh, cmuli<T>(ky, r.hi));
+        }
+      }
       if (j < M) {
         xs[c * ldx + j] = mk<T>((T)0.5 * (r.lo.x + r.hi.x), (T)0.5 * (r.lo.y + r.hi.y));
         xs[c * ldx + M + 1 + j] = mk<T>((T)0.5 * (r.lo.x - r.hi.x), (T)0.5 * (r.lo.y - r.hi.y));
@@ -577,6 +598,18 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
   gemm_panel<T, DEPTH>((const T*)a.op, a.lda, pn, xs, [&](int row, int c, C2<T> v) { xs[c * ldx + row] = v; });
   __syncthreads();
   const int M = (nz - 1) / 2;
+  if (grad_early) {
+    TY* y = (TY*)a.y + (3 * f + 2) * a.ys + c0 * nz;
+    for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
+      const int c = idx / nz, row = idx - c * nz;
+      // y_i = s_i + d_i, y_{N-1-i} = -(s_i - d_i) (D1 centro-antisymmetric), i <= M
+      const bool top = row > M;
+      const int i = top ? nz - 1 - row : row;
+      const C2<T> sv = xs[c * ldx + i], dv = xs[c * ldx + M + 1 + i];
+      st<TY, T>(y + idx, top ? (pn.parity > 0 ? csub<T>(sv, dv) : csub<T>(dv, sv)) : cadd<T>(sv, dv));
+    }
+    return;
+  }
   batched<sizeof(T) == 4 ? 2 : 4, EpiIn<T>>(ncol * nz, [&](int idx) {
     return epi_load<T, TX>(a, f, c0 * nz + idx);
   }, [&](int idx, const EpiIn<T>& in) {
@@ -602,7 +635,8 @@ void zgemm(const ZArgs& a, int nfield, hipStream_t st) {
             "zgemm > operator layout does not match the column length");
   const Panel pn = make_panel(a.nz, sizeof(C2<T>), kLdsLimit, a.parity);
   const size_t lds = sizeof(C2<T>) * pn.ncol * pn.ldx;
-  TBD_CHECK(lds <= 160 * 1024, "zgemm > nz too large for one LDS panel");
+  TBD_CHECK(pn.ncol <= kSplitMaxCol, "zgemm > too many columns per panel");
+  TBD_CHECK(lds + sizeof(double2) * kSplitMaxCol <= 160 * 1024, "zgemm > nz too large for one LDS panel");
   const int depth = buffered_depth(pn);
   TBD_CHECK(depth <= kMaxU, "zgemm > nz too large for the register-buffered contraction");
   dim3 grid(ceil_div(a.P, pn.ncol), nfield);
