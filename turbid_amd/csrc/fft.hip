@@ -271,22 +271,23 @@ __global__ __launch_bounds__(NT) void k_ypf2(const C2<T>* __restrict__ tmp, Spec
   const bool act_a = t < S * N2 && sa < ns;
   const int sb = t % S, k1 = t / S;
   const bool act_b = t < S * N1 && sb < ns;
+  constexpr int SW = (N1 == 16 && N2 == 16) ? 1 : 0;
   C2<T> v[N1];
   if (act_a) {
     const C2<T>* src = tmp + (((long long)d * nplanes + z0 + sa) * mx + k) * N + n2;
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) v[n1] = src[N2 * n1];
     level_a_regs<T, N1, N2, -1>(v, n2, tw);
-    level_a_put<T, N1, N2>(v, E, sa, n2, 0);
+    level_a_put<T, N1, N2, SW>(v, E, sa, n2, 0);
   }
   __syncthreads();
   C2<T> w[N2];
-  if (act_b) level_b_get<T, N1, N2>(w, E, sb, k1, 0);
+  if (act_b) level_b_get<T, N1, N2, SW>(w, E, sb, k1, 0);
   __syncthreads();
-  if (act_a) level_a_put<T, N1, N2>(v, E, sa, n2, 1);
+  if (act_a) level_a_put<T, N1, N2, SW>(v, E, sa, n2, 1);
   __syncthreads();
   if (act_b) {
-    level_b_get<T, N1, N2>(w, E, sb, k1, 1);
+    level_b_get<T, N1, N2, SW>(w, E, sb, k1, 1);
     level_b_dft<T, N2, -1>(w);
     const int half = nyc / 2;
     SpecT* out = spec + (long long)d * sstride + (long long)k * nyc * scol + zoff + z0 + sb;
@@ -321,6 +322,7 @@ __global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, lon
   const bool act_a = t < S * N2 && sa < ns;
   const int k1 = t % N1, sb = t / N1;
   const bool act_b = t < S * N1 && sb < ns;
+  constexpr int SW = (N1 == 16 && N2 == 16) ? 2 : 0;
   C2<T> v[N1];
   if (act_a) {
     const int half = nyc / 2;
@@ -337,16 +339,16 @@ __global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, lon
       }
     }
     level_a_regs<T, N1, N2, +1>(v, n2, tw);
-    level_a_put<T, N1, N2>(v, E, sa, n2, 0);
+    level_a_put<T, N1, N2, SW>(v, E, sa, n2, 0);
   }
   __syncthreads();
   C2<T> w[N2];
-  if (act_b) level_b_get<T, N1, N2>(w, E, sb, k1, 0);
+  if (act_b) level_b_get<T, N1, N2, SW>(w, E, sb, k1, 0);
   __syncthreads();
-  if (act_a) level_a_put<T, N1, N2>(v, E, sa, n2, 1);
+  if (act_a) level_a_put<T, N1, N2, SW>(v, E, sa, n2, 1);
   __syncthreads();
   if (act_b) {
-    level_b_get<T, N1, N2>(w, E, sb, k1, 1);
+    level_b_get<T, N1, N2, SW>(w, E, sb, k1, 1);
     level_b_dft<T, N2, +1>(w);
     C2<T>* out = tmp + (((long long)d * nplanes + z0 + sb) * mx + k) * N + k1;
 #pragma unroll

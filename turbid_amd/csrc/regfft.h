@@ -170,19 +170,35 @@ __device__ __forceinline__ void level_a_regs(C2<T>* v, int n2, const C2<T>* __re
   for (int k1 = 1; k1 < N1; ++k1)
     if (n2 != 0) v[k1] = cmul<T>(v[k1], twiddle<T, SIGN>(tw, n2 * k1));
 }
-template <typename T, int N1, int N2>
-__device__ __forceinline__ void level_a_put(const C2<T>* v, T* E, int s, int n2, int part) {
-  T* e = E + s * xseq<N1, N2>() + n2;
-#pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) e[k1 * xrow<N1, N2>()] = part == 0 ? v[k1].x : v[k1].y;
+//
+// Slot of (s, k1, n2) in the real plane.  SW = 0: rows of N2+1, sequence
+// blocks of xseq (odd strides).  For N1 = N2 = 16 the plane can instead be
+// swizzled (256 words per sequence): the bank's low four bits are s XOR the
+// index that runs over 16 lanes of one pass, the two bits above them
+// (s + the index that runs over 4 lanes) mod 4, so that every 64-lane pass
+// touches 64 distinct banks.  SW = 1: n2 in the low bits, for the y forward
+// pass (put: lanes over n2 and 4 sequences; get: lanes over 16 sequences and
+// 4 k1).  SW = 2: k1 in the low bits, for the y backward pass (put: lanes
+// over 16 sequences and 4 n2; get: lanes over k1 and 4 sequences).
+template <int N1, int N2, int SW>
+__device__ __forceinline__ int xslot(int s, int k1, int n2) {
+  static_assert(SW == 0 || (N1 == 16 && N2 == 16), "swizzled plane is for 16 x 16");
+  if (SW == 1) return s * 256 + 64 * (k1 >> 2) + 16 * ((k1 + s) & 3) + ((s ^ n2) & 15);
+  if (SW == 2) return s * 256 + 64 * (n2 >> 2) + 16 * ((n2 + s) & 3) + ((s ^ k1) & 15);
+  return s * xseq<N1, N2>() + k1 * xrow<N1, N2>() + n2;
 }
-template <typename T, int N1, int N2>
+template <typename T, int N1, int N2, int SW = 0>
+__device__ __forceinline__ void level_a_put(const C2<T>* v, T* E, int s, int n2, int part) {
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) E[xslot<N1, N2, SW>(s, k1, n2)] = part == 0 ? v[k1].x : v[k1].y;
+}
+template <typename T, int N1, int N2, int SW = 0>
 __device__ __forceinline__ void level_b_get(C2<T>* w, const T* E, int s, int k1, int part) {
-  const T* e = E + s * xseq<N1, N2>() + k1 * xrow<N1, N2>();
 #pragma unroll
   for (int n2 = 0; n2 < N2; ++n2) {
-    if (part == 0) w[n2].x = e[n2];
-    else w[n2].y = e[n2];
+    const T u = E[xslot<N1, N2, SW>(s, k1, n2)];
+    if (part == 0) w[n2].x = u;
+    else w[n2].y = u;
   }
 }
 template <typename T, int N2, int SIGN>
