@@ -123,6 +123,19 @@ int tbd_get_operators(tbd_ctx* ctx, double* gridz, double* d1, double* d2);
 /* which: 0 u1, 1 u2, 2 u3, 3 p, 4 scalar; V~, V~^-1 nz*nz column-major,
  * lambda[nz], fix[4*nz] (cmdz2_decomposition.cc:47-139). */
 int tbd_get_factorisation(tbd_ctx* ctx, int which, double* V, double* Vinv, double* lambda, double* fix);
+/* Operators supplied by the caller instead (the host side of the reference
+ * builds them in ChebyshevSpaceStepper::Configure, stepper_space_chebyshev.cc:
+ * 101-228, and CmDz2DecompositionCplx::Construct, cmdz2_decomposition.cc:
+ * 19-297).  tbd_set_operators replaces D1 and D2 (nz*nz column-major) and
+ * rebuilds every factorisation from them with the configured boundary rows;
+ * tbd_set_factorisation replaces one factorisation (layout as in
+ * tbd_get_factorisation; the eigenvectors may come in any order and scaling,
+ * the even/odd form is rebuilt from them).  Both wait for the device,
+ * replace the device copies and drop the captured steps; call them between
+ * steps, from the thread that owns the context. */
+int tbd_set_operators(tbd_ctx* ctx, const double* d1, const double* d2);
+int tbd_set_factorisation(tbd_ctx* ctx, int which, const double* V, const double* Vinv, const double* lambda,
+                          const double* fix);
 
 /* Stand-alone device kernels on host buffers (parity tests of single pieces).
  * DftForward / DftBackward of nv components (src/cube_real_cplx.cc:525-555);

@@ -178,6 +178,21 @@ int tbd_get_factorisation(tbd_ctx* ctx, int which, double* V, double* Vinv, doub
   });
 }
 
+int tbd_set_operators(tbd_ctx* ctx, const double* d1, const double* d2) {
+  return guard([&] {
+    TBD_CHECK(ctx && d1 && d2, "tbd_set_operators > null argument");
+    ctx->c->set_operators(d1, d2);
+  });
+}
+
+int tbd_set_factorisation(tbd_ctx* ctx, int which, const double* V, const double* Vinv, const double* lambda,
+                          const double* fix) {
+  return guard([&] {
+    TBD_CHECK(ctx && V && Vinv && lambda && fix, "tbd_set_factorisation > null argument");
+    ctx->c->set_factorisation(which, V, Vinv, lambda, fix);
+  });
+}
+
 int tbd_dft_forward(int nx, int ny, int nz, int nv, int fp32, const double* phys, double* spec) {
   return guard([&] {
     TBD_CHECK(phys && spec && nv >= 1, "tbd_dft_forward > bad argument");

@@ -239,78 +239,16 @@ Context::Context(const tbd_config& c) : cfg(c) {
 
   // operators
   ch = make_chebyshev(cfg.nz, cfg.mapping, cfg.mapping_coefficient);
-  fac[0] = make_factorisation(ch, cfg.vt_u1);
-  fac[1] = std::equal(cfg.vt_u1, cfg.vt_u1 + 4, cfg.vt_u2) ? fac[0] : make_factorisation(ch, cfg.vt_u2);
-  for (int i = 0; i < 5; ++i) fac[1].vt[i] = cfg.vt_u2[i];
-  fac[2] = make_factorisation(ch, cfg.vt_u3);
-  fac[3] = make_factorisation(ch, cfg.vt_p);
-  if (cfg.nphase > 1) {   // Robin rows 1e7*(Re Sc ws, 1) on both walls (phase_scalar_flat.cc:194-200)
-    double bc[5] = {cfg.reynolds * cfg.schmidt * cfg.settling_velocity, 1,
-                    cfg.reynolds * cfg.schmidt * cfg.settling_velocity, 1, 0};
-    for (int i = 0; i < 4; ++i) bc[i] = bc[i] * 10E+6;
-    fac[4] = make_factorisation(ch, bc);
-  }
+  build_factorisations();
   opentop_ = std::fabs(cfg.vt_u1[2] - 0) <= 10E-12 && std::fabs(cfg.vt_u1[3] - 1) <= 10E-12;   // boundary.cc:59-71
 
+  install_operators();
   const int nz = d.nz;
-  lda_ = (nz + 3) / 4 * 4;
   auto up = [&](const std::vector<double>& h) {
     double* p = (double*)alloc(sizeof(double) * h.size());
     TBD_HIP(hipMemcpy(p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
     return p;
   };
-  d1_ = up(rowmajor(ch.d1, nz, lda_));
-  d2_ = up(rowmajor(ch.d2, nz, lda_));
-  {
-    std::vector<double> r = rowmajor(ch.d1, nz, lda_);
-    std::vector<float> f(r.begin(), r.end());
-    d1f_ = (float*)alloc(sizeof(float) * f.size());
-    TBD_HIP(hipMemcpy(d1f_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
-  }
-  // D1 is centro-antisymmetric and D2 centro-symmetric on the unmapped CGL
-  // grid: apply them through their even/odd split (half the MFMA work)
-  {
-    const std::vector<double> r1 = rowmajor(ch.d1, nz, lda_), r2 = rowmajor(ch.d2, nz, lda_);
-    auto centro = [&](const std::vector<double>& A, double sgn) {
-      double mx = 0, dv = 0;
-      for (int i = 0; i < nz; ++i)
-        for (int j = 0; j < nz; ++j) {
-          mx = std::max(mx, std::fabs(A[(size_t)i * lda_ + j]));
-          dv = std::max(dv, std::fabs(A[(size_t)i * lda_ + j] - sgn * A[(size_t)(nz - 1 - i) * lda_ + nz - 1 - j]));
-        }
-      return dv <= 1e-12 * mx;
-    };
-    const char* np = std::getenv("TBD_NO_PARITY");
-    parity_ = nz % 2 == 1 && !(np && np[0] == '1') && centro(r1, -1.0) && centro(r2, 1.0);
-    if (parity_) {
-      d1p_ = up(pack_parity<double>(r1.data(), nz, lda_, &ldp_));
-      d2p_ = up(pack_parity<double>(r2.data(), nz, lda_, &ldp_));
-      const std::vector<float> f = pack_parity<float>(r1.data(), nz, lda_, &ldp_);
-      d1pf_ = (float*)alloc(sizeof(float) * f.size());
-      TBD_HIP(hipMemcpy(d1pf_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
-    }
-  }
-  const char* nps = std::getenv("TBD_NO_PARITY");
-  for (int k = 0; k < (cfg.nphase > 1 ? 5 : 4); ++k) {
-    vinv_[k] = up(rowmajor(fac[k].vinv, nz, lda_));
-    vnorm_[k] = up(rowmajor(fac[k].vnorm, nz, lda_));
-    lam_[k] = up(fac[k].lambda);
-    fix_[k] = up(fac[k].fix);
-    vinvp_[k] = nz % 2 == 1 ? up(permute_columns(rowmajor(fac[k].vinv, nz, lda_), nz, lda_)) : nullptr;
-    hsplit_[k] = fac[k].pM > 0 && !(nps && nps[0] == '1');
-    if (hsplit_[k]) {
-      pld_[k] = fac[k].pld;
-      pinv_[k] = up(fac[k].pinv);
-      pnorm_[k] = up(fac[k].pnorm);
-      plam_[k] = up(fac[k].plam);
-      pfix_[k] = up(fac[k].pfix);
-    }
-  }
-  {
-    const char* e = std::getenv("TBD_RHS_UNFUSED");
-    fuse_rhs_ = parity_ && nz % 2 == 1 && nz >= 7 && !(e && e[0] == '1');
-    fuse_p_ = fuse_rhs_ && hsplit_[3];
-  }
   // Fourier multipliers with the reference's sign at iy = nyc/2 (stepper_space.cc:181-213)
   std::vector<double> kx(d.mx), ky(d.nyc);
   const double kx1 = 2 * kPi / cfg.lx, ky1 = 2 * kPi / cfg.ly;
@@ -396,6 +334,7 @@ Context::~Context() {
   if (st3_) (void)hipStreamSynchronize(st3_);
   drop_graphs();
   for (void* p : allocs_) (void)hipFree(p);
+  for (void* p : opallocs_) (void)hipFree(p);
   fft_.destroy();
   if (evf_) (void)hipEventDestroy(evf_);
   if (evj_) (void)hipEventDestroy(evj_);
@@ -412,6 +351,121 @@ void Context::set_dt(double dt) {
   TBD_CHECK(dt > 0, "Context::set_dt > dt must be positive");
   if (dt != dt_) drop_graphs();
   dt_ = dt;
+}
+
+// Boundary-condition sets of the factorisations: u, v, w, p and (two phases)
+// the scalar's Robin rows 1e7*(Re Sc ws, 1) on both walls (phase_scalar_flat.cc:194-200)
+void Context::build_factorisations() {
+  fac[0] = make_factorisation(ch, cfg.vt_u1);
+  fac[1] = std::equal(cfg.vt_u1, cfg.vt_u1 + 4, cfg.vt_u2) ? fac[0] : make_factorisation(ch, cfg.vt_u2);
+  for (int i = 0; i < 5; ++i) fac[1].vt[i] = cfg.vt_u2[i];
+  fac[2] = make_factorisation(ch, cfg.vt_u3);
+  fac[3] = make_factorisation(ch, cfg.vt_p);
+  if (cfg.nphase > 1) {
+    double bc[5] = {cfg.reynolds * cfg.schmidt * cfg.settling_velocity, 1,
+                    cfg.reynolds * cfg.schmidt * cfg.settling_velocity, 1, 0};
+    for (int i = 0; i < 4; ++i) bc[i] = bc[i] * 10E+6;
+    fac[4] = make_factorisation(ch, bc);
+  }
+}
+
+void* Context::alloc_op(size_t bytes) {
+  void* p = nullptr;
+  TBD_HIP(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+  opallocs_.push_back(p);
+  return p;
+}
+
+// Device copies of D1, D2 and the factorisations with their derived forms
+// (even/odd splits, permuted V~^-1).  A repeated call (the operators were
+// replaced) waits for the device and frees the previous copies first.
+void Context::install_operators() {
+  if (!opallocs_.empty()) {
+    TBD_HIP(hipDeviceSynchronize());
+    for (void* q : opallocs_) (void)hipFree(q);
+    opallocs_.clear();
+  }
+  const int nz = d.nz;
+  lda_ = (nz + 3) / 4 * 4;
+  auto up = [&](const std::vector<double>& h) {
+    double* p = (double*)alloc_op(sizeof(double) * h.size());
+    TBD_HIP(hipMemcpy(p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    return p;
+  };
+  d1_ = up(rowmajor(ch.d1, nz, lda_));
+  d2_ = up(rowmajor(ch.d2, nz, lda_));
+  {
+    std::vector<double> r = rowmajor(ch.d1, nz, lda_);
+    std::vector<float> f(r.begin(), r.end());
+    d1f_ = (float*)alloc_op(sizeof(float) * f.size());
+    TBD_HIP(hipMemcpy(d1f_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  }
+  // D1 is centro-antisymmetric and D2 centro-symmetric on the unmapped CGL
+  // grid: apply them through their even/odd split (half the MFMA work)
+  {
+    const std::vector<double> r1 = rowmajor(ch.d1, nz, lda_), r2 = rowmajor(ch.d2, nz, lda_);
+    auto centro = [&](const std::vector<double>& A, double sgn) {
+      double mx = 0, dv = 0;
+      for (int i = 0; i < nz; ++i)
+        for (int j = 0; j < nz; ++j) {
+          mx = std::max(mx, std::fabs(A[(size_t)i * lda_ + j]));
+          dv = std::max(dv, std::fabs(A[(size_t)i * lda_ + j] - sgn * A[(size_t)(nz - 1 - i) * lda_ + nz - 1 - j]));
+        }
+      return dv <= 1e-12 * mx;
+    };
+    const char* np = std::getenv("TBD_NO_PARITY");
+    d1p_ = d2p_ = nullptr;
+    d1pf_ = nullptr;
+    parity_ = nz % 2 == 1 && !(np && np[0] == '1') && centro(r1, -1.0) && centro(r2, 1.0);
+    if (parity_) {
+      d1p_ = up(pack_parity<double>(r1.data(), nz, lda_, &ldp_));
+      d2p_ = up(pack_parity<double>(r2.data(), nz, lda_, &ldp_));
+      const std::vector<float> f = pack_parity<float>(r1.data(), nz, lda_, &ldp_);
+      d1pf_ = (float*)alloc_op(sizeof(float) * f.size());
+      TBD_HIP(hipMemcpy(d1pf_, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+    }
+  }
+  const char* nps = std::getenv("TBD_NO_PARITY");
+  for (int k = 0; k < (cfg.nphase > 1 ? 5 : 4); ++k) {
+    vinv_[k] = up(rowmajor(fac[k].vinv, nz, lda_));
+    vnorm_[k] = up(rowmajor(fac[k].vnorm, nz, lda_));
+    lam_[k] = up(fac[k].lambda);
+    fix_[k] = up(fac[k].fix);
+    vinvp_[k] = nz % 2 == 1 ? up(permute_columns(rowmajor(fac[k].vinv, nz, lda_), nz, lda_)) : nullptr;
+    hsplit_[k] = fac[k].pM > 0 && !(nps && nps[0] == '1');
+    pinv_[k] = pnorm_[k] = plam_[k] = pfix_[k] = nullptr;
+    if (hsplit_[k]) {
+      pld_[k] = fac[k].pld;
+      pinv_[k] = up(fac[k].pinv);
+      pnorm_[k] = up(fac[k].pnorm);
+      plam_[k] = up(fac[k].plam);
+      pfix_[k] = up(fac[k].pfix);
+    }
+  }
+  {
+    const char* e = std::getenv("TBD_RHS_UNFUSED");
+    fuse_rhs_ = parity_ && nz % 2 == 1 && nz >= 7 && !(e && e[0] == '1');
+    fuse_p_ = fuse_rhs_ && hsplit_[3];
+  }
+}
+
+void Context::set_operators(const double* d1, const double* d2) {
+  const size_t n = (size_t)d.nz * d.nz;
+  for (size_t i = 0; i < n; ++i)
+    TBD_CHECK(std::isfinite(d1[i]) && std::isfinite(d2[i]), "tbd_set_operators > operators are not finite");
+  ch.d1.assign(d1, d1 + n);
+  ch.d2.assign(d2, d2 + n);
+  build_factorisations();
+  install_operators();
+  drop_graphs();
+}
+
+void Context::set_factorisation(int which, const double* V, const double* Vinv, const double* lambda,
+                                const double* fix) {
+  TBD_CHECK(which >= 0 && which < (cfg.nphase > 1 ? 5 : 4), "tbd_set_factorisation > bad index");
+  fac[which] = factorisation_from(d.nz, fac[which].vt, V, Vinv, lambda, fix);
+  install_operators();
+  drop_graphs();
 }
 
 void Context::drop_graphs() {

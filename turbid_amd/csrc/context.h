@@ -70,6 +70,12 @@ class Context {
   // values, x fastest): Gradient of u, v, w, DftBackward, per-point
   // eigenvalues (FlatChannelPostprocessor::Execute), all in FP64
   void swirling(double* out);
+  // Replace the wall-normal operators D1, D2 (column-major nz*nz; every
+  // factorisation is rebuilt from them with the configured boundary rows) or
+  // one factorisation (bordered V~, V~^-1, lambda, fix), then re-upload the
+  // device copies and drop the captured steps.
+  void set_operators(const double* d1, const double* d2);
+  void set_factorisation(int which, const double* V, const double* Vinv, const double* lambda, const double* fix);
 
   tbd_config cfg;
   Dims d;
@@ -119,6 +125,10 @@ class Context {
     xpb<T>(tmp, phys, pstride, nv, nplanes);
   }
   void* alloc(size_t bytes);
+  void build_factorisations();
+  void install_operators();
+  void* alloc_op(size_t bytes);        // operator copies (replaced by install_operators)
+  std::vector<void*> opallocs_;
 
   hipStream_t st_ = nullptr;
   FftPlan fft_;

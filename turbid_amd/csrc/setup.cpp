@@ -382,8 +382,10 @@ int real_eigen(int n, const double* a, std::vector<double>& lambda, std::vector<
 // the two half-size blocks inverted separately; nothing is built when the
 // operator is not centro-symmetric or an eigenvector is not clearly even or
 // odd (a near-degenerate pair of opposite parity).
+// optol: tolerance of the operator's centro-symmetry check, relative to its
+// largest entry (looser for an operator reassembled from supplied V, lambda).
 static void make_split(Factorisation& f, const std::vector<double>& fwd, const std::vector<double>& lam,
-                       const std::vector<double>& vec) {
+                       const std::vector<double>& vec, double optol = 1e-12) {
   const int N = f.nz;
   if (N % 2 == 0 || N < 7) return;
   const int m = N - 2, M = (N - 1) / 2;
@@ -393,7 +395,7 @@ static void make_split(Factorisation& f, const std::vector<double>& fwd, const s
       mx = std::max(mx, std::fabs(fwd[i + (size_t)m * j]));
       dv = std::max(dv, std::fabs(fwd[i + (size_t)m * j] - fwd[(m - 1 - i) + (size_t)m * (m - 1 - j)]));
     }
-  if (!(dv <= 1e-12 * mx)) return;
+  if (!(dv <= optol * mx)) return;
   std::vector<int> ev, od;
   for (int k = 0; k < m; ++k) {
     const double* v = &vec[(size_t)m * k];
@@ -533,6 +535,46 @@ Factorisation make_factorisation(const Chebyshev& ch, const double* vt) {
   }
   f.nullrow = (bmin < 10E-12 || bmin < 1e-9 * bmax) ? best : -1;
   make_split(f, fwd, lam, vec);
+  return f;
+}
+
+Factorisation factorisation_from(int nz, const double* vt, const double* V, const double* Vinv,
+                                 const double* lambda, const double* fix) {
+  const int N = nz, m = N - 2;
+  if (N < 4) throw Error("CmDz2DecompositionCplx::Construct > nz too small");
+  Factorisation f;
+  f.nz = N;
+  for (int i = 0; i < 5; ++i) f.vt[i] = vt[i];
+  f.vnorm.assign(V, V + (size_t)N * N);
+  f.vinv.assign(Vinv, Vinv + (size_t)N * N);
+  f.lambda.assign(lambda, lambda + N);
+  f.fix.assign(fix, fix + 4 * (size_t)N);
+  for (double v : f.vnorm)
+    if (!std::isfinite(v)) throw Error("CmDz2DecompositionCplx::Construct > V is not finite");
+  for (double v : f.vinv)
+    if (!std::isfinite(v)) throw Error("CmDz2DecompositionCplx::Construct > V^-1 is not finite");
+  double bmin = 1e300, bmax = 0.0;
+  int best = -1;
+  for (int i = 1; i < N - 1; ++i) {
+    bmax = std::max(bmax, std::fabs(f.lambda[i]));
+    if (std::fabs(f.lambda[i]) < bmin) { bmin = std::fabs(f.lambda[i]); best = i; }
+  }
+  f.nullrow = (bmin < 10E-12 || bmin < 1e-9 * bmax) ? best : -1;
+  // interior eigenvectors, eigenvalues and operator V lambda V^-1
+  std::vector<double> vec((size_t)m * m), lam(m), fwd((size_t)m * m, 0.0);
+  for (int j = 0; j < m; ++j) {
+    lam[j] = f.lambda[j + 1];
+    for (int i = 0; i < m; ++i) vec[i + (size_t)m * j] = f.vnorm[(i + 1) + (size_t)N * (j + 1)];
+  }
+  for (int j = 0; j < m; ++j)
+    for (int k = 0; k < m; ++k) {
+      const double w = lam[k] * f.vinv[(k + 1) + (size_t)N * (j + 1)];
+      if (w == 0.0) continue;
+      for (int i = 0; i < m; ++i) fwd[i + (size_t)m * j] += vec[i + (size_t)m * k] * w;
+    }
+  // the reassembled operator carries the rounding of V lambda V^-1; the
+  // parity test of every eigenvector is what admits the split
+  make_split(f, fwd, lam, vec, 1e-9);
   return f;
 }
 

@@ -56,4 +56,12 @@ struct Factorisation {
 // Builds the factorisation from D1, D2 and the boundary rows a*f + b*df/dz.
 Factorisation make_factorisation(const Chebyshev& ch, const double* vt);
 
+// A factorisation supplied by the caller (the arrays of CmDz2DecompositionCplx:
+// bordered V~, V~^-1 column-major nz*nz, lambda[nz], fix[4*nz]) for the
+// boundary rows vt.  The null row follows the same rule as
+// make_factorisation; the even/odd form is rebuilt from the interior blocks
+// (operator V lambda V^-1, eigenvectors V) when they allow one.
+Factorisation factorisation_from(int nz, const double* vt, const double* V, const double* Vinv,
+                                 const double* lambda, const double* fix);
+
 }  // namespace tbd

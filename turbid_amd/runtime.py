@@ -36,7 +36,7 @@ EXPORTS = [
     "tbd_create", "tbd_destroy", "tbd_last_error", "tbd_upload_state", "tbd_download_state",
     "tbd_download_physical_velocity", "tbd_substage", "tbd_step", "tbd_set_dt", "tbd_features",
     "tbd_sync", "tbd_profile_enable", "tbd_profile_read", "tbd_profile_reset", "tbd_get_operators",
-    "tbd_get_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
+    "tbd_get_factorisation", "tbd_set_operators", "tbd_set_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
     "tbd_project", "tbd_swirling_strength", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
     "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
     "tbd_bf16_gemm",
@@ -106,6 +106,8 @@ def lib():
         L.tbd_profile_reset.argtypes = [vp]
         L.tbd_get_operators.argtypes = [vp, dp, dp, dp]
         L.tbd_get_factorisation.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
+        L.tbd_set_operators.argtypes = [vp, dp, dp]
+        L.tbd_set_factorisation.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
         L.tbd_dft_forward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
         L.tbd_dft_backward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
         L.tbd_zcontract.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp, dp, dp]
@@ -300,6 +302,24 @@ class Solver:
         fix = np.zeros(4 * n)
         _check(lib().tbd_get_factorisation(self._h, which, _p(V), _p(Vi), _p(lam), _p(fix)))
         return V, Vi, lam, fix.reshape(4, n)
+
+    def set_operators(self, d1, d2):
+        """Replace D1, D2 (nz x nz) and rebuild every factorisation from them."""
+        n = self.nz
+        a = np.asfortranarray(d1, dtype=np.float64)
+        b = np.asfortranarray(d2, dtype=np.float64)
+        assert a.shape == (n, n) and b.shape == (n, n)
+        _check(lib().tbd_set_operators(self._h, _p(a), _p(b)))
+
+    def set_factorisation(self, which, V, Vi, lam, fix):
+        """Replace one factorisation (layout of factorisation(which))."""
+        n = self.nz
+        V = np.asfortranarray(V, dtype=np.float64)
+        Vi = np.asfortranarray(Vi, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+        fix = np.ascontiguousarray(fix, dtype=np.float64).reshape(-1)
+        assert V.shape == (n, n) and Vi.shape == (n, n) and lam.size == n and fix.size == 4 * n
+        _check(lib().tbd_set_factorisation(self._h, which, _p(V), _p(Vi), _p(lam), _p(fix)))
 
     def helmholtz(self, which, stage, mode, x):
         xx = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
