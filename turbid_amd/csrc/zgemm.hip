@@ -1294,9 +1294,40 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
   __syncthreads();
   backward_fixes<true>(xs, ldx, pd.ncol, nz, M, a.pfix, a.pfix + nz, a.fix[3 * nz], tfix);
   __syncthreads();
-  // p to HBM; the panel's (a, b) are p's even/odd halves except at the wall
-  // pair, which is replaced by ((p_0 + p_N-1)/2, (p_0 - p_N-1)/2)
-  store_columns<true>(xs, ldx, ncol, nz, M, tfix, a.p + c0 * nz);
+  // p to HBM, and u = su - cf i kx p, v = sv - cf i ky p while p is in the
+  // panel (natural row order; the wall rows of (i kx p, i ky p) saved).  The
+  // panel's (a, b) are p's even/odd halves except at the wall pair, which is
+  // then replaced by ((p_0 + p_N-1)/2, (p_0 - p_N-1)/2) for D1 (this pass
+  // reads neither slot).
+  struct UV2 { double2 u, v; };
+  batched<4, UV2>(ncol * nz, [&](int idx) {
+    const long long off = c0 * nz + idx;
+    UV2 r = {su[off], sv[off]};
+    return r;
+  }, [&](int idx, UV2 r) {
+    const int c = idx / nz, k = idx - c * nz;
+    double2 pv;
+    if (k == 0) {
+      pv = tfix[0][c];
+    } else if (k == nz - 1) {
+      pv = tfix[1][c];
+    } else if (k <= M) {
+      pv = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
+    } else {
+      const int i = nz - 1 - k;
+      pv = csub<double>(xs[c * ldx + i], xs[c * ldx + M + 1 + i]);
+    }
+    const long long off = c0 * nz + idx;
+    a.p[off] = pv;
+    const double2 gx = cmuli<double>(kv[c].x, pv), gy = cmuli<double>(kv[c].y, pv);
+    a.y[off] = csub<double>(r.u, cscale<double>(a.cf, gx));
+    a.y[S + off] = csub<double>(r.v, cscale<double>(a.cf, gy));
+    if (k == 0 || k == nz - 1) {
+      double2* wb = (k == 0) ? a.wall_bot : a.wall_top;
+      wb[c0 + c] = gx;
+      wb[a.P + c0 + c] = gy;
+    }
+  });
   if ((int)threadIdx.x < pd.ncol) {
     const int c = threadIdx.x;
     const double2 t0 = tfix[0][c], tn = tfix[1][c];
@@ -1306,47 +1337,41 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
   __syncthreads();
   gemm_panel<double, DEPTH>(a.d1p, a.ldp, pd, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  // projection per row pair
-  struct In { double2 pl, ph, ul, uh, vl, vh, wl, wh; };
-  batched<2, In>(pd.ncol * (M + 1), [&](int idx) {
+  // w = sw - cf D1 p per row pair (mean w := 0), wall rows of D1 p (and of
+  // (i kx, i ky) D1 p when open_top)
+  struct Wv { double2 wl, wh; };
+  batched<4, Wv>(pd.ncol * (M + 1), [&](int idx) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
-    In r = {zero, zero, zero, zero, zero, zero, zero, zero};
+    Wv r = {zero, zero};
     if (c < ncol) {
       const long long o = (c0 + c) * nz;
-      const long long lo = o + j, hi = o + nz - 1 - j;
-      r.pl = a.p[lo]; r.ul = su[lo]; r.vl = sv[lo]; r.wl = sw[lo];
-      if (j < M) { r.ph = a.p[hi]; r.uh = su[hi]; r.vh = sv[hi]; r.wh = sw[hi]; }
+      r.wl = sw[o + j];
+      if (j < M) r.wh = sw[o + nz - 1 - j];
     }
     return r;
-  }, [&](int idx, In r) {
+  }, [&](int idx, Wv r) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
     if (c >= ncol) return;
     const long long col = c0 + c;
     const double kx = kv[c].x, ky = kv[c].y;
     const double2 s2 = xs[c * ldx + j], d2 = xs[c * ldx + M + 1 + j];
-    auto put = [&](int row, double2 v, double2 pv, double2 u0, double2 v0, double2 w0) {
+    auto put = [&](int row, double2 v, double2 w0) {
       const long long off = col * nz + row;
-      const double2 gx = cmuli<double>(kx, pv), gy = cmuli<double>(ky, pv);
-      a.y[off] = csub<double>(u0, cscale<double>(a.cf, gx));
-      a.y[S + off] = csub<double>(v0, cscale<double>(a.cf, gy));
       double2 w = csub<double>(w0, cscale<double>(a.cf, v));
       if (col == 0) w = make_double2(0.0, 0.0);
       a.y[2 * S + off] = w;
       if (row == 0 || row == nz - 1) {
         double2* wb = (row == 0) ? a.wall_bot : a.wall_top;
-        const long long P = a.P;
-        wb[col] = gx;
-        wb[P + col] = gy;
-        wb[2 * P + col] = v;
+        wb[2 * a.P + col] = v;
         if (a.open_top) {
           double2* w2 = (row == 0) ? a.wall2_bot : a.wall2_top;
           w2[col] = cmuli<double>(kx, v);
-          w2[P + col] = cmuli<double>(ky, v);
+          w2[a.P + col] = cmuli<double>(ky, v);
         }
       }
     };
-    put(j, cadd<double>(s2, d2), r.pl, r.ul, r.vl, r.wl);
-    if (j < M) put(nz - 1 - j, csub<double>(d2, s2), r.ph, r.uh, r.vh, r.wh);
+    put(j, cadd<double>(s2, d2), r.wl);
+    if (j < M) put(nz - 1 - j, csub<double>(d2, s2), r.wh);
   });
 }
 
