@@ -1092,8 +1092,15 @@ void Context::swirling(double* out) {
 void Context::helmholtz_host(int which, int stage, int mode, const double* x, double* y) {
   TBD_CHECK(which >= 0 && which < (cfg.nphase > 1 ? 5 : 4), "Context::helmholtz > bad factorisation index");
   const size_t S16 = sizeof(double2) * d.S;
-  double2* dx = (double2*)alloc(S16);
-  double2* dy = (double2*)alloc(S16);
+  // buffers of this call only (hipFree waits for the device)
+  struct Tmp {
+    void* p = nullptr;
+    ~Tmp() { if (p) (void)hipFree(p); }
+  } bx, by;
+  TBD_HIP(hipMalloc(&bx.p, S16));
+  TBD_HIP(hipMalloc(&by.p, S16));
+  double2* dx = (double2*)bx.p;
+  double2* dy = (double2*)by.p;
   TBD_HIP(hipMemcpyAsync(dx, x, S16, hipMemcpyHostToDevice, st_));
   double sig = 0.0;
   if (which < 3) sig = -cfg.reynolds / dt_ / coef_[2][stage];
@@ -1102,7 +1109,6 @@ void Context::helmholtz_host(int which, int stage, int mode, const double* x, do
   solve(mode, 1, &which, &sig, &xx, &dy);
   TBD_HIP(hipMemcpyAsync(y, dy, S16, hipMemcpyDeviceToHost, st_));
   sync();
-  // temporary buffers are released with the context
 }
 
 }  // namespace tbd

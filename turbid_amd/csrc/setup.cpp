@@ -553,6 +553,10 @@ Factorisation factorisation_from(int nz, const double* vt, const double* V, cons
     if (!std::isfinite(v)) throw Error("CmDz2DecompositionCplx::Construct > V is not finite");
   for (double v : f.vinv)
     if (!std::isfinite(v)) throw Error("CmDz2DecompositionCplx::Construct > V^-1 is not finite");
+  for (double v : f.lambda)
+    if (!std::isfinite(v)) throw Error("CmDz2DecompositionCplx::Construct > lambda is not finite");
+  for (double v : f.fix)
+    if (!std::isfinite(v)) throw Error("CmDz2DecompositionCplx::Construct > fix vectors are not finite");
   double bmin = 1e300, bmax = 0.0;
   int best = -1;
   for (int i = 1; i < N - 1; ++i) {
