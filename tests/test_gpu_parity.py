@@ -92,8 +92,8 @@ def test_helmholtz_vs_oracle(which, mode, nz):
     # the Neumann pressure problem at Nz = 257 is the worst conditioned (its
     # eigenvalues run from the null mode to ~Nz^4): over 12 random right-hand
     # sides its error against the oracle has median 1.0e-11 and maximum
-    # 1.7e-11, the same with the contraction kernels of either build; every
-    # other case stays below 3e-12
+    # 1.7e-11 (set by the conditioning, not by the order in which the
+    # contractions sum); every other case stays below 3e-12
     tol = 5e-11 if (which == 3 and nz >= 257) else 1e-11
     assert rel_l2(got, ref) < tol
 
