@@ -191,7 +191,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
               "(the single-DNS shards and the ensemble members are independent per GPU); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)

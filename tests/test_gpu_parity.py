@@ -4,7 +4,7 @@ Tolerances (relative L2 unless stated):
   FP64 transforms / contractions / solves: 1e-12
   FP32 transforms / contractions: 2e-6
   FP64 RK3 steps (several steps, both nonlinear forms): 1e-9
-  mixed-precision RK3 steps: 1e-5 (fields), the north-star's bound
+  mixed-precision RK3 steps: 1e-5 (fields)
 """
 import numpy as np
 import pytest

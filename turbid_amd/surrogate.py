@@ -268,7 +268,7 @@ def init_params(n, grid, C, L, seed):
 
 
 # ---------------------------------------------------------------------------
-# statistics used to judge a start
+# statistics used to assess a start
 # ---------------------------------------------------------------------------
 def flow_stats(uvw, c, feat):
     """Wall shear of the mean mode (dU/dz, dV/dz at the bottom), turbulent
