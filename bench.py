@@ -1,7 +1,7 @@
 """RK3 steps/s of the pseudospectral carrier + sediment substage on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64|both]
-               [--workload turb256] [--no-cpu-baseline]
+               [--workload turb256] [--no-cpu-baseline] [--no-autotune]
 
 Workload (default): 256x256x257 oscillatory bottom boundary layer with fine
 sediment (Re_delta = 800, settling velocity 5e-4, Sc = 0.5), synthetic
@@ -11,6 +11,10 @@ inputs resident in HBM; the loop mirrors FlatChannelSolver::Execute
 (TimeStepper::Update with the previous step's CFL feature, three substages,
 alternation toggle).  The headline value is the mixed-precision rate; the
 plain FP64 rate is measured in the same run.
+
+Kernel variant: before the warmup the library's autotuner (tbd_autotune)
+times the step's kernel variants on a scratch copy of the context and keeps
+the fastest (reported under `autotune`; --no-autotune keeps the default).
 
 Timing: after the warmup, K steps are timed as the library runs them (one
 captured HIP graph per step; `value`, `ms_per_step`), then K further steps
@@ -68,12 +72,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run(cfg, state, steps, warmup):
-    """Warm up, time `steps` steps as the library runs them (one captured HIP
-    graph per step), then time the same number of steps again with HIP events
-    around every launch (eager substages) for the per-kernel figures."""
+def run(cfg, state, steps, warmup, autotune):
+    """Pick the kernel variant (the library's autotuner, on a scratch copy of
+    the context, outside the timed region), warm up, time `steps` steps as the
+    library runs them (one captured HIP graph per step), then time the same
+    number of steps again with HIP events around every launch (eager
+    substages) for the per-kernel figures."""
     s = RT.Solver(cfg)
     s.upload(*state)
+    tune = None
+    if autotune:
+        bits, ms = s.autotune(5)
+        tune = {"kept": {k: bool(bits & b) for k, b in RT.Solver.VARIANT_BITS.items()},
+                "ms_per_step": {str(i): (None if np.isnan(v) else round(float(v), 3)) for i, v in enumerate(ms)}}
     st = RT.Stepper(cfg["dt"])
     RT.run_steps(s, st, warmup)
     s.sync()
@@ -90,7 +101,7 @@ def run(cfg, state, steps, warmup):
     prof = {name: s.profile_read(name) for name in KERNELS}
     s.profile(False)
     s.close()
-    return (t1 - t0), (tp1 - tp0), feat, prof
+    return (t1 - t0), (tp1 - tp0), feat, prof, tune
 
 
 TRAFFIC = os.path.join(ROOT, "profiles", "turb256_hbm_traffic.json")
@@ -183,6 +194,7 @@ def main():
     ap.add_argument("--precision", default="both", choices=["mixed", "fp64", "both"])
     ap.add_argument("--workload", default="turb256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="run the default kernel variant")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (reference for HBM byte counters)")
@@ -208,8 +220,10 @@ def main():
         cfg["precision"] = 1 if prec == "mixed" else 0
         log("bench: %s warmup %d, timing %d steps (graph), then %d steps with per-launch events"
             % (prec, args.warmup, args.steps, args.steps))
-        t, tp, feat, prof = run(cfg, state, args.steps, args.warmup)
-        results[prec] = {"seconds": t, "steps_per_s": args.steps / t, "ms_per_step": 1e3 * t / args.steps,
+        t, tp, feat, prof, tune = run(cfg, state, args.steps, args.warmup, not args.no_autotune)
+        if tune:
+            log("bench: %s variant kept %s" % (prec, tune["kept"]))
+        results[prec] = {"tune": tune, "seconds": t, "steps_per_s": args.steps / t, "ms_per_step": 1e3 * t / args.steps,
                          "profiled_ms_per_step": 1e3 * tp / args.steps,
                          "roofline": roofline(prof), "prof": {k: list(v) for k, v in prof.items()},
                          "cfl": float(feat[19] * cfg["dt"]), "max_u": float(feat[0])}
@@ -247,6 +261,9 @@ def main():
                                   for prec, r in results.items()}
     line["profiled_ms_per_step"] = {prec: r["profiled_ms_per_step"] for prec, r in results.items()}
     line["class_rates"] = {prec: class_rates(r["prof"]) for prec, r in results.items()}
+    # kernel variant the autotuner kept (ms per step of every variant it tried,
+    # keyed by variant bits: 1 fused x transforms, 2 fused rhs + solve, 4 overlap)
+    line["autotune"] = {prec: r["tune"] for prec, r in results.items()}
     if rank == 0 and not args.no_cpu_baseline:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("bench: cpu baseline (oracle, %d threads)" % nthreads)

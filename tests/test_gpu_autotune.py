@@ -1,0 +1,81 @@
+"""Kernel variants of a step and their autotuner (tbd_autotune,
+tbd_get_variant, tbd_set_variant): every variant steps to the same result as
+the default one, within the precision's tolerance; tuning leaves the
+context's state alone and keeps the fastest variant it tried."""
+import math
+
+import numpy as np
+import pytest
+
+from turbid_amd import runtime as RT
+from turbid_amd import workloads as W
+from helpers import rel_l2, small_config
+
+pytestmark = pytest.mark.gpu
+
+DEFAULT = RT.Solver.VARIANT_BITS["rhs"] | RT.Solver.VARIANT_BITS["overlap"]
+
+
+def _solver(cfg, bits=None):
+    g = RT.Solver(cfg)
+    uvw, p, c = W.initial_state(cfg)
+    g.upload(uvw, p, c if cfg["nphase"] > 1 else None)
+    if bits is not None:
+        g.set_variant(bits)
+    return g
+
+
+def _steps(g, cfg, n):
+    RT.run_steps(g, RT.Stepper(cfg["dt"]), n)
+    return g.download()
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-9), (1, 1e-5)])
+def test_every_variant_steps_like_the_default(precision, tol):
+    cfg = small_config(precision=precision)
+    ref = _steps(_solver(cfg, DEFAULT), cfg, 6)
+    for bits in range(8):
+        g = _solver(cfg, bits)
+        assert g.variant() == bits
+        got = _steps(g, cfg, 6)
+        for a, b in zip(got, ref):
+            assert rel_l2(a, b) < tol, bits
+
+
+def test_autotune_keeps_state_and_the_fastest_variant():
+    cfg = small_config()
+    g = _solver(cfg)
+    RT.run_steps(g, RT.Stepper(cfg["dt"]), 2)
+    before = g.download()
+    bits, ms = g.autotune(3)
+    after = g.download()
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    tried = [i for i in range(8) if not math.isnan(ms[i])]
+    assert len(tried) == 8
+    assert all(ms[i] > 0 for i in tried)
+    assert bits == min(tried, key=lambda i: ms[i])
+    assert g.variant() == bits
+    # the tuned context keeps stepping like an untuned one
+    ref = _solver(cfg)
+    RT.run_steps(ref, RT.Stepper(cfg["dt"]), 2)
+    for a, b in zip(_steps(g, cfg, 3), _steps(ref, cfg, 3)):
+        assert rel_l2(a, b) < 1e-9
+
+
+def test_environment_pins_a_switch(monkeypatch):
+    monkeypatch.setenv("TBD_FUSED", "0")
+    cfg = small_config()
+    g = _solver(cfg)
+    g.set_variant(7)
+    assert g.variant() & RT.Solver.VARIANT_BITS["fused"] == 0
+    bits, ms = g.autotune(2)
+    assert bits & RT.Solver.VARIANT_BITS["fused"] == 0
+    assert all(math.isnan(ms[i]) for i in range(8) if i & 1)
+    assert all(not math.isnan(ms[i]) for i in range(8) if not i & 1)
+
+
+def test_set_variant_rejects_bad_bits():
+    g = _solver(small_config())
+    with pytest.raises(RT.TurbidError, match="out of range"):
+        g.set_variant(9)

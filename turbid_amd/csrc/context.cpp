@@ -3,6 +3,7 @@
 #include "context.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -227,10 +228,21 @@ Context::Context(const tbd_config& c) : cfg(c) {
   TBD_HIP(hipStreamCreate(&st3_));
   TBD_HIP(hipEventCreateWithFlags(&evf3_, hipEventDisableTiming));
   TBD_HIP(hipEventCreateWithFlags(&evj3_, hipEventDisableTiming));
-  if (const char* e = std::getenv("TBD_NO_OVERLAP")) overlap_ = std::string(e) != "1";
-  else overlap_ = true;
+  if (const char* e = std::getenv("TBD_NO_OVERLAP")) {
+    overlap_ = std::string(e) != "1";
+    pin_overlap_ = true;
+  } else {
+    overlap_ = true;
+  }
   fft_.create(d);
-  if (const char* e = std::getenv("TBD_FUSED")) fused_ = std::string(e) == "1";
+  if (const char* e = std::getenv("TBD_FUSED")) {
+    fused_ = std::string(e) == "1";
+    pin_fused_ = true;
+  }
+  if (const char* e = std::getenv("TBD_RHS_UNFUSED")) {
+    want_rhs_ = !(e[0] == '1');
+    pin_rhs_ = true;
+  }
   dt_ = olddt_ = cfg.dt;
   const double c1[3] = {0.0 / 1.0, -5.0 / 9.0, -153.0 / 128.0};    // stepper_time_rk3.cc:29-31
   const double c2[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
@@ -442,11 +454,92 @@ void Context::install_operators() {
       pfix_[k] = up(fac[k].pfix);
     }
   }
-  {
-    const char* e = std::getenv("TBD_RHS_UNFUSED");
-    fuse_rhs_ = parity_ && nz % 2 == 1 && nz >= 7 && !(e && e[0] == '1');
-    fuse_p_ = fuse_rhs_ && hsplit_[3];
+  fuse_rhs_ = parity_ && nz % 2 == 1 && nz >= 7 && want_rhs_;
+  fuse_p_ = fuse_rhs_ && hsplit_[3];
+}
+
+void Context::copy_operators_from(const Context& o) {
+  TBD_CHECK(o.d.nz == d.nz, "Context::copy_operators_from > different nz");
+  ch = o.ch;
+  for (int k = 0; k < 5; ++k) fac[k] = o.fac[k];
+  install_operators();
+  drop_graphs();
+}
+
+Context::Variant Context::variant() const {
+  Variant v;
+  v.fused = fused_;
+  v.rhs = want_rhs_;
+  v.overlap = overlap_;
+  return v;
+}
+
+void Context::set_variant(const Variant& v) {
+  sync();
+  if (!pin_fused_) fused_ = v.fused;
+  if (!pin_rhs_) want_rhs_ = v.rhs;
+  if (!pin_overlap_) overlap_ = v.overlap;
+  fuse_rhs_ = parity_ && d.nz % 2 == 1 && d.nz >= 7 && want_rhs_;
+  fuse_p_ = fuse_rhs_ && hsplit_[3];
+  drop_graphs();
+}
+
+int Context::autotune(int steps, double* ms8) {
+  TBD_CHECK(steps >= 1 && steps <= 10000, "tbd_autotune > steps out of range");
+  for (int i = 0; i < 8; ++i) ms8[i] = std::nan("");
+  // this context's state, uploaded to the scratch context before each variant
+  const size_t n2 = 2 * (size_t)d.S;
+  std::vector<double> u(3 * n2), p(n2), c(cfg.nphase > 1 ? n2 : 0);
+  download(u.data(), p.data(), c.empty() ? nullptr : c.data());
+  Context scratch(cfg);
+  scratch.pin_fused_ = scratch.pin_rhs_ = scratch.pin_overlap_ = false;   // the loop below decides
+  scratch.copy_operators_from(*this);
+  scratch.set_dt(dt_);
+  const bool rhs_ok = parity_ && d.nz % 2 == 1 && d.nz >= 7;
+  const Variant cur = variant();
+  int best = cur.bits();
+  double best_ms = 0.0;
+  std::vector<double> feat(50);
+  for (int bits = 0; bits < 8; ++bits) {
+    Variant v;
+    v.fused = (bits & 1) != 0;
+    v.rhs = (bits & 2) != 0;
+    v.overlap = (bits & 4) != 0;
+    if ((pin_fused_ && v.fused != cur.fused) || (pin_rhs_ && v.rhs != cur.rhs) ||
+        (pin_overlap_ && v.overlap != cur.overlap))
+      continue;
+    // the fused right-hand side needs the even/odd operators; without them
+    // both settings run the same kernels
+    if (!rhs_ok && v.rhs != cur.rhs) continue;
+    scratch.set_variant(v);
+    scratch.upload(u.data(), p.data(), c.empty() ? nullptr : c.data());
+    double time = 0.0;
+    int alt = 0;
+    auto one = [&] {
+      time += dt_;
+      scratch.step(time, alt);
+      alt ^= 1;
+      scratch.features(feat.data());
+    };
+    for (int i = 0; i < 3; ++i) one();
+    scratch.sync();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; ++i) one();
+    scratch.sync();
+    const auto t1 = std::chrono::steady_clock::now();
+    const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+    ms8[bits] = ms;
+    if (best_ms == 0.0 || ms < best_ms) {
+      best_ms = ms;
+      best = bits;
+    }
   }
+  Variant v;
+  v.fused = (best & 1) != 0;
+  v.rhs = (best & 2) != 0;
+  v.overlap = (best & 4) != 0;
+  set_variant(v);
+  return variant().bits();
 }
 
 void Context::set_operators(const double* d1, const double* d2) {

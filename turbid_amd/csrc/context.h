@@ -76,6 +76,28 @@ class Context {
   // device copies and drop the captured steps.
   void set_operators(const double* d1, const double* d2);
   void set_factorisation(int which, const double* V, const double* Vinv, const double* lambda, const double* fix);
+  void copy_operators_from(const Context& o);
+
+  // Kernel variants of a step, each tested against the oracle:
+  //   fused   x transforms fused around the nonlinear products
+  //   rhs     right-hand side + boundary rows + solve as one kernel, and the
+  //           divergence + pressure solve + projection chain (needs the
+  //           even/odd D1, D2; without them the switch has no effect)
+  //   overlap scalar stages on a second stream, the even/odd-form solve of w
+  //           beside those of u, v on a third
+  // A switch set in the environment at creation (TBD_FUSED, TBD_RHS_UNFUSED,
+  // TBD_NO_OVERLAP) is pinned: set_variant leaves it and autotune does not vary it.
+  struct Variant {
+    bool fused = false, rhs = true, overlap = true;
+    int bits() const { return (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0); }
+  };
+  Variant variant() const;
+  void set_variant(const Variant& v);
+  // Times every admissible variant on a scratch context (same configuration,
+  // operators and dt; this context's state uploaded to it before each) for
+  // `steps` captured steps after three warm-up steps, and keeps the fastest.
+  // ms[bits] = ms per step (NaN where not tried); returns the kept bits.
+  int autotune(int steps, double* ms8);
 
   tbd_config cfg;
   Dims d;
@@ -163,6 +185,8 @@ class Context {
   double* pphys_ = nullptr;    // 9 physical gradients + 1 output
   double* vinvp_[5] = {nullptr};   // V~^-1 with permuted columns (rhs_helmholtz)
   bool fuse_rhs_ = false;          // rhs + boundary rows + Helmholtz fused (TBD_RHS_UNFUSED=1 disables)
+  bool want_rhs_ = true;           // the variant switch (fuse_rhs_ also needs the even/odd operators)
+  bool pin_fused_ = false, pin_rhs_ = false, pin_overlap_ = false;
   bool fuse_p_ = false;            // divergence + pressure solve + projection fused (same switch)
   // The scalar's Correct of stage s reads the carrier's physical velocity of
   // stage s and writes only scalar state; without buoyancy the carrier never

@@ -41,11 +41,13 @@ def member_configs(n=16, seed=0, precision=1):
 
 
 class Ensemble:
-    def __init__(self, cfgs, member_overlap=False):
+    def __init__(self, cfgs, member_overlap=False, autotune=False):
         """member_overlap: let each member also run its scalar on a second
         stream.  Off by default: the members already overlap one another, and
         three streams per member over the process's few hardware queues
-        serialise them (1593 vs 990 member-steps/s for the 16 members)."""
+        serialise them (1593 vs 990 member-steps/s for the 16 members).
+        autotune: each member picks its own kernel variant (tbd_autotune,
+        run alone; the stream switch stays as member_overlap sets it)."""
         self.cfgs = cfgs
         self.solvers = []
         self.steppers = []
@@ -65,6 +67,7 @@ class Ensemble:
                 os.environ.pop("TBD_NO_OVERLAP", None)
             else:
                 os.environ["TBD_NO_OVERLAP"] = saved
+        self.variants = [s.autotune(3)[0] if autotune else s.variant() for s in self.solvers]
 
     def step(self, members=None):
         """One step of the given members (all by default), concurrently."""
@@ -105,9 +108,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true", help="each member picks its kernel variant")
     args = ap.parse_args(argv)
     cfgs = member_configs(args.members, args.seed, args.precision)
-    ens = Ensemble(cfgs)
+    ens = Ensemble(cfgs, autotune=args.autotune)
     for _ in range(args.warmup):
         ens.step()
     tc = ens.run_concurrent(args.steps)
@@ -118,7 +122,8 @@ def main(argv=None):
                precision="mixed" if args.precision == 1 else "fp64", finite=ens.finite(),
                grids=[[c["nx"], c["ny"], c["nz"]] for c in cfgs],
                re_delta=[round(c["re_delta"], 1) for c in cfgs],
-               settling=[float("%.3g" % c["settling_velocity"]) for c in cfgs])
+               settling=[float("%.3g" % c["settling_velocity"]) for c in cfgs],
+               variants=ens.variants)
     ens.close()
     print(json.dumps(out), flush=True)
 

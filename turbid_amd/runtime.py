@@ -36,7 +36,8 @@ EXPORTS = [
     "tbd_create", "tbd_destroy", "tbd_last_error", "tbd_upload_state", "tbd_download_state",
     "tbd_download_physical_velocity", "tbd_substage", "tbd_step", "tbd_set_dt", "tbd_features",
     "tbd_sync", "tbd_profile_enable", "tbd_profile_read", "tbd_profile_reset", "tbd_get_operators",
-    "tbd_get_factorisation", "tbd_set_operators", "tbd_set_factorisation", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
+    "tbd_get_factorisation", "tbd_set_operators", "tbd_set_factorisation", "tbd_autotune", "tbd_get_variant",
+    "tbd_set_variant", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
     "tbd_project", "tbd_swirling_strength", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
     "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
     "tbd_bf16_gemm",
@@ -108,6 +109,9 @@ def lib():
         L.tbd_get_factorisation.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
         L.tbd_set_operators.argtypes = [vp, dp, dp]
         L.tbd_set_factorisation.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
+        L.tbd_autotune.argtypes = [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_int)]
+        L.tbd_get_variant.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.tbd_set_variant.argtypes = [vp, ctypes.c_int]
         L.tbd_dft_forward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
         L.tbd_dft_backward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
         L.tbd_zcontract.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp, dp, dp]
@@ -320,6 +324,26 @@ class Solver:
         fix = np.ascontiguousarray(fix, dtype=np.float64).reshape(-1)
         assert V.shape == (n, n) and Vi.shape == (n, n) and lam.size == n and fix.size == 4 * n
         _check(lib().tbd_set_factorisation(self._h, which, _p(V), _p(Vi), _p(lam), _p(fix)))
+
+    # kernel variants: bit 1 fused x transforms, 2 fused rhs + solve and
+    # pressure chain, 4 scalar on a second stream
+    VARIANT_BITS = {"fused": 1, "rhs": 2, "overlap": 4}
+
+    def autotune(self, steps=5):
+        """Times the kernel variants on a scratch copy and keeps the fastest;
+        returns (bits kept, ms per step of each variant, NaN where not tried)."""
+        ms = np.zeros(8)
+        b = ctypes.c_int()
+        _check(lib().tbd_autotune(self._h, int(steps), _p(ms), ctypes.byref(b)))
+        return b.value, ms
+
+    def variant(self):
+        b = ctypes.c_int()
+        _check(lib().tbd_get_variant(self._h, ctypes.byref(b)))
+        return b.value
+
+    def set_variant(self, bits):
+        _check(lib().tbd_set_variant(self._h, int(bits)))
 
     def helmholtz(self, which, stage, mode, x):
         xx = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
