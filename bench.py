@@ -262,7 +262,8 @@ def main():
     line["profiled_ms_per_step"] = {prec: r["profiled_ms_per_step"] for prec, r in results.items()}
     line["class_rates"] = {prec: class_rates(r["prof"]) for prec, r in results.items()}
     # kernel variant the autotuner kept (ms per step of every variant it tried,
-    # keyed by variant bits: 1 fused x transforms, 2 fused rhs + solve, 4 overlap)
+    # keyed by variant bits: 1 fused x transforms, 2 fused rhs + solve, 4 overlap,
+    # 8 wide transform kernels)
     line["autotune"] = {prec: r["tune"] for prec, r in results.items()}
     if rank == 0 and not args.no_cpu_baseline:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)

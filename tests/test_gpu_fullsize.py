@@ -58,6 +58,22 @@ def test_full_size_step_every_field(oracle_step, precision, tol, ptol):
     assert errs["p"] < ptol, errs
 
 
+def test_full_size_step_wide_transform_kernels(oracle_step):
+    # the transform kernels at twice their default workgroup size (an
+    # autotuner variant) on the same step
+    cfg, state, time, alternation, (uo, po, co) = oracle_step
+    g = RT.Solver(cfg)
+    g.upload(*state)
+    g.set_variant(g.variant() | RT.Solver.VARIANT_BITS["wide_fft"])
+    g.step(time, alternation)
+    ug, pg, cg = g.download()
+    g.close()
+    uo, ug = uo.reshape(3, -1), ug.reshape(3, -1)
+    assert max(rel_l2(ug[i], uo[i]) for i in range(3)) < 1e-9
+    assert rel_l2(cg, co) < 1e-9
+    assert rel_l2(pg, po) < 1e-7
+
+
 def test_full_size_transforms():
     rng = np.random.default_rng(7)
     u = rng.standard_normal((1, 257, 256, 256))

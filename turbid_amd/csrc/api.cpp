@@ -196,7 +196,7 @@ int tbd_set_factorisation(tbd_ctx* ctx, int which, const double* V, const double
 int tbd_autotune(tbd_ctx* ctx, int steps, double* ms_per_step, int* bits) {
   return guard([&] {
     TBD_CHECK(ctx, "tbd_autotune > null context");
-    double ms[8];
+    double ms[tbd::Context::kVariants];
     const int b = ctx->c->autotune(steps, ms);
     if (ms_per_step) std::memcpy(ms_per_step, ms, sizeof(ms));
     if (bits) *bits = b;
@@ -213,12 +213,8 @@ int tbd_get_variant(tbd_ctx* ctx, int* bits) {
 int tbd_set_variant(tbd_ctx* ctx, int bits) {
   return guard([&] {
     TBD_CHECK(ctx, "tbd_set_variant > null context");
-    TBD_CHECK(bits >= 0 && bits < 8, "tbd_set_variant > bits out of range");
-    tbd::Context::Variant v;
-    v.fused = (bits & 1) != 0;
-    v.rhs = (bits & 2) != 0;
-    v.overlap = (bits & 4) != 0;
-    ctx->c->set_variant(v);
+    TBD_CHECK(bits >= 0 && bits < tbd::Context::kVariants, "tbd_set_variant > bits out of range");
+    ctx->c->set_variant(tbd::Context::Variant::of(bits));
   });
 }
 

@@ -235,6 +235,10 @@ Context::Context(const tbd_config& c) : cfg(c) {
     overlap_ = true;
   }
   fft_.create(d);
+  if (const char* e = std::getenv("TBD_FFT_WIDE")) {
+    fft_.wide = std::string(e) == "1";
+    pin_wide_ = true;
+  }
   if (const char* e = std::getenv("TBD_FUSED")) {
     fused_ = std::string(e) == "1";
     pin_fused_ = true;
@@ -471,6 +475,7 @@ Context::Variant Context::variant() const {
   v.fused = fused_;
   v.rhs = want_rhs_;
   v.overlap = overlap_;
+  v.wide = fft_.wide;
   return v;
 }
 
@@ -479,6 +484,7 @@ void Context::set_variant(const Variant& v) {
   if (!pin_fused_) fused_ = v.fused;
   if (!pin_rhs_) want_rhs_ = v.rhs;
   if (!pin_overlap_) overlap_ = v.overlap;
+  if (!pin_wide_) fft_.wide = v.wide;
   fuse_rhs_ = parity_ && d.nz % 2 == 1 && d.nz >= 7 && want_rhs_;
   fuse_p_ = fuse_rhs_ && hsplit_[3];
   drop_graphs();
@@ -486,13 +492,13 @@ void Context::set_variant(const Variant& v) {
 
 int Context::autotune(int steps, double* ms8) {
   TBD_CHECK(steps >= 1 && steps <= 10000, "tbd_autotune > steps out of range");
-  for (int i = 0; i < 8; ++i) ms8[i] = std::nan("");
+  for (int i = 0; i < kVariants; ++i) ms8[i] = std::nan("");
   // this context's state, uploaded to the scratch context before each variant
   const size_t n2 = 2 * (size_t)d.S;
   std::vector<double> u(3 * n2), p(n2), c(cfg.nphase > 1 ? n2 : 0);
   download(u.data(), p.data(), c.empty() ? nullptr : c.data());
   Context scratch(cfg);
-  scratch.pin_fused_ = scratch.pin_rhs_ = scratch.pin_overlap_ = false;   // the loop below decides
+  scratch.pin_fused_ = scratch.pin_rhs_ = scratch.pin_overlap_ = scratch.pin_wide_ = false;   // the loop below decides
   scratch.copy_operators_from(*this);
   scratch.set_dt(dt_);
   const bool rhs_ok = parity_ && d.nz % 2 == 1 && d.nz >= 7;
@@ -500,14 +506,13 @@ int Context::autotune(int steps, double* ms8) {
   int best = cur.bits();
   double best_ms = 0.0;
   std::vector<double> feat(50);
-  for (int bits = 0; bits < 8; ++bits) {
-    Variant v;
-    v.fused = (bits & 1) != 0;
-    v.rhs = (bits & 2) != 0;
-    v.overlap = (bits & 4) != 0;
+  for (int bits = 0; bits < kVariants; ++bits) {
+    const Variant v = Variant::of(bits);
     if ((pin_fused_ && v.fused != cur.fused) || (pin_rhs_ && v.rhs != cur.rhs) ||
-        (pin_overlap_ && v.overlap != cur.overlap))
+        (pin_overlap_ && v.overlap != cur.overlap) || (pin_wide_ && v.wide != cur.wide))
       continue;
+    // the wide kernels exist only for the two-level transform family
+    if (v.wide && !fft_.two_level) continue;
     // the fused right-hand side needs the even/odd operators; without them
     // both settings run the same kernels
     if (!rhs_ok && v.rhs != cur.rhs) continue;
@@ -534,11 +539,7 @@ int Context::autotune(int steps, double* ms8) {
       best = bits;
     }
   }
-  Variant v;
-  v.fused = (best & 1) != 0;
-  v.rhs = (best & 2) != 0;
-  v.overlap = (best & 4) != 0;
-  set_variant(v);
+  set_variant(Variant::of(best));
   return variant().bits();
 }
 

@@ -85,19 +85,32 @@ class Context {
   //           even/odd D1, D2; without them the switch has no effect)
   //   overlap scalar stages on a second stream, the even/odd-form solve of w
   //           beside those of u, v on a third
+  //   wide    two-level transform kernels with twice the default workgroup
+  //           size (twice the sequences per workgroup)
   // A switch set in the environment at creation (TBD_FUSED, TBD_RHS_UNFUSED,
-  // TBD_NO_OVERLAP) is pinned: set_variant leaves it and autotune does not vary it.
+  // TBD_NO_OVERLAP, TBD_FFT_WIDE) is pinned: set_variant leaves it and
+  // autotune does not vary it.
+  static constexpr int kVariants = 16;
   struct Variant {
-    bool fused = false, rhs = true, overlap = true;
-    int bits() const { return (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0); }
+    bool fused = false, rhs = true, overlap = true, wide = false;
+    int bits() const { return (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0) | (wide ? 8 : 0); }
+    static Variant of(int bits) {
+      Variant v;
+      v.fused = (bits & 1) != 0;
+      v.rhs = (bits & 2) != 0;
+      v.overlap = (bits & 4) != 0;
+      v.wide = (bits & 8) != 0;
+      return v;
+    }
   };
   Variant variant() const;
   void set_variant(const Variant& v);
   // Times every admissible variant on a scratch context (same configuration,
   // operators and dt; this context's state uploaded to it before each) for
   // `steps` captured steps after three warm-up steps, and keeps the fastest.
-  // ms[bits] = ms per step (NaN where not tried); returns the kept bits.
-  int autotune(int steps, double* ms8);
+  // ms[bits] = ms per step (NaN where not tried, kVariants entries); returns
+  // the kept bits.
+  int autotune(int steps, double* ms);
 
   tbd_config cfg;
   Dims d;
@@ -186,7 +199,7 @@ class Context {
   double* vinvp_[5] = {nullptr};   // V~^-1 with permuted columns (rhs_helmholtz)
   bool fuse_rhs_ = false;          // rhs + boundary rows + Helmholtz fused (TBD_RHS_UNFUSED=1 disables)
   bool want_rhs_ = true;           // the variant switch (fuse_rhs_ also needs the even/odd operators)
-  bool pin_fused_ = false, pin_rhs_ = false, pin_overlap_ = false;
+  bool pin_fused_ = false, pin_rhs_ = false, pin_overlap_ = false, pin_wide_ = false;
   bool fuse_p_ = false;            // divergence + pressure solve + projection fused (same switch)
   // The scalar's Correct of stage s reads the carrier's physical velocity of
   // stage s and writes only scalar state; without buoyancy the carrier never

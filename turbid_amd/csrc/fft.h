@@ -33,6 +33,9 @@ struct FftPlan {
   // two-level register kernels where the length allows; false selects the
   // LDS Stockham kernels for every length (environment TBD_FFT_STOCKHAM=1)
   bool two_level = true;
+  // two-level kernels with twice the default workgroup size (twice the
+  // sequences per workgroup); chosen per grid by the autotuner
+  bool wide = false;
   void create(const Dims& dims);
   void destroy();
 };

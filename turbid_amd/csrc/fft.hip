@@ -494,10 +494,9 @@ template <> struct Nt2<float> { static constexpr int v = 256; };
   F(384, 24, 16) F(256, 16, 16) F(192, 16, 12) F(128, 16, 8) F(96, 12, 8) F(64, 8, 8) F(48, 8, 6)    \
   F(32, 8, 4) F(24, 6, 4) F(16, 4, 4) F(12, 4, 3) F(8, 4, 2) F(6, 3, 2) F(4, 4, 1) F(3, 3, 1) F(2, 2, 1)
 
-template <typename T, typename SpecT>
-static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
+template <typename T, typename SpecT, int NT>
+static bool ypf2_nt(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
                  hipStream_t st, int scol, const int* gate) {
-  constexpr int NT = Nt2<T>::v;
   const T scale = (T)(1.0 / ((double)p.d.nx * (double)p.d.ny));
   const int sc = scol > 0 ? scol : p.d.nz;
   switch (p.d.ny) {
@@ -516,10 +515,9 @@ static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstr
   }
 }
 
-template <typename T, typename SpecT>
-static bool ypb2(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
+template <typename T, typename SpecT, int NT>
+static bool ypb2_nt(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
                  hipStream_t st, int scol) {
-  constexpr int NT = Nt2<T>::v;
   const int sc = scol > 0 ? scol : p.d.nz;
   switch (p.d.ny) {
 #define TBD_CASE(NN, A, B)                                                                                  \
@@ -537,10 +535,9 @@ static bool ypb2(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* 
   }
 }
 
-template <typename T>
-static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
+template <typename T, int NT>
+static bool xpf2_nt(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
                  hipStream_t st, const int* gate) {
-  constexpr int NT = Nt2<T>::v;
   switch (p.d.nx) {
 #define TBD_CASE(NN, A, B)                                                                              \
   case NN: {                                                                                            \
@@ -557,10 +554,9 @@ static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp,
   }
 }
 
-template <typename T>
-static bool xpb2(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
+template <typename T, int NT>
+static bool xpb2_nt(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
                  hipStream_t st) {
-  constexpr int NT = Nt2<T>::v;
   switch (p.d.nx) {
 #define TBD_CASE(NN, A, B)                                                                              \
   case NN: {                                                                                            \
@@ -575,6 +571,34 @@ static bool xpb2(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride,
     default:
       return false;
   }
+}
+
+// The two-level kernels at the default workgroup size (Nt2: 16 sequences of
+// 256 per FP32 workgroup, 8 per FP64 one) or twice it (FftPlan::wide), a
+// choice the autotuner makes per grid.
+template <typename T, typename SpecT>
+static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
+                 hipStream_t st, int scol, const int* gate) {
+  return p.wide ? ypf2_nt<T, SpecT, 2 * Nt2<T>::v>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol, gate)
+                : ypf2_nt<T, SpecT, Nt2<T>::v>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol, gate);
+}
+template <typename T, typename SpecT>
+static bool ypb2(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
+                 hipStream_t st, int scol) {
+  return p.wide ? ypb2_nt<T, SpecT, 2 * Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol)
+                : ypb2_nt<T, SpecT, Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol);
+}
+template <typename T>
+static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
+                 hipStream_t st, const int* gate) {
+  return p.wide ? xpf2_nt<T, 2 * Nt2<T>::v>(p, phys, pstride, tmp, nv, nplanes, st, gate)
+                : xpf2_nt<T, Nt2<T>::v>(p, phys, pstride, tmp, nv, nplanes, st, gate);
+}
+template <typename T>
+static bool xpb2(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
+                 hipStream_t st) {
+  return p.wide ? xpb2_nt<T, 2 * Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st)
+                : xpb2_nt<T, Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -326,13 +326,14 @@ class Solver:
         _check(lib().tbd_set_factorisation(self._h, which, _p(V), _p(Vi), _p(lam), _p(fix)))
 
     # kernel variants: bit 1 fused x transforms, 2 fused rhs + solve and
-    # pressure chain, 4 scalar on a second stream
-    VARIANT_BITS = {"fused": 1, "rhs": 2, "overlap": 4}
+    # pressure chain, 4 scalar on a second stream, 8 wide transform kernels
+    VARIANT_BITS = {"fused": 1, "rhs": 2, "overlap": 4, "wide_fft": 8}
+    N_VARIANTS = 16
 
     def autotune(self, steps=5):
         """Times the kernel variants on a scratch copy and keeps the fastest;
         returns (bits kept, ms per step of each variant, NaN where not tried)."""
-        ms = np.zeros(8)
+        ms = np.zeros(self.N_VARIANTS)
         b = ctypes.c_int()
         _check(lib().tbd_autotune(self._h, int(steps), _p(ms), ctypes.byref(b)))
         return b.value, ms
