@@ -34,7 +34,8 @@ def _steps(g, cfg, n):
 def test_every_variant_steps_like_the_default(precision, tol):
     cfg = small_config(precision=precision)
     ref = _steps(_solver(cfg, DEFAULT), cfg, 6)
-    for bits in range(RT.Solver.N_VARIANTS):
+    wide = [RT.Solver.VARIANT_BITS[k] for k in ("wide_yb", "wide_yf", "wide_xb", "wide_xf")]
+    for bits in list(range(8)) + [DEFAULT | w for w in wide] + [DEFAULT | RT.Solver.WIDE_FFT]:
         g = _solver(cfg, bits)
         assert g.variant() == bits
         got = _steps(g, cfg, 6)
@@ -52,7 +53,9 @@ def test_autotune_keeps_state_and_the_fastest_variant():
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
     tried = [i for i in range(RT.Solver.N_VARIANTS) if not math.isnan(ms[i])]
-    assert len(tried) == RT.Solver.N_VARIANTS
+    # the eight step-level combinations, then each of the four widths
+    assert len(tried) == 12
+    assert all(i in tried for i in range(8))
     assert all(ms[i] > 0 for i in tried)
     assert bits == min(tried, key=lambda i: ms[i])
     assert g.variant() == bits
@@ -71,11 +74,14 @@ def test_environment_pins_a_switch(monkeypatch):
     assert g.variant() & RT.Solver.VARIANT_BITS["fused"] == 0
     bits, ms = g.autotune(2)
     assert bits & RT.Solver.VARIANT_BITS["fused"] == 0
-    assert all(math.isnan(ms[i]) for i in range(RT.Solver.N_VARIANTS) if i & 1)
-    assert all(not math.isnan(ms[i]) for i in range(RT.Solver.N_VARIANTS) if not i & 1)
+    tried = [i for i in range(RT.Solver.N_VARIANTS) if not math.isnan(ms[i])]
+    assert not any(i & 1 for i in tried)
+    # the four admissible step-level combinations (at the widths set above),
+    # then the four widths
+    assert {i & 7 for i in tried} == {0, 2, 4, 6} and len(tried) == 8
 
 
 def test_set_variant_rejects_bad_bits():
     g = _solver(small_config())
     with pytest.raises(RT.TurbidError, match="out of range"):
-        g.set_variant(16)
+        g.set_variant(RT.Solver.N_VARIANTS)

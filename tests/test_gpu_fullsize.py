@@ -64,7 +64,7 @@ def test_full_size_step_wide_transform_kernels(oracle_step):
     cfg, state, time, alternation, (uo, po, co) = oracle_step
     g = RT.Solver(cfg)
     g.upload(*state)
-    g.set_variant(g.variant() | RT.Solver.VARIANT_BITS["wide_fft"])
+    g.set_variant(g.variant() | RT.Solver.WIDE_FFT)
     g.step(time, alternation)
     ug, pg, cg = g.download()
     g.close()

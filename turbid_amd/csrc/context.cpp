@@ -236,7 +236,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   }
   fft_.create(d);
   if (const char* e = std::getenv("TBD_FFT_WIDE")) {
-    fft_.wide = std::string(e) == "1";
+    fft_.wide_yb = fft_.wide_yf = fft_.wide_xb = fft_.wide_xf = std::string(e) == "1";
     pin_wide_ = true;
   }
   if (const char* e = std::getenv("TBD_FUSED")) {
@@ -475,7 +475,10 @@ Context::Variant Context::variant() const {
   v.fused = fused_;
   v.rhs = want_rhs_;
   v.overlap = overlap_;
-  v.wide = fft_.wide;
+  v.wide[0] = fft_.wide_yb;
+  v.wide[1] = fft_.wide_yf;
+  v.wide[2] = fft_.wide_xb;
+  v.wide[3] = fft_.wide_xf;
   return v;
 }
 
@@ -484,7 +487,12 @@ void Context::set_variant(const Variant& v) {
   if (!pin_fused_) fused_ = v.fused;
   if (!pin_rhs_) want_rhs_ = v.rhs;
   if (!pin_overlap_) overlap_ = v.overlap;
-  if (!pin_wide_) fft_.wide = v.wide;
+  if (!pin_wide_) {
+    fft_.wide_yb = v.wide[0];
+    fft_.wide_yf = v.wide[1];
+    fft_.wide_xb = v.wide[2];
+    fft_.wide_xf = v.wide[3];
+  }
   fuse_rhs_ = parity_ && d.nz % 2 == 1 && d.nz >= 7 && want_rhs_;
   fuse_p_ = fuse_rhs_ && hsplit_[3];
   drop_graphs();
@@ -506,16 +514,20 @@ int Context::autotune(int steps, double* ms8) {
   int best = cur.bits();
   double best_ms = 0.0;
   std::vector<double> feat(50);
-  for (int bits = 0; bits < kVariants; ++bits) {
-    const Variant v = Variant::of(bits);
+  auto admissible = [&](const Variant& v) {
     if ((pin_fused_ && v.fused != cur.fused) || (pin_rhs_ && v.rhs != cur.rhs) ||
-        (pin_overlap_ && v.overlap != cur.overlap) || (pin_wide_ && v.wide != cur.wide))
-      continue;
-    // the wide kernels exist only for the two-level transform family
-    if (v.wide && !fft_.two_level) continue;
+        (pin_overlap_ && v.overlap != cur.overlap))
+      return false;
+    for (int i = 0; i < 4; ++i) {
+      if (pin_wide_ && v.wide[i] != cur.wide[i]) return false;
+      // the wide kernels exist only for the two-level transform family
+      if (v.wide[i] && !fft_.two_level) return false;
+    }
     // the fused right-hand side needs the even/odd operators; without them
     // both settings run the same kernels
-    if (!rhs_ok && v.rhs != cur.rhs) continue;
+    return rhs_ok || v.rhs == cur.rhs;
+  };
+  auto time_variant = [&](const Variant& v) {
     scratch.set_variant(v);
     scratch.upload(u.data(), p.data(), c.empty() ? nullptr : c.data());
     double time = 0.0;
@@ -533,11 +545,23 @@ int Context::autotune(int steps, double* ms8) {
     scratch.sync();
     const auto t1 = std::chrono::steady_clock::now();
     const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
-    ms8[bits] = ms;
+    ms8[v.bits()] = ms;
     if (best_ms == 0.0 || ms < best_ms) {
       best_ms = ms;
-      best = bits;
+      best = v.bits();
     }
+  };
+  // every combination of the three step-level switches at the current widths
+  for (int low = 0; low < 8; ++low) {
+    Variant v = Variant::of(low);
+    for (int i = 0; i < 4; ++i) v.wide[i] = cur.wide[i];
+    if (admissible(v)) time_variant(v);
+  }
+  // then each transform width in turn, from the best so far
+  for (int i = 0; i < 4; ++i) {
+    Variant v = Variant::of(best);
+    v.wide[i] = !v.wide[i];
+    if (admissible(v)) time_variant(v);
   }
   set_variant(Variant::of(best));
   return variant().bits();

@@ -85,29 +85,37 @@ class Context {
   //           even/odd D1, D2; without them the switch has no effect)
   //   overlap scalar stages on a second stream, the even/odd-form solve of w
   //           beside those of u, v on a third
-  //   wide    two-level transform kernels with twice the default workgroup
+  //   wide[4] two-level transform kernels of the y backward, y forward,
+  //           x backward and x forward pass with twice the default workgroup
   //           size (twice the sequences per workgroup)
   // A switch set in the environment at creation (TBD_FUSED, TBD_RHS_UNFUSED,
-  // TBD_NO_OVERLAP, TBD_FFT_WIDE) is pinned: set_variant leaves it and
-  // autotune does not vary it.
-  static constexpr int kVariants = 16;
+  // TBD_NO_OVERLAP; TBD_FFT_WIDE for the four widths) is pinned: set_variant
+  // leaves it and autotune does not vary it.
+  static constexpr int kVariants = 128;
   struct Variant {
-    bool fused = false, rhs = true, overlap = true, wide = false;
-    int bits() const { return (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0) | (wide ? 8 : 0); }
+    bool fused = false, rhs = true, overlap = true;
+    bool wide[4] = {false, false, false, false};
+    int bits() const {
+      int b = (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0);
+      for (int i = 0; i < 4; ++i) b |= wide[i] ? 8 << i : 0;
+      return b;
+    }
     static Variant of(int bits) {
       Variant v;
       v.fused = (bits & 1) != 0;
       v.rhs = (bits & 2) != 0;
       v.overlap = (bits & 4) != 0;
-      v.wide = (bits & 8) != 0;
+      for (int i = 0; i < 4; ++i) v.wide[i] = (bits & (8 << i)) != 0;
       return v;
     }
   };
   Variant variant() const;
   void set_variant(const Variant& v);
-  // Times every admissible variant on a scratch context (same configuration,
-  // operators and dt; this context's state uploaded to it before each) for
-  // `steps` captured steps after three warm-up steps, and keeps the fastest.
+  // Times variants on a scratch context (same configuration, operators and
+  // dt; this context's state uploaded to it before each) for `steps`
+  // captured steps after three warm-up steps: every combination of fused,
+  // rhs and overlap, then each transform width in turn from the best so far
+  // (kept when faster), and keeps the fastest.
   // ms[bits] = ms per step (NaN where not tried, kVariants entries); returns
   // the kept bits.
   int autotune(int steps, double* ms);

@@ -33,9 +33,10 @@ struct FftPlan {
   // two-level register kernels where the length allows; false selects the
   // LDS Stockham kernels for every length (environment TBD_FFT_STOCKHAM=1)
   bool two_level = true;
-  // two-level kernels with twice the default workgroup size (twice the
-  // sequences per workgroup); chosen per grid by the autotuner
-  bool wide = false;
+  // two-level kernels of each pass (y backward, y forward, x backward, x
+  // forward) with twice the default workgroup size (twice the sequences per
+  // workgroup); chosen per grid and pass by the autotuner
+  bool wide_yb = false, wide_yf = false, wide_xb = false, wide_xf = false;
   void create(const Dims& dims);
   void destroy();
 };

@@ -574,30 +574,30 @@ static bool xpb2_nt(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstri
 }
 
 // The two-level kernels at the default workgroup size (Nt2: 16 sequences of
-// 256 per FP32 workgroup, 8 per FP64 one) or twice it (FftPlan::wide), a
-// choice the autotuner makes per grid.
+// 256 per FP32 workgroup, 8 per FP64 one) or twice it (FftPlan::wide_*), a
+// choice the autotuner makes per grid and pass.
 template <typename T, typename SpecT>
 static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
                  hipStream_t st, int scol, const int* gate) {
-  return p.wide ? ypf2_nt<T, SpecT, 2 * Nt2<T>::v>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol, gate)
+  return p.wide_yf ? ypf2_nt<T, SpecT, 2 * Nt2<T>::v>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol, gate)
                 : ypf2_nt<T, SpecT, Nt2<T>::v>(p, tmp, spec, sstride, nv, zoff, nplanes, st, scol, gate);
 }
 template <typename T, typename SpecT>
 static bool ypb2(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
                  hipStream_t st, int scol) {
-  return p.wide ? ypb2_nt<T, SpecT, 2 * Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol)
+  return p.wide_yb ? ypb2_nt<T, SpecT, 2 * Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol)
                 : ypb2_nt<T, SpecT, Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol);
 }
 template <typename T>
 static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
                  hipStream_t st, const int* gate) {
-  return p.wide ? xpf2_nt<T, 2 * Nt2<T>::v>(p, phys, pstride, tmp, nv, nplanes, st, gate)
+  return p.wide_xf ? xpf2_nt<T, 2 * Nt2<T>::v>(p, phys, pstride, tmp, nv, nplanes, st, gate)
                 : xpf2_nt<T, Nt2<T>::v>(p, phys, pstride, tmp, nv, nplanes, st, gate);
 }
 template <typename T>
 static bool xpb2(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
                  hipStream_t st) {
-  return p.wide ? xpb2_nt<T, 2 * Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st)
+  return p.wide_xb ? xpb2_nt<T, 2 * Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st)
                 : xpb2_nt<T, Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st);
 }
 

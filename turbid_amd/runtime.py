@@ -326,9 +326,13 @@ class Solver:
         _check(lib().tbd_set_factorisation(self._h, which, _p(V), _p(Vi), _p(lam), _p(fix)))
 
     # kernel variants: bit 1 fused x transforms, 2 fused rhs + solve and
-    # pressure chain, 4 scalar on a second stream, 8 wide transform kernels
-    VARIANT_BITS = {"fused": 1, "rhs": 2, "overlap": 4, "wide_fft": 8}
-    N_VARIANTS = 16
+    # pressure chain, 4 scalar on a second stream, 8/16/32/64 y backward /
+    # y forward / x backward / x forward transform kernels at twice their
+    # workgroup size
+    VARIANT_BITS = {"fused": 1, "rhs": 2, "overlap": 4, "wide_yb": 8, "wide_yf": 16, "wide_xb": 32,
+                    "wide_xf": 64}
+    WIDE_FFT = 8 | 16 | 32 | 64
+    N_VARIANTS = 128
 
     def autotune(self, steps=5):
         """Times the kernel variants on a scratch copy and keeps the fastest;
