@@ -82,9 +82,13 @@ def run(cfg, state, steps, warmup, autotune):
     s.upload(*state)
     tune = None
     if autotune:
-        bits, ms = s.autotune(5)
-        tune = {"kept": {k: bool(bits & b) for k, b in RT.Solver.VARIANT_BITS.items()},
-                "ms_per_step": {str(i): (None if np.isnan(v) else round(float(v), 3)) for i, v in enumerate(ms)}}
+        try:
+            bits, ms = s.autotune(5)
+        except RT.TurbidError as e:   # e.g. no device memory for the scratch context
+            log("bench: autotune failed (%s); default variant" % e)
+        else:
+            tune = {"kept": {k: bool(bits & b) for k, b in RT.Solver.VARIANT_BITS.items()},
+                    "ms_per_step": {str(i): round(float(v), 3) for i, v in enumerate(ms) if not np.isnan(v)}}
     st = RT.Stepper(cfg["dt"])
     RT.run_steps(s, st, warmup)
     s.sync()
