@@ -74,6 +74,33 @@ def test_full_size_step_wide_transform_kernels(oracle_step):
     assert rel_l2(pg, po) < 1e-7
 
 
+def test_384_grid_step():
+    # the same physics on the 384 x 384 x 257 grid, FP64, with the variant the
+    # autotuner keeps there (all four transform passes wide)
+    cfg = W.workload("turb384", precision=0)
+    state = W.initial_state(cfg)
+    o = O.Oracle(cfg, nthreads=NTHREADS)
+    o.set_state(*state)
+    st = RT.Stepper(cfg["dt"])
+    st.update(0.0)
+    for s in range(3):
+        o.substage(s, st.time, st.alternation)
+    uo, po, co = o.get_state()
+    o.close()
+    g = RT.Solver(cfg)
+    g.upload(*state)
+    g.set_variant(g.variant() | RT.Solver.WIDE_FFT)
+    g.step(st.time, st.alternation)
+    ug, pg, cg = g.download()
+    g.close()
+    uo, ug = uo.reshape(3, -1), ug.reshape(3, -1)
+    errs = {"u": rel_l2(ug[0], uo[0]), "v": rel_l2(ug[1], uo[1]), "w": rel_l2(ug[2], uo[2]),
+            "p": rel_l2(pg, po), "c": rel_l2(cg, co)}
+    print("384x384x257, FP64, one step, relative L2 per field: %s" % errs)
+    assert max(errs["u"], errs["v"], errs["w"], errs["c"]) < 1e-9, errs
+    assert errs["p"] < 1e-7, errs
+
+
 def test_full_size_transforms():
     rng = np.random.default_rng(7)
     u = rng.standard_normal((1, 257, 256, 256))
