@@ -89,6 +89,7 @@ def lib():
         L.tbo_dft_backward.argtypes = [ctypes.c_int] * 4 + [dp, dp]
         L.tbo_chebyshev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, dp, dp, dp]
         L.tbo_eig.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.tbo_positivity_filter.argtypes = [ctypes.c_int] * 3 + [dp]
         L.tbo_helmholtz.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, dp, ctypes.c_int, dp, dp]
         _lib = L
@@ -145,6 +146,16 @@ def eig(a):
     if rc != 0:
         raise RuntimeError("oracle eig failed (%d)" % rc)
     return lam, v
+
+
+def positivity_filter(phys):
+    """The scalar's positivity filter on physical planes phys[nz, ny, nx]
+    (returns a filtered copy)."""
+    out = np.ascontiguousarray(phys, dtype=np.float64).copy()
+    nz, ny, nx = out.shape
+    rc = lib().tbo_positivity_filter(nx, ny, nz, _p(out))
+    assert rc == 0
+    return out
 
 
 def wavenumbers(cfg):

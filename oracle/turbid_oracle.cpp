@@ -568,6 +568,35 @@ dvec MappedGrid(const dvec& gridz, int mapping, double coef) {
 // ---------------------------------------------------------------------------
 // CmDz2DecompositionCplx (cmdz2_decomposition.cc:19-297)
 // ---------------------------------------------------------------------------
+// Bartnicki positivity filter on physical z-planes (phase_scalar_flat.cc:
+// 419-478): five passes; each pass sums per plane the positive values and
+// their count, the zeros and the negative values; a plane whose total is
+// negative is zeroed, otherwise the positive points share the negative mass
+// equally (v += sum_neg / n_pos) and every other point is set to 0.
+void PositivityFilter(double* phys, long long plane, int nplanes) {
+  for (int it = 0; it < 5; ++it) {
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nplanes; ++z) {
+      double* pl = phys + z * plane;
+      double m[6] = {0, 0, 0, 0, 0, 0};
+      for (long long e = 0; e < plane; ++e) {
+        const double v = pl[e];
+        if (v > 0) { m[0] += v; m[1] += 1; }
+        else if (v < 0) { m[4] += v; m[5] += 1; }
+        else { m[2] += v; m[3] += 1; }
+      }
+      if (m[0] + m[2] + m[4] < 0) {
+        for (long long e = 0; e < plane; ++e) pl[e] = 0;
+        continue;
+      }
+      for (long long e = 0; e < plane; ++e) {
+        if (pl[e] > 0) pl[e] += m[4] / m[1];
+        else pl[e] = 0;
+      }
+    }
+  }
+}
+
 struct Decomposition {
   int n = 0;
   dvec vt;              // zboundarytype (5)
@@ -1166,28 +1195,7 @@ struct Oracle {
     Helmholtz(dc[4], stage, 0, rhsphi.data(), c.data(), P);                         // :399
     // Bartnicki positivity filter (:419-478)
     fft->Backward(c.data(), ptmp1_r.data(), 1);
-    const long long plane = (long long)Nx * Ny;
-    for (int it = 0; it < 5; ++it) {
-#pragma omp parallel for schedule(static)
-      for (int z = 0; z < Nz; ++z) {
-        double* pl = ptmp1_r.data() + z * plane;
-        double m[6] = {0, 0, 0, 0, 0, 0};
-        for (long long e = 0; e < plane; ++e) {
-          const double v = pl[e];
-          if (v > 0) { m[0] += v; m[1] += 1; }
-          else if (v < 0) { m[4] += v; m[5] += 1; }
-          else { m[2] += v; m[3] += 1; }
-        }
-        if (m[0] + m[2] + m[4] < 0) {
-          for (long long e = 0; e < plane; ++e) pl[e] = 0;
-          continue;
-        }
-        for (long long e = 0; e < plane; ++e) {
-          if (pl[e] > 0) pl[e] += m[4] / m[1];
-          else pl[e] = 0;
-        }
-      }
-    }
+    PositivityFilter(ptmp1_r.data(), (long long)Nx * Ny, Nz);
     fft->Forward(ptmp1_r.data(), c.data(), 1);
     featlocal[21] = Trapz(c.data()) / 2;                                            // :485-486
   }
@@ -1346,6 +1354,12 @@ int tbo_chebyshev(int nz, int mapping, double coef, double* gridz, double* d1, d
   if (gridz) std::memcpy(gridz, g.data(), sizeof(double) * nz);
   if (d1) std::memcpy(d1, a.data(), sizeof(double) * nz * nz);
   if (d2) std::memcpy(d2, b.data(), sizeof(double) * nz * nz);
+  return 0;
+}
+
+int tbo_positivity_filter(int nx, int ny, int nz, double* phys) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) { g_err = "bad plane shape"; return -1; }
+  PositivityFilter(phys, (long long)nx * ny, nz);
   return 0;
 }
 

@@ -82,6 +82,9 @@ int tbo_dft_forward(int nx, int ny, int nz, int nv, const double* phys, double* 
 int tbo_dft_backward(int nx, int ny, int nz, int nv, const double* spec, double* phys);
 /* CGL grid + D1, D2 (stepper_space_chebyshev.cc:101-228), col-major. */
 int tbo_chebyshev(int nz, int mapping, double coef, double* gridz, double* d1, double* d2);
+/* The scalar's positivity filter, in place, on nz physical planes of nx*ny
+ * values (x fastest). */
+int tbo_positivity_filter(int nx, int ny, int nz, double* phys);
 /* Real eigen-decomposition of a real n*n col-major matrix with real spectrum:
  * lambda (n), V (n*n col-major, unit columns).  Returns 0, or <0 if complex. */
 int tbo_eig(int n, const double* a, double* lambda, double* v);

@@ -372,6 +372,42 @@ def test_uniform_concentration_invariant():
     assert np.abs(C).max() < 1e-8
 
 
+@pytest.mark.parametrize("plane,expect", [
+    # two passes: the first leaves two small positives negative, the second
+    # moves their mass onto the one large value
+    ([0.1, 0.1, -1.0, 5.0], [0.0, 0.0, 0.0, 4.2]),
+    # exact zeros are neither positive nor negative: they take no share
+    ([2.0, 0.0, -1.0, 0.0], [1.0, 0.0, 0.0, 0.0]),
+    # a plane with negative total mass is zeroed
+    ([1.0, -2.0, 0.5, 0.0], [0.0, 0.0, 0.0, 0.0]),
+    # zero total: the positive value absorbs all of it
+    ([1.0, -1.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0]),
+])
+def test_positivity_filter_hand_cases(plane, expect):
+    got = O.positivity_filter(np.array(plane).reshape(1, 2, 2))
+    np.testing.assert_allclose(got.ravel(), expect, rtol=0, atol=1e-14)
+
+
+def test_positivity_filter_invariants():
+    rng = np.random.default_rng(3)
+    c = 1.0 + 2.0 * rng.standard_normal((6, 10, 12))
+    c[2] -= 3.0                    # negative total: zeroed
+    c[4] = np.abs(c[4])            # non-negative: left alone, bit for bit
+    got = O.positivity_filter(c)
+    sums = c.sum(axis=(1, 2))
+    for z in range(c.shape[0]):
+        if sums[z] < 0:
+            assert not got[z].any()
+        else:
+            # the mass of each plane is kept and no negative value is left
+            assert abs(got[z].sum() - sums[z]) < 1e-12 * np.abs(c[z]).sum()
+            assert got[z].min() >= 0.0
+    assert np.array_equal(got[4], c[4])
+    # planes are filtered independently
+    for z in range(c.shape[0]):
+        assert np.array_equal(O.positivity_filter(c[z:z + 1])[0], got[z])
+
+
 def test_divergence_free_after_correct():
     cfg = small_config(nx=12, ny=12, nz=33)
     uvw, p, c = W.initial_state(cfg)
