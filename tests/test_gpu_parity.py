@@ -335,3 +335,20 @@ def test_steps_positivity_filter_active(precision):
     tol = 1e-9 if precision == 0 else 1e-5
     assert rel_l2(cg, co) < tol
     assert rel_l2(ug, uo) < tol
+
+
+def test_bad_step_arguments_are_errors():
+    # the reference's Class::Method > Message errors, returned through the
+    # C ABI; the context stays usable afterwards
+    cfg = small_config()
+    g = RT.Solver(cfg)
+    uvw, p, c = W.initial_state(cfg)
+    g.upload(uvw, p, c)
+    with pytest.raises(RT.TurbidError, match="stage must be 0, 1 or 2"):
+        g.substage(3, 0.0, 0)
+    for dt in (0.0, -1e-3, float("inf"), float("nan")):
+        with pytest.raises(RT.TurbidError, match="dt must be positive and finite"):
+            g.set_dt(dt)
+    g.step(0.0, 0)
+    u, _, _ = g.download()
+    assert np.all(np.isfinite(u))

@@ -364,7 +364,7 @@ Context::~Context() {
 void Context::sync() { TBD_HIP(hipStreamSynchronize(st_)); }
 
 void Context::set_dt(double dt) {
-  TBD_CHECK(dt > 0, "Context::set_dt > dt must be positive");
+  TBD_CHECK(dt > 0 && std::isfinite(dt), "Context::set_dt > dt must be positive and finite");
   if (dt != dt_) drop_graphs();
   dt_ = dt;
 }
