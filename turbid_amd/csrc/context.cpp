@@ -444,7 +444,8 @@ void Context::install_operators() {
   const char* nps = std::getenv("TBD_NO_PARITY");
   for (int k = 0; k < (cfg.nphase > 1 ? 5 : 4); ++k) {
     vinv_[k] = up(rowmajor(fac[k].vinv, nz, lda_));
-    vnorm_[k] = up(rowmajor(fac[k].vnorm, nz, lda_));
+    // the full-form solves read V~ with the backward fixes folded in
+    vnorm_[k] = up(rowmajor(fold_backward_fixes(fac[k].vnorm, fac[k].fix, nz), nz, lda_));
     lam_[k] = up(fac[k].lambda);
     fix_[k] = up(fac[k].fix);
     vinvp_[k] = nz % 2 == 1 ? up(permute_columns(rowmajor(fac[k].vinv, nz, lda_), nz, lda_)) : nullptr;

@@ -538,6 +538,26 @@ Factorisation make_factorisation(const Chebyshev& ch, const double* vt) {
   return f;
 }
 
+std::vector<double> fold_backward_fixes(const std::vector<double>& vnorm, const std::vector<double>& fix, int nz) {
+  TBD_CHECK(vnorm.size() == (size_t)nz * nz && fix.size() == (size_t)4 * nz,
+            "fold_backward_fixes > bad factorisation size");
+  const double* wb = fix.data() + 2 * (size_t)nz;
+  const double* wt = fix.data() + 3 * (size_t)nz;
+  std::vector<double> out(vnorm);
+  for (int j = 0; j < nz; ++j) {
+    const double* col = vnorm.data() + (size_t)nz * j;
+    long double b = 0.0L, t = 0.0L;
+    for (int i = 0; i < nz; ++i) {
+      b += (long double)wb[i] * col[i];
+      t += (long double)wt[i] * col[i];
+    }
+    t += (long double)wt[0] * (b - (long double)col[0]);
+    out[(size_t)nz * j] = (double)b;
+    out[(size_t)nz * j + nz - 1] = (double)t;
+  }
+  return out;
+}
+
 Factorisation factorisation_from(int nz, const double* vt, const double* V, const double* Vinv,
                                  const double* lambda, const double* fix) {
   const int N = nz, m = N - 2;

@@ -64,4 +64,12 @@ Factorisation make_factorisation(const Chebyshev& ch, const double* vt);
 Factorisation factorisation_from(int nz, const double* vt, const double* V, const double* Vinv,
                                  const double* lambda, const double* fix);
 
+// V~ (column-major nz*nz) with the backward fixes folded into its rows 0 and
+// nz-1 (cmdz2_decomposition.cc:54-80 builds them; phase_carrier_flat.cc:
+// 984-985 and phase_scalar_flat.cc:659-660 apply them to Y = V~ Z, the top
+// after the bottom): row 0 becomes
+// wb V~ and row nz-1 becomes wt V~ + wt_0 (wb V~ - V~_0), wb, wt = fix rows
+// 2 and 3, so that one contraction with it gives the fixed Y.
+std::vector<double> fold_backward_fixes(const std::vector<double>& vnorm, const std::vector<double>& fix, int nz);
+
 }  // namespace tbd

@@ -71,7 +71,7 @@ void zgemm(const ZArgs& a, int nfield, hipStream_t st);
 // factorisation.  FP64.
 struct HField {
   const double* vinv = nullptr;   // row-major, lda
-  const double* vnorm = nullptr;
+  const double* vnorm = nullptr;  // helmholtz(): V~ with the backward fixes folded in (fold_backward_fixes)
   const double* lambda = nullptr; // nz
   const double* fix = nullptr;    // 4*nz
   double sigma = 0;               // stage coefficient (0 for pressure)
@@ -107,7 +107,8 @@ double helmholtz_split_flops(int nz, long long P);
 //   gradient, boundary_carrier_flat.cc:185-246; scalar: zero flux and the
 //   erosion/deposition bottom row, phase_scalar_flat.cc:340-384), then
 //   y = the Helmholtz solve of rhs (forward fix, V~^-1, 1/(lambda - k^2 + sigma),
-//   V~, sequential backward fixes).
+//   V~, sequential backward fixes; the full form's V~ has the backward fixes
+//   folded into its rows 0 and nz-1).
 // D2 is applied through its even/odd split (nz odd).  Plain factorisations
 // take V~^-1 with its columns permuted to the order (0..M, nz-1..M+1)
 // (permute_columns); split ones take their even/odd form.  y may alias x.
@@ -118,7 +119,7 @@ struct RHField {
   double2* y = nullptr;           // solution
   int comp = 0;                   // carrier component (0..2) for the wall-row arrays
   const double* vinv = nullptr;   // permuted V~^-1, or pinv
-  const double* vnorm = nullptr;  // V~, or pnorm
+  const double* vnorm = nullptr;  // V~ with the backward fixes folded in, or pnorm
   const double* lambda = nullptr; // lambda, or plam
   const double* fix = nullptr;    // natural 4*nz fix array
   const double* pfix = nullptr;   // split only

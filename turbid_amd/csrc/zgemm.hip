@@ -711,35 +711,10 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HArgs a, 
     xs[c * ldx + row] = cscale<double>(r, v);
   });
   __syncthreads();
-  // Y = V~ Z, written over the panel
+  // Y = V~ Z, written over the panel; rows 0 and nz-1 of this V~ carry the
+  // backward fixes (the bottom, then the top after it), so Y is final
   gemm_panel<double, DEPTH>(vnorm, a.lda, pn, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  // sequential backward fixes: row 0 from Y, then row N-1 from the updated Y
-  const double* fbb = fix + 2 * nz;
-  const double* fbt = fix + 3 * nz;
-  const int part = threadIdx.x & 15;      // 16 threads per column
-  for (int pass = 0; pass < 2; ++pass) {
-    const double* w = pass == 0 ? fbb : fbt;
-    for (int cb = 0; cb < pn.ncol; cb += kNT / 16) {
-      const int c = cb + (threadIdx.x >> 4);
-      double sr = 0.0, si = 0.0;
-      if (c < pn.ncol) {
-        for (int j = part; j < nz; j += 16) {
-          const double2 v = xs[c * ldx + j];
-          sr += w[j] * v.x;
-          si += w[j] * v.y;
-        }
-      }
-#pragma unroll
-      for (int o = 8; o > 0; o >>= 1) {
-        sr += __shfl_xor(sr, o, 16);
-        si += __shfl_xor(si, o, 16);
-      }
-      __syncthreads();
-      if (part == 0 && c < pn.ncol) xs[c * ldx + (pass == 0 ? 0 : nz - 1)] = make_double2(sr, si);
-      __syncthreads();
-    }
-  }
   for (int idx = threadIdx.x; idx < pn.ncol * nz; idx += kNT) {
     const int cc = idx / nz, k = idx - cc * nz;
     if (cc < ncol) y[(c0 + cc) * nz + k] = xs[cc * ldx + k];
@@ -783,11 +758,11 @@ __host__ inline void make_split_panels(int nz, size_t lds_limit, Panel& p1, Pane
   p1.ntail = tailrows * p1.ncol;
 }
 
-// Backward fixes of every panel column, 16 lanes per column: t0 = wb . Y and
-// tN = wt . Y + fbt0 (t0 - Y_0) (the top fix sees row 0 already replaced);
-// Y is the panel column in natural order, or its split (a, b) when SPLIT, in
-// which case Y_0 = a_0 + b_0 and wb, wt are the folded weights.
-template <bool SPLIT>
+// Backward fixes of every panel column of the even/odd form, 16 to 64 lanes
+// per column: t0 = wb . Y and tN = wt . Y + fbt0 (t0 - Y_0) (the top fix sees
+// row 0 already replaced); Y is the split (a, b) of the panel column, so
+// Y_0 = a_0 + b_0 and wb, wt are the weights folded onto (a, b).  (The full
+// form has these rows folded into its V~ instead.)
 __device__ __forceinline__ void backward_fixes(const double2* xs, int ldx, int ncolp, int nz, int M,
                                                const double* wb, const double* wt, double fbt0,
                                                double2 (*tfix)[kSplitMaxCol]) {
@@ -834,29 +809,29 @@ __device__ __forceinline__ void backward_fixes(const double2* xs, int ldx, int n
       ti += __shfl_xor(ti, o, L);
     }
     if (part == 0 && c < ncolp) {
-      double2 y0 = xs[c * ldx];
-      if (SPLIT) y0 = cadd<double>(y0, xs[c * ldx + M + 1]);
+      const double2 y0 = cadd<double>(xs[c * ldx], xs[c * ldx + M + 1]);
       tfix[0][c] = make_double2(br, bi);
       tfix[1][c] = make_double2(tr + fbt0 * (br - y0.x), ti + fbt0 * (bi - y0.y));
     }
   }
 }
 
-// Writes the panel's ncol columns to y (column stride nz): rows 0 and nz-1
-// from tfix, the others from the panel (natural order, or y_i = a_i + b_i,
-// y_{nz-1-i} = a_i - b_i when SPLIT).
+// Writes the panel's ncol columns to y (column stride nz): the full form from
+// the panel in natural order (its V~ carries the backward fixes in rows 0 and
+// nz-1); the even/odd form with rows 0 and nz-1 from tfix and y_i = a_i + b_i,
+// y_{nz-1-i} = a_i - b_i from the panel.
 template <bool SPLIT>
 __device__ __forceinline__ void store_columns(const double2* xs, int ldx, int ncol, int nz, int M,
                                               const double2 (*tfix)[kSplitMaxCol], double2* y) {
   for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
     const int c = idx / nz, k = idx - c * nz;
     double2 v;
-    if (k == 0) {
+    if (!SPLIT) {
+      v = xs[c * ldx + k];
+    } else if (k == 0) {
       v = tfix[0][c];
     } else if (k == nz - 1) {
       v = tfix[1][c];
-    } else if (!SPLIT) {
-      v = xs[c * ldx + k];
     } else if (k <= M) {
       v = cadd<double>(xs[c * ldx + k], xs[c * ldx + M + 1 + k]);
     } else {
@@ -934,7 +909,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
   __syncthreads();
   gemm_panel<double, DEPTH>(a.f[0].vnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  backward_fixes<true>(xs, ldx, p1.ncol, nz, M, a.f[0].pfix, a.f[0].pfix + nz, fix[3 * nz], tfix);
+  backward_fixes(xs, ldx, p1.ncol, nz, M, a.f[0].pfix, a.f[0].pfix + nz, fix[3 * nz], tfix);
   __syncthreads();
   store_columns<true>(xs, ldx, ncol, nz, M, tfix, a.f[0].y + c0 * nz);
 }
@@ -1161,9 +1136,10 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   __syncthreads();
   gemm_panel<double, DEPTH>(F.vnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  backward_fixes<SPLIT>(xs, ldx, pd.ncol, nz, M, SPLIT ? F.pfix : fix + 2 * nz, SPLIT ? F.pfix + nz : fix + 3 * nz,
-                        fix[3 * nz], tfix);
-  __syncthreads();
+  if (SPLIT) {
+    backward_fixes(xs, ldx, pd.ncol, nz, M, F.pfix, F.pfix + nz, fix[3 * nz], tfix);
+    __syncthreads();
+  }
   store_columns<SPLIT>(xs, ldx, ncol, nz, M, tfix, F.y + c0 * nz);
 }
 
@@ -1292,7 +1268,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
   __syncthreads();
   gemm_panel<double, DEPTH>(a.pnorm, a.lda, p2, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
   __syncthreads();
-  backward_fixes<true>(xs, ldx, pd.ncol, nz, M, a.pfix, a.pfix + nz, a.fix[3 * nz], tfix);
+  backward_fixes(xs, ldx, pd.ncol, nz, M, a.pfix, a.pfix + nz, a.fix[3 * nz], tfix);
   __syncthreads();
   // p to HBM, and u = su - cf i kx p, v = sv - cf i ky p while p is in the
   // panel (natural row order; the wall rows of (i kx p, i ky p) saved).  The
