@@ -71,7 +71,8 @@ def _pair(cfg):
 
 
 @pytest.mark.parametrize("which,mode,nz", [(0, 0, 65), (2, 0, 65), (3, 1, 65), (4, 0, 65),
-                                          (2, 0, 129), (3, 1, 129), (2, 0, 257), (3, 1, 257)])
+                                          (2, 0, 129), (3, 1, 129), (0, 0, 257), (2, 0, 257),
+                                          (3, 1, 257), (4, 0, 257)])
 def test_helmholtz_vs_oracle(which, mode, nz):
     # w (no-slip at both walls) and p (Neumann at both walls) run through the
     # even/odd form of their factorisation; u and the scalar through the full one
