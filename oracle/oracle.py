@@ -54,9 +54,11 @@ def fill_config(struct_type, cfg):
 
 
 def build(force=False):
-    """Compile the restatement with its Makefile (g++, OpenMP)."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    """Compile the restatement with its Makefile (g++, OpenMP).  make runs on
+    every call (it is incremental), so an edited source is never tested
+    through a stale library."""
+    cmd = ["make", "-s", "-C", _HERE] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True)
     return _LIB_PATH
 
 
@@ -83,6 +85,8 @@ def lib():
         L.tbo_stepper.argtypes = [ctypes.c_void_p, ip, dp, dp, ip]
         L.tbo_features.argtypes = [ctypes.c_void_p, dp]
         L.tbo_get_physical_velocity.argtypes = [ctypes.c_void_p, dp]
+        L.tbo_nonlinear.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
+        L.tbo_get_history.argtypes = [ctypes.c_void_p, dp, dp]
         L.tbo_get_operators.argtypes = [ctypes.c_void_p, dp, dp, dp]
         L.tbo_get_decomposition.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, dp]
         L.tbo_dft_forward.argtypes = [ctypes.c_int] * 4 + [dp, dp]
@@ -254,6 +258,10 @@ class Oracle:
             raise RuntimeError(lib().tbo_last_error().decode())
         return rc
 
+    def set_dt(self, dt):
+        """Time mode: the next substages use dt (FlatCarrierPhase::Update)."""
+        lib().tbo_set_dt(self._h, float(dt))
+
     def stepper(self):
         st, al = ctypes.c_int(), ctypes.c_int()
         t, cfl = ctypes.c_double(), ctypes.c_double()
@@ -269,6 +277,22 @@ class Oracle:
         out = np.zeros(3 * self.nx * self.ny * self.nz)
         lib().tbo_get_physical_velocity(self._h, _p(out))
         return out.reshape(3, self.nz, self.ny, self.nx)
+
+    def nonlinear(self, alternation):
+        """Spectral nonlinear term u.grad u of the current state, (3, mx, nyc,
+        nz): convective form (alternation 0) or divergence form (1)."""
+        out = np.zeros(3 * self.S, dtype=np.complex128)
+        rc = lib().tbo_nonlinear(self._h, int(alternation), _p(out.view(np.float64)))
+        if rc != 0:
+            raise RuntimeError(lib().tbo_last_error().decode())
+        return out.reshape(3, self.mx, self.nyc, self.nz)
+
+    def history(self):
+        """Explicit-term histories (H of u, v, w; H of c or None)."""
+        h = np.zeros(3 * self.S, dtype=np.complex128)
+        hc = np.zeros(self.S, dtype=np.complex128) if self.cfg.get("nphase", 1) > 1 else None
+        lib().tbo_get_history(self._h, _p(h.view(np.float64)), None if hc is None else _p(hc.view(np.float64)))
+        return h, hc
 
     def operators(self):
         n = self.nz

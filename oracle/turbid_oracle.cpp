@@ -1307,6 +1307,25 @@ int tbo_get_physical_velocity(const tbo_ctx* ctx, double* out) {
   return 0;
 }
 
+int tbo_nonlinear(tbo_ctx* ctx, int alternation, double* adv) {
+  try {
+    Oracle* o = ctx->o;
+    const bool keep = o->alternation;
+    o->alternation = alternation != 0;
+    o->NonlinearVelocity(0);
+    o->alternation = keep;
+    std::memcpy(adv, o->advuvw.data(), sizeof(cx) * 3 * o->S);
+    return 0;
+  } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int tbo_get_history(const tbo_ctx* ctx, double* huvw, double* hc) {
+  const Oracle* o = ctx->o;
+  if (huvw) std::memcpy(huvw, o->huvw.data(), sizeof(cx) * 3 * o->S);
+  if (hc && o->cfg.nphase > 1) std::memcpy(hc, o->hphi.data(), sizeof(cx) * o->S);
+  return 0;
+}
+
 int tbo_get_operators(const tbo_ctx* ctx, double* gridz, double* d1, double* d2) {
   const Oracle* o = ctx->o;
   if (gridz) std::memcpy(gridz, o->gridz.data(), sizeof(double) * o->Nz);

@@ -68,6 +68,13 @@ void tbo_features(const tbo_ctx* ctx, double* out50);
 
 /* Physical velocity (uvwnonlinear, R*3 doubles) of the carrier. */
 int tbo_get_physical_velocity(const tbo_ctx* ctx, double* uvw_phys);
+/* FlatCarrierPhase::NonlinearVelocity of the current state in the
+ * convective (alternation 0) or divergence (1) form: adv = S*3 complex
+ * (phase_carrier_flat.cc:716-813).  The state is left unchanged. */
+int tbo_nonlinear(tbo_ctx* ctx, int alternation, double* adv);
+/* Explicit-term histories H of the carrier (S*3 complex) and of the scalar
+ * (S complex; ignored when nphase == 1) after the last substage. */
+int tbo_get_history(const tbo_ctx* ctx, double* huvw, double* hc);
 
 /* Operator tables built by the restatement (for pinning). */
 int tbo_get_operators(const tbo_ctx* ctx, double* gridz, double* d1, double* d2);

@@ -31,3 +31,45 @@ def spectral_split(state, cfg):
     """Flat RealCplxCube arrays -> (nv, mx, nyc, nz)."""
     _, nyc, mx = W.dims(cfg["nx"], cfg["ny"], cfg["nz"])
     return state.reshape(-1, mx, nyc, cfg["nz"])
+
+
+def manufactured_flow(cfg, amp=0.2, phase=(0.3, 0.7)):
+    """A divergence-free, band-limited velocity with closed-form u.grad(u).
+
+    u = curl(A) with A = (a1 h, a2 h, 0):
+        u = -a2 h',  v = a1 h',  w = (d_x a2 - d_y a1) h,
+    a1 = amp cos(al x + be y + f1), a2 = amp sin(al x) cos(be y + f2),
+    al = 2 pi / lx, be = 2 pi / ly, h = (1 - z^2)^2 (1 + z/2): every component
+    and h' vanish at both walls, the field is exactly divergence-free on the
+    collocation grid (D1 is exact on polynomials of degree < nz), and its
+    products stay inside the retained Fourier modes (|kx|, |ky| <= 2).
+
+    Returns (phys (3, nz, ny, nx), nonlinear (3, nz, ny, nx)) on the grid
+    x = lx/nx*i, y = ly/ny*j, z = CGL (bottom first), with
+    nonlinear_i = u_j d_j u_i from the analytic derivatives."""
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    al, be = 2 * math.pi / cfg["lx"], 2 * math.pi / cfg["ly"]
+    f1, f2 = phase
+    x = (cfg["lx"] / nx * np.arange(nx))[None, None, :]
+    y = (cfg["ly"] / ny * np.arange(ny))[None, :, None]
+    z = W.cgl_grid(nz)
+    hp = np.polynomial.Polynomial([1.0, 0.5]) * np.polynomial.Polynomial([1.0, 0.0, -1.0]) ** 2
+    h, h1, h2 = (q(z)[:, None, None] for q in (hp, hp.deriv(1), hp.deriv(2)))
+    t1 = al * x + be * y + f1
+    t2 = be * y + f2
+    a1, a1x, a1y = amp * np.cos(t1), -amp * al * np.sin(t1), -amp * be * np.sin(t1)
+    a1xy, a1yy = -amp * al * be * np.cos(t1), -amp * be * be * np.cos(t1)
+    a2 = amp * np.sin(al * x) * np.cos(t2)
+    a2x = amp * al * np.cos(al * x) * np.cos(t2)
+    a2y = -amp * be * np.sin(al * x) * np.sin(t2)
+    a2xx = -amp * al * al * np.sin(al * x) * np.cos(t2)
+    a2xy = -amp * al * be * np.cos(al * x) * np.sin(t2)
+    full = np.broadcast_to(np.zeros((nz, ny, nx)), (nz, ny, nx))
+    u, ux, uy, uz = -a2 * h1, -a2x * h1, -a2y * h1, -a2 * h2
+    v, vx, vy, vz = a1 * h1, a1x * h1, a1y * h1, a1 * h2
+    g = a2x - a1y
+    w, wx, wy, wz = g * h, (a2xx - a1xy) * h, (a2xy - a1yy) * h, g * h1
+    vel = [u + full, v + full, w + full]
+    grads = [(ux, uy, uz), (vx, vy, vz), (wx, wy, wz)]
+    nl = np.stack([vel[0] * gx + vel[1] * gy + vel[2] * gz for gx, gy, gz in grads])
+    return np.stack(vel), nl
