@@ -1289,7 +1289,20 @@ int tbo_step(tbo_ctx* ctx) {
   try { return ctx->o->Step(); } catch (const std::exception& e) { g_err = e.what(); return -1; }
 }
 
-void tbo_set_dt(tbo_ctx* ctx, double dt) { ctx->o->dt = dt; }
+// Time mode: the new dt, and Phase::Update's reassembly of the solves for it
+// (FlatCarrierPhase::Update, phase_carrier_flat.cc:250-265: u1, u2, u3 with
+// -Re/(dt c3), the pressure unchanged; FlatScalarPhase::Update,
+// phase_scalar_flat.cc:222-234: -Re Sc/(dt c3)).  Assemble rebuilds only the
+// reciprocals, which this restatement forms from sigma at each solve.
+void tbo_set_dt(tbo_ctx* ctx, double dt) {
+  Oracle& o = *ctx->o;
+  o.dt = dt;
+  for (int s = 0; s < 3; ++s) {
+    const double sv = -o.cfg.reynolds / dt / o.coef[2][s];
+    for (int k = 0; k < 3; ++k) o.dc[k].sigma[s] = sv;
+    if (o.cfg.nphase > 1) o.dc[4].sigma[s] = -o.cfg.reynolds * o.cfg.schmidt / dt / o.coef[2][s];
+  }
+}
 
 void tbo_stepper(const tbo_ctx* ctx, int* step, double* time, double* cfl, int* alternation) {
   if (step) *step = ctx->o->step;

@@ -318,6 +318,27 @@ def test_stokes_layer_analytic(top):
     assert np.abs(u.reshape(3, o.mx, o.nyc, o.nz)[1:, 0, 0]).max() < 1e-12
 
 
+def test_stokes_layer_across_a_dt_change():
+    # time mode hands a new dt to the phases, whose Update reassembles the
+    # Helmholtz solves for it (phase_carrier_flat.cc:250-265): after halving
+    # dt mid-run the mean mode stays on the analytic layer (a solve left at
+    # the old dt is a different diffusion operator and leaves it by 1e-3)
+    cfg = W.base_config(6, 6, 65, 4 * math.pi, 2 * math.pi, 180.0, W.TWO_PI / 2000.0, nphase=1)
+    uvw, p, _ = W.initial_state(cfg, noise_u=0.0)
+    o = O.Oracle(cfg)
+    o.set_state(uvw, p)
+    for _ in range(150):
+        o.step()
+    o.set_dt(cfg["dt"] / 2)
+    for _ in range(300):
+        o.step()
+    _, t, _, _ = o.stepper()
+    assert abs(t - 300 * cfg["dt"]) < 1e-12
+    u, _, _ = o.get_state()
+    U = u.reshape(3, o.mx, o.nyc, o.nz)[0, 0, 0].real
+    assert np.abs(U - W.stokes_profile(180.0, W.cgl_grid(65), t)).max() < 2e-6
+
+
 def test_stokes_layer_second_order_in_time():
     errs = []
     for ndt in (500, 1000):
