@@ -239,6 +239,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
     fft_.wide_yb = fft_.wide_yf = fft_.wide_xb = fft_.wide_xf = std::string(e) == "1";
     pin_wide_ = true;
   }
+  if (const char* e = std::getenv("TBD_FFT_SLAB_MB")) slab_mb_ = std::max(0, std::atoi(e));
   if (const char* e = std::getenv("TBD_FUSED")) {
     fused_ = std::string(e) == "1";
     pin_fused_ = true;
@@ -363,9 +364,11 @@ Context::~Context() {
 
 void Context::sync() { TBD_HIP(hipStreamSynchronize(st_)); }
 
+// Time mode: the next step runs with dt.  Captured steps are keyed by
+// (dt, previous dt), so none is dropped here; step() replays, captures or
+// runs eagerly as the pair demands.
 void Context::set_dt(double dt) {
   TBD_CHECK(dt > 0 && std::isfinite(dt), "Context::set_dt > dt must be positive and finite");
-  if (dt != dt_) drop_graphs();
   dt_ = dt;
 }
 
@@ -587,7 +590,15 @@ void Context::set_factorisation(int which, const double* V, const double* Vinv, 
   drop_graphs();
 }
 
+// Destroys the captured steps.  A replay may still be running on the
+// context's streams (a step returns before its graph has finished), so the
+// streams are drained first: an exec is never destroyed while in flight.
+// (Errors of the drain are left to the next checked call: this also runs in
+// the destructor.)
 void Context::drop_graphs() {
+  if (graphs_.empty()) return;
+  for (hipStream_t s : {st_, st2_, st3_})
+    if (s) (void)hipStreamSynchronize(s);
   for (auto& g : graphs_) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -599,9 +610,15 @@ void Context::drop_graphs() {
 // (alternation, wall-row slot, dt, previous dt) into a HIP graph and replayed;
 // the only per-step input, the mean-mode force, is written to device memory
 // before the launch.  The host-side state a step advances (wall-row slot,
-// previous dt) is advanced the same way as by the eager substages.
+// previous dt) is advanced the same way as by the eager substages.  A step
+// whose (dt, previous dt) differs from the last step's runs eagerly: in time
+// mode dt moves almost every step, and such a step's graph would not be
+// replayed.  At most kMaxGraphs captured steps are kept.
 void Context::step(double time, int alternation) {
-  if (!use_graphs_ || prof.on) {
+  const bool moved = last_dt_ >= 0 && (dt_ != last_dt_ || olddt_ != last_olddt_);
+  last_dt_ = dt_;
+  last_olddt_ = olddt_;
+  if (!use_graphs_ || prof.on || moved) {
     stages(time, alternation);
     return;
   }
@@ -617,6 +634,7 @@ void Context::step(double time, int alternation) {
       return;
     }
   }
+  if (graphs_.size() >= kMaxGraphs) drop_graphs();
   StepGraph g;
   g.alternation = alternation;
   g.slot = slot0;

@@ -3,6 +3,7 @@
 // src/phase_scalar_flat.cc), launched on one HIP stream.
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/turbid_amd.h"
@@ -155,18 +156,41 @@ class Context {
   template <typename T, typename SpecT>
   void ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol = -1,
            const int* gate = nullptr);
+  // A field-transform is its two passes through the x-pencil intermediate.
+  // With a slab budget (slab_mb_ > 0) the planes go in slabs whose
+  // intermediate takes at most that many MiB, each slab's second pass right
+  // after its first, so the intermediate is read back while it is still in
+  // the device's last-level cache.
+  template <typename T>
+  int slab_planes(int nv, int nplanes) const {
+    if (slab_mb_ <= 0) return nplanes;
+    const double per = (double)nv * d.mx * d.ny * sizeof(C2<T>);
+    const int ns = (int)std::max(1.0, (double)slab_mb_ * 1048576.0 / per);
+    return std::min(ns, nplanes);
+  }
   template <typename T, typename SpecT>
   void fwd(const T* phys, long long pstride, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
            C2<T>* tmp, int scol = -1, const int* gate = nullptr) {
-    xpf<T>(phys, pstride, tmp, nv, nplanes, gate);
-    ypf<T, SpecT>(tmp, spec, sstride, nv, zoff, nplanes, scol, gate);
+    const int ns = slab_planes<T>(nv, nplanes);
+    const long long plane = (long long)d.nx * d.ny;
+    for (int z0 = 0; z0 < nplanes; z0 += ns) {
+      const int n = std::min(ns, nplanes - z0);
+      xpf<T>(phys + z0 * plane, pstride, tmp, nv, n, gate);
+      ypf<T, SpecT>(tmp, spec, sstride, nv, zoff + z0, n, scol, gate);
+    }
   }
   template <typename T, typename SpecT>
   void bwd(const SpecT* spec, long long sstride, T* phys, long long pstride, int nv, int zoff, int nplanes,
            C2<T>* tmp, int scol = -1) {
-    ypb<T, SpecT>(spec, sstride, tmp, nv, zoff, nplanes, scol);
-    xpb<T>(tmp, phys, pstride, nv, nplanes);
+    const int ns = slab_planes<T>(nv, nplanes);
+    const long long plane = (long long)d.nx * d.ny;
+    for (int z0 = 0; z0 < nplanes; z0 += ns) {
+      const int n = std::min(ns, nplanes - z0);
+      ypb<T, SpecT>(spec, sstride, tmp, nv, zoff + z0, n, scol);
+      xpb<T>(tmp, phys + z0 * plane, pstride, nv, n);
+    }
   }
+  int slab_mb_ = 0;         // slab budget of the transforms (MiB; 0: whole field at once)
   void* alloc(size_t bytes);
   void build_factorisations();
   void install_operators();
@@ -272,6 +296,11 @@ class Context {
     hipGraphExec_t exec = nullptr;
   };
   std::vector<StepGraph> graphs_;
+  static constexpr size_t kMaxGraphs = 8;
+  // (dt, previous dt) of the last step: a step whose pair differs from the
+  // last one's runs eagerly (time mode moves dt almost every step, and a
+  // capture that is never replayed costs more than it saves)
+  double last_dt_ = -1, last_olddt_ = -1;
   bool use_graphs_ = true;
   bool capturing_ = false;
   double* force_ = nullptr;    // mean-mode force read by the captured H update
