@@ -23,8 +23,11 @@ substages), which gives each kernel's launch count, average duration and
 algorithmic work.  Roofline: the kernel with the largest total time;
 `achieved` = its algorithmic work per launch (bytes read plus written by a
 transform pass or product kernel, or 4*Nz^2*P flops per contraction of one
-component) over its average launch duration.  cpu_baseline: the FP64 CPU restatement (oracle/) timed on the
-host for one step of the same workload.
+component) over its average launch duration.  step_roofline: the whole
+step's time at the roofline (minimal algorithmic transform bytes and
+contraction flops per step) over the measured time.  cpu_baseline: the FP64
+CPU restatement (oracle/) timed on the host's cores for one warm-up step and
+then --cpu-steps steps of the same workload.
 """
 import argparse
 import json
@@ -72,7 +75,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run(cfg, state, steps, warmup, autotune):
+def run(cfg, state, steps, warmup, autotune, variant=None):
     """Pick the kernel variant (the library's autotuner, on a scratch copy of
     the context, outside the timed region), warm up, time `steps` steps as the
     library runs them (one captured HIP graph per step), then time the same
@@ -81,7 +84,10 @@ def run(cfg, state, steps, warmup, autotune):
     s = RT.Solver(cfg)
     s.upload(*state)
     tune = None
-    if autotune:
+    if variant is not None:
+        s.set_variant(variant)
+        tune = {"kept": {k: bool(s.variant() & b) for k, b in RT.Solver.VARIANT_BITS.items()}, "given": True}
+    elif autotune:
         try:
             bits, ms = s.autotune(5)
         except RT.TurbidError as e:   # e.g. no device memory for the scratch context
@@ -160,18 +166,53 @@ def class_rates(prof):
     return out
 
 
-def cpu_baseline(cfg, state, nthreads):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, state, nthreads, ksteps):
+    """The FP64 CPU restatement on the host: one warm-up step, then ksteps
+    timed steps of the same workload (wall clock)."""
     from oracle import oracle as O
     t0 = time.perf_counter()
     o = O.Oracle(cfg, nthreads=nthreads)
     o.set_state(*state)
     setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
     o.step()
+    t0 = time.perf_counter()
+    for _ in range(ksteps):
+        o.step()
     dt = time.perf_counter() - t0
     o.close()
-    return {"value": 1.0 / dt, "unit": "RK3 steps/s", "cores": nthreads, "kind": "port",
-            "sample": "1 RK3 step (3 substages, carrier + scalar) of the same workload after %.1f s setup" % setup}
+    return {"value": ksteps / dt, "unit": "RK3 steps/s", "cores": nthreads, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": "%d RK3 steps (3 substages, carrier + scalar) of the same workload after 1 warm-up step "
+                      "and %.1f s setup; baseline only" % (ksteps, setup)}
+
+
+def step_roofline(cfg, mixed, ms_per_step):
+    """Whole-step roofline with the minimal algorithmic counts per RK3 step:
+    58.5 field-transforms of w*R + 2*w*S bytes and 63 z-contractions of
+    4*Nz^2*P flops (mixed: FP32 transforms, 12 FP32 and 51 FP64
+    contractions); frac = T_roof / T_measured."""
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    nyc, mx = ny // 3 * 2, (nx // 3 * 2) // 2 + 1
+    P = mx * nyc
+    R, S = nx * ny * nz, nz * P
+    w = 4 if mixed else 8
+    t_fft = 58.5 * (w * R + 2 * w * S) / HBM_PEAK
+    zf = 4.0 * nz * nz * P
+    t_z = (51 * zf / MFMA_PEAK["fp64"] + 12 * zf / MFMA_PEAK["fp32"]) if mixed else 63 * zf / MFMA_PEAK["fp64"]
+    t_roof = 1e3 * (t_fft + t_z)
+    return {"t_roof_ms": t_roof, "t_fft_ms": 1e3 * t_fft, "t_z_ms": 1e3 * t_z, "ms_per_step": ms_per_step,
+            "frac": t_roof / ms_per_step}
 
 
 def copy_calibration():
@@ -200,6 +241,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="run the default kernel variant")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU-baseline steps after one warm-up step")
+    ap.add_argument("--variant", default=None,
+                    help="run this kernel variant (tbd_set_variant bits) instead of autotuning: one value for "
+                         "every precision, or FP64,MIXED")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -217,6 +262,10 @@ def main():
     log("bench: workload %s %dx%dx%d, building initial state" % (args.workload, cfg64["nx"], cfg64["ny"], cfg64["nz"]))
     uvw, p, c = W.initial_state(cfg64)
     state = (uvw, p, c)
+    variants = {}
+    if args.variant is not None:
+        v = [int(x) for x in str(args.variant).split(",")]
+        variants = {"fp64": v[0], "mixed": v[-1]}
     results = {}
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     for prec in precs:
@@ -224,7 +273,8 @@ def main():
         cfg["precision"] = 1 if prec == "mixed" else 0
         log("bench: %s warmup %d, timing %d steps (graph), then %d steps with per-launch events"
             % (prec, args.warmup, args.steps, args.steps))
-        t, tp, feat, prof, tune = run(cfg, state, args.steps, args.warmup, not args.no_autotune)
+        t, tp, feat, prof, tune = run(cfg, state, args.steps, args.warmup, not args.no_autotune,
+                                      variants.get(prec))
         if tune:
             log("bench: %s variant kept %s" % (prec, tune["kept"]))
         results[prec] = {"tune": tune, "seconds": t, "steps_per_s": args.steps / t, "ms_per_step": 1e3 * t / args.steps,
@@ -269,10 +319,14 @@ def main():
     # keyed by variant bits: 1 fused x transforms, 2 fused rhs + solve, 4 overlap,
     # 8 wide transform kernels)
     line["autotune"] = {prec: r["tune"] for prec, r in results.items()}
+    line["step_roofline"] = {prec: step_roofline(cfg64, prec == "mixed", r["ms_per_step"])
+                             for prec, r in results.items()}
     if rank == 0 and not args.no_cpu_baseline:
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("bench: cpu baseline (oracle, %d threads)" % nthreads)
-        line["cpu_baseline"] = cpu_baseline(cfg64, state, nthreads)
+        # all the host cores this process may use: the OpenMP thread count the
+        # host sets for it, else every CPU
+        nthreads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+        log("bench: cpu baseline (oracle, %d threads, %d steps after a warm-up step)" % (nthreads, args.cpu_steps))
+        line["cpu_baseline"] = cpu_baseline(cfg64, state, nthreads, args.cpu_steps)
     else:
         line["cpu_baseline"] = None
     if rank == 0:
