@@ -161,6 +161,19 @@ int tbd_set_variant(tbd_ctx* ctx, int bits);
  * fp32 != 0 runs the FP32 transform on FP64 data. */
 int tbd_dft_forward(int nx, int ny, int nz, int nv, int fp32, const double* phys, double* spec);
 int tbd_dft_backward(int nx, int ny, int nz, int nv, int fp32, const double* spec, double* phys);
+/* No counterpart in the reference.  tbd_dft_forward for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): phys (nv*Nx*Ny*Nz doubles) and spec
+ * (nv*Nz*P interleaved complex doubles, RealCplxCube order) are DEVICE
+ * pointers of the caller.  The twiddles, the transform intermediate and the
+ * FP32 staging live in one fixed device array of the library, 64 MiB per GPU,
+ * held from the library's first use on that GPU until the process ends; the
+ * z-planes are transformed in chunks that fit it.  A size whose single
+ * z-plane of nv components does not fit is rejected (error, nothing
+ * launched).  Runs on the null stream and returns when it has finished.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several host threads) race. */
+int tbd_dft_forward_noalloc(int nx, int ny, int nz, int nv, int fp32, const double* phys, double* spec);
 /* Y = M X for ncol complex columns of length nz, M nz*nz column-major
  * (RealCplxCube::Multiply(cmat, result), src/cube_real_cplx.cc:461-471). */
 int tbd_zcontract(int nz, long long ncol, int fp32, const double* m, const double* x, double* y);

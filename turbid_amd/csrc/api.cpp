@@ -16,6 +16,10 @@ struct tbd_ctx {
   tbd::Context* c;
 };
 
+namespace tbd {
+void dft_forward_noalloc(int nx, int ny, int nz, int nv, int fp32, const double* phys, double* spec);
+}
+
 namespace {
 thread_local std::string g_err;
 
@@ -245,6 +249,10 @@ int tbd_dft_forward(int nx, int ny, int nz, int nv, int fp32, const double* phys
     TBD_HIP(hipDeviceSynchronize());
     p.destroy();
   });
+}
+
+int tbd_dft_forward_noalloc(int nx, int ny, int nz, int nv, int fp32, const double* phys, double* spec) {
+  return guard([&] { tbd::dft_forward_noalloc(nx, ny, nz, nv, fp32, phys, spec); });
 }
 
 int tbd_dft_backward(int nx, int ny, int nz, int nv, int fp32, const double* spec, double* phys) {

@@ -37,7 +37,7 @@ EXPORTS = [
     "tbd_download_physical_velocity", "tbd_substage", "tbd_step", "tbd_set_dt", "tbd_features",
     "tbd_sync", "tbd_profile_enable", "tbd_profile_read", "tbd_profile_reset", "tbd_get_operators",
     "tbd_get_factorisation", "tbd_set_operators", "tbd_set_factorisation", "tbd_autotune", "tbd_get_variant",
-    "tbd_set_variant", "tbd_dft_forward", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
+    "tbd_set_variant", "tbd_dft_forward", "tbd_dft_forward_noalloc", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
     "tbd_project", "tbd_swirling_strength", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
     "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
     "tbd_bf16_gemm",
@@ -114,6 +114,7 @@ def lib():
         L.tbd_set_variant.argtypes = [vp, ctypes.c_int]
         L.tbd_dft_forward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
         L.tbd_dft_backward.argtypes = [ctypes.c_int] * 5 + [dp, dp]
+        L.tbd_dft_forward_noalloc.argtypes = [ctypes.c_int] * 5 + [vp, vp]
         L.tbd_zcontract.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, dp, dp, dp]
         L.tbd_helmholtz.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.tbd_project.argtypes = [vp]
@@ -161,6 +162,14 @@ def dft_forward(phys, fp32=False):
     out = np.zeros((nv, mx, nyc, nz), dtype=np.complex128)
     _check(lib().tbd_dft_forward(nx, ny, nz, nv, int(fp32), _p(src), _p(out.view(np.float64))))
     return out
+
+
+def dft_forward_noalloc(nx, ny, nz, nv, phys_ptr, spec_ptr, fp32=False):
+    """tbd_dft_forward_noalloc on device buffers: phys_ptr holds nv*nz*ny*nx
+    doubles, spec_ptr receives nv*mx*nyc*nz complex doubles (device
+    addresses the caller allocated)."""
+    _check(lib().tbd_dft_forward_noalloc(nx, ny, nz, nv, int(fp32), ctypes.c_void_p(phys_ptr),
+                                         ctypes.c_void_p(spec_ptr)))
 
 
 def dft_backward(spec, nx, ny, fp32=False):
