@@ -73,3 +73,45 @@ def manufactured_flow(cfg, amp=0.2, phase=(0.3, 0.7)):
     grads = [(ux, uy, uz), (vx, vy, vz), (wx, wy, wz)]
     nl = np.stack([vel[0] * gx + vel[1] * gy + vel[2] * gz for gx, gy, gz in grads])
     return np.stack(vel), nl
+
+
+def oracle_steps(cfg, state, nsteps, nthreads=0, progress=None, stepper=None):
+    """The oracle through FlatChannelSolver::Execute's loop (TimeStepper::Update
+    with the previous step's CFL feature, three substages, toggle) for nsteps
+    steps from `state`; returns (uvw, p, c).  progress(n) is called after
+    every step when given.  stepper: a runtime.Stepper to use instead of a
+    fresh step-mode one (a TimeModeStepper also sets the oracle's dt each
+    step, as Phase::Update does)."""
+    from oracle import oracle as O
+    from turbid_amd import runtime as RT
+    o = O.Oracle(cfg, nthreads=nthreads)
+    o.set_state(*state)
+    st = RT.Stepper(cfg["dt"]) if stepper is None else stepper
+    timed = isinstance(st, RT.TimeModeStepper)
+    feat = np.zeros(50)
+    for n in range(nsteps):
+        if not st.forward:
+            break
+        st.update(feat[19])
+        if timed:
+            o.set_dt(st.dt)
+        for s in range(3):
+            o.substage(s, st.time, st.alternation)
+        st.toggle()
+        feat = o.features()
+        if progress:
+            progress(n + 1)
+    out = o.get_state()
+    o.close()
+    return out
+
+
+def field_errors(got, ref):
+    """Relative L2 per field (u, v, w, p, c) of two (uvw, p, c) states."""
+    ug, pg, cg = got
+    uo, po, co = ref
+    ug, uo = np.asarray(ug).reshape(3, -1), np.asarray(uo).reshape(3, -1)
+    errs = {"u": rel_l2(ug[0], uo[0]), "v": rel_l2(ug[1], uo[1]), "w": rel_l2(ug[2], uo[2]), "p": rel_l2(pg, po)}
+    if co is not None:
+        errs["c"] = rel_l2(cg, co)
+    return errs

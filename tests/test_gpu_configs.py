@@ -1,0 +1,195 @@
+"""Every benchmark configuration on the GPU against the oracle.
+
+- 32 x 32 x 65 laminar oscillatory boundary layer (plain FP64): steps
+  against the oracle, and the mean mode against the analytic Stokes layer.
+- 256 x 256 x 257 turbulent boundary layer with fine sediment: 50 steps of
+  the seeded start against 50 oracle steps, per field, in both precisions
+  (the 1e-5 mixed-precision bound of the path is stated after 50 steps).
+- The same physics at 384 x 384 x 257: one step in both precisions.
+- Ensemble members drawn as the ensemble driver draws them ({64, 96, 128}^2
+  x {65, 97, 129}, Re_delta in [400, 1000], settling log-uniform in
+  [1e-4, 2e-3]): concurrent against alone, bit for bit, and against the
+  oracle.
+- The spin-up proposal network at its benchmark size (24 x 24 x 33 points,
+  32 channels, 4 layers, 7 inputs, 4 outputs): forward and gradient against
+  the FP64 restatement.
+
+The oracle runs on up to 16 host threads (about 3.5 s per 256^3 step)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import sno_oracle as S
+from turbid_amd import ensemble as E
+from turbid_amd import runtime as RT
+from turbid_amd import workloads as W
+
+from helpers import field_errors, oracle_steps, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu_steps(cfg, state, nsteps):
+    g = RT.Solver(cfg)
+    g.upload(*state)
+    RT.run_steps(g, RT.Stepper(cfg["dt"]), nsteps)
+    out = g.download()
+    g.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# 32 x 32 x 65 laminar Stokes layer
+# ---------------------------------------------------------------------------
+def test_laminar32_against_oracle():
+    cfg = W.workload("laminar32", precision=0)
+    state = W.initial_state(cfg)
+    ref = oracle_steps(cfg, state, 60, NTHREADS)
+    errs = field_errors(_gpu_steps(cfg, state, 60), ref)
+    _say("laminar32, 60 steps, relative L2 per field: %s" % errs)
+    assert max(errs.values()) < 1e-9, errs
+
+
+def test_laminar32_stokes_layer():
+    # the mean mode follows the analytic layer of a no-slip bottom and a
+    # free-slip top, U = i (cosh(k (z-1)) / cosh(2k) - 1), k = (1+i) sqrt(Re/2)
+    # (tmtools/get_loflow.m:17-19 with the top condition U'(1) = 0): the oracle
+    # stays within 0.9e-5 - 1.4e-5 of it over the first period at this
+    # resolution (start-up of the explicit history and the CN error)
+    cfg = W.workload("laminar32", precision=0)
+    z = W.cgl_grid(cfg["nz"])
+    _, nyc, mx = W.dims(cfg["nx"], cfg["ny"], cfg["nz"])
+    g = RT.Solver(cfg)
+    g.upload(*W.initial_state(cfg))
+    st = RT.Stepper(cfg["dt"])
+    worst = 0.0
+    for chunk in range(4):            # a quarter period each
+        RT.run_steps(g, st, 500)
+        u, _, _ = g.download()
+        U = u.reshape(3, mx, nyc, cfg["nz"])[0, 0, 0].real
+        ref = W.stokes_profile(cfg["reynolds"], z, st.time)
+        worst = max(worst, np.abs(U - ref).max())
+        # the mean flow across the forcing and through the walls stays at the
+        # level of the seeded noise's Reynolds stresses (amplitude 1e-4)
+        assert np.abs(u.reshape(3, mx, nyc, cfg["nz"])[1:, 0, 0]).max() < 1e-4
+    g.close()
+    _say("laminar32, one period: max |U - U_Stokes| = %.3e" % worst)
+    assert worst < 2e-5
+
+
+# ---------------------------------------------------------------------------
+# 256 x 256 x 257: 50 steps against the oracle
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def turb256_50():
+    cfg = W.workload("turb256", precision=0)
+    state = W.initial_state(cfg)
+
+    def progress(n):
+        if n % 10 == 0:
+            _say("oracle 256x256x257: step %d of 50" % n)
+
+    ref = oracle_steps(cfg, state, 50, NTHREADS, progress)
+    return cfg, state, ref
+
+
+@pytest.mark.parametrize("precision,tol,ptol", [(0, 1e-9, 1e-7), (1, 1e-5, 1e-5)])
+def test_turb256_50_steps(turb256_50, precision, tol, ptol):
+    # tolerances: FP64 as the single step at this size (p is the most
+    # rounding-sensitive field: it solves Laplace(p) = div(s)/cf far below the
+    # size of the terms of div(s)); mixed: the 1e-5 bound for every field
+    cfg, state, ref = turb256_50
+    c = dict(cfg)
+    c["precision"] = precision
+    errs = field_errors(_gpu_steps(c, state, 50), ref)
+    _say("256x256x257, precision %d, 50 steps, relative L2 per field: %s" % (precision, errs))
+    assert max(errs[k] for k in "uvwc") < tol, errs
+    assert errs["p"] < ptol, errs
+
+
+# ---------------------------------------------------------------------------
+# 384 x 384 x 257: one step in both precisions
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def turb384_1():
+    cfg = W.workload("turb384", precision=0)
+    state = W.initial_state(cfg)
+    return cfg, state, oracle_steps(cfg, state, 1, NTHREADS)
+
+
+@pytest.mark.parametrize("precision,tol,ptol", [(0, 1e-9, 1e-7), (1, 1e-5, 1e-5)])
+def test_turb384_step(turb384_1, precision, tol, ptol):
+    cfg, state, ref = turb384_1
+    c = dict(cfg)
+    c["precision"] = precision
+    errs = field_errors(_gpu_steps(c, state, 1), ref)
+    _say("384x384x257, precision %d, one step, relative L2 per field: %s" % (precision, errs))
+    assert max(errs[k] for k in "uvwc") < tol, errs
+    assert errs["p"] < ptol, errs
+
+
+# ---------------------------------------------------------------------------
+# ensemble members as the driver draws them
+# ---------------------------------------------------------------------------
+def test_drawn_members_concurrent_alone_and_oracle():
+    cfgs = E.member_configs(16, seed=0, precision=1)
+    grids = {(c["nx"], c["nz"]) for c in cfgs}
+    assert {n for n, _ in grids} <= {64, 96, 128} and {z for _, z in grids} <= {65, 97, 129}
+    nsteps = 3
+    ens = E.Ensemble(cfgs)
+    for _ in range(nsteps):
+        ens.step()
+    got = [s.download() for s in ens.solvers]
+    ens.close()
+    # every member against itself run alone
+    for m, (cfg, g) in enumerate(zip(cfgs, got)):
+        a = _gpu_steps(cfg, W.initial_state(cfg), nsteps)
+        for x, y in zip(g, a):
+            assert np.array_equal(x, y), m
+    # one member of each wall-normal size against the oracle (mixed: 1e-5)
+    picked = {}
+    for m, c in enumerate(cfgs):
+        picked.setdefault(c["nz"], m)
+    for nz, m in sorted(picked.items()):
+        cfg = dict(cfgs[m])
+        errs = field_errors(got[m], oracle_steps(cfg, W.initial_state(cfg), nsteps, NTHREADS))
+        _say("member %d %dx%dx%d Re_delta %.0f ws %.2e: %s"
+             % (m, cfg["nx"], cfg["ny"], nz, cfg["re_delta"], cfg["settling_velocity"], errs))
+        assert max(errs.values()) < 1e-5, (m, errs)
+
+
+# ---------------------------------------------------------------------------
+# spin-up proposal network at its benchmark size
+# ---------------------------------------------------------------------------
+def test_network_benchmark_size_forward_and_gradient():
+    nx, ny, nz, C, L, cin, cout = 24, 24, 33, 32, 4, 7, 4
+    net = S.Net(nx=nx, ny=ny, nz=nz, C=C, L=L, cin=cin, cout=cout)
+    dev = RT.SpectralOperatorNet(nx=nx, ny=ny, nz=nz, channels=C, layers=L, cin=cin, cout=cout, batch=2)
+    assert dev.nparams == net.lay.total
+    p = S.init_params(net.lay, seed=11)
+    dev.set_params(p)
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((2, dev.points, cin))
+    t = rng.standard_normal((2, dev.points, cout))
+    w = np.array([1.0, 1.0, 1.0, 0.5])
+    ef = rel_l2(dev.forward(x), net.forward(p, x))
+    loss_ref, g_ref = net.loss_grad(p, x, t, w)
+    loss = dev.loss_grad(x, t, w)
+    eg = rel_l2(dev.grads(), g_ref)
+    dev.close()
+    _say("network 24x24x33 C32 L4: forward %.3e, loss %.3e vs %.3e, gradient %.3e"
+         % (ef, loss, loss_ref, eg))
+    # BF16 operands (8 significant bits) with FP32 sums, against FP64
+    assert ef < 3e-2
+    assert abs(loss - loss_ref) < 3e-2 * loss_ref
+    assert eg < 6e-2

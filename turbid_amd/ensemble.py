@@ -15,7 +15,6 @@ alternation toggle).
 import argparse
 import json
 import math
-import os
 import sys
 import time
 
@@ -46,28 +45,28 @@ class Ensemble:
         stream.  Off by default: the members already overlap one another, and
         three streams per member over the process's few hardware queues
         serialise them (1593 vs 990 member-steps/s for the 16 members).
+        The switch is the overlap bit of each member's kernel variant
+        (tbd_set_variant), set per context.
         autotune: each member picks its own kernel variant (tbd_autotune,
-        run alone; the stream switch stays as member_overlap sets it)."""
+        run alone); the overlap bit is then set back as member_overlap asks."""
         self.cfgs = cfgs
         self.solvers = []
         self.steppers = []
         self.feats = []
-        saved = os.environ.get("TBD_NO_OVERLAP")
-        if not member_overlap:
-            os.environ["TBD_NO_OVERLAP"] = "1"   # read when a context is created
-        try:
-            for cfg in cfgs:
-                s = RT.Solver(cfg)
-                s.upload(*W.initial_state(cfg))
-                self.solvers.append(s)
-                self.steppers.append(RT.Stepper(cfg["dt"]))
-                self.feats.append(np.zeros(50))
-        finally:
-            if saved is None:
-                os.environ.pop("TBD_NO_OVERLAP", None)
-            else:
-                os.environ["TBD_NO_OVERLAP"] = saved
+        overlap = RT.Solver.VARIANT_BITS["overlap"]
+        for cfg in cfgs:
+            s = RT.Solver(cfg)
+            if not member_overlap:
+                s.set_variant(s.variant() & ~overlap)
+            s.upload(*W.initial_state(cfg))
+            self.solvers.append(s)
+            self.steppers.append(RT.Stepper(cfg["dt"]))
+            self.feats.append(np.zeros(50))
         self.variants = [s.autotune(3)[0] if autotune else s.variant() for s in self.solvers]
+        if autotune and not member_overlap:
+            for s in self.solvers:
+                s.set_variant(s.variant() & ~overlap)
+            self.variants = [s.variant() for s in self.solvers]
 
     def step(self, members=None):
         """One step of the given members (all by default), concurrently."""

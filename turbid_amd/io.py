@@ -88,8 +88,11 @@ def _fmt(v):
 
 
 def write_ini(path, cfg, max_step=100, write_interval=100, start_step=0, max_cfl=0.5, print_log=0, write_log=1,
-              work_index_range=None):
-    """configuration.ini in the reference's layout from a workload dict."""
+              work_index_range=None, time_mode=False, max_time=100000.0, output_interval=100.0,
+              start_time_index=-1, initial_dt=None):
+    """configuration.ini in the reference's layout from a workload dict.
+    time_mode: output_based_on_step = 0 (adaptive dt from initial_dt, output
+    every output_interval of time until max_time; TimeStepper::Update2)."""
     scalar = "ErosionDeposition" if cfg.get("scalar_bc", 1) == 1 else "ZeroFlux"
     lines = [
         "[System]",
@@ -114,15 +117,15 @@ def write_ini(path, cfg, max_step=100, write_interval=100, start_step=0, max_cfl
         "[Time Stepper]",
         "applied_stepper_type      = RungeKutta3",
         "maximum_cfl_number        = %s" % _fmt(float(max_cfl)),
-        "output_based_on_step      = 1",
+        "output_based_on_step      = %d" % (0 if time_mode else 1),
         "maximum_task_step         = %d" % max_step,
         "result_write_interval     = %d" % write_interval,
         "marching_time_step        = %s" % _fmt(float(cfg["dt"])),
         "task_start_step           = %d" % start_step,
-        "max_marching_time         = 100000",
-        "output_interval           = 100",
-        "task_start_time_index     = -1",
-        "initial_time_step         = %s" % _fmt(float(cfg["dt"])),
+        "max_marching_time         = %s" % _fmt(float(max_time)),
+        "output_interval           = %s" % _fmt(float(output_interval)),
+        "task_start_time_index     = %d" % start_time_index,
+        "initial_time_step         = %s" % _fmt(float(cfg["dt"] if initial_dt is None else initial_dt)),
         "cfl_adjust_scale          = 0.95 0.9",
         "result_write_initial      = 0",
         "[Phase Carrier]",

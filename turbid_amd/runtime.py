@@ -397,13 +397,65 @@ class Stepper:
         self.alternation = not self.alternation
 
 
+class TimeModeStepper(Stepper):
+    """TimeStepper mirror in time mode (output_based_on_step = 0; reference
+    src/stepper_time.cc:120-208 Update2 and :215-246 Configure): dt adapts
+    to the previous step's CFL (times top**k when above top*maxcfl, times
+    1.05 when below bot*maxcfl, (top, bot) = cfl_adjust_scale) and is cut
+    so that the run lands on every output time; the run stops at
+    max_marching_time, or when the CFL exceeds 1.1*maxcfl or dt falls below
+    initial_time_step/1000."""
+
+    def __init__(self, initial_dt, max_time, output_interval, max_cfl, adjust=(0.95, 0.9), start_time_index=0):
+        super().__init__(initial_dt, max_cfl=max_cfl)
+        self.step = 0
+        self.initial_dt = initial_dt
+        self.max_time = max_time
+        self.output_interval = output_interval
+        self.adjust = adjust
+        self.output_index = start_time_index
+        self.time = start_time_index * output_interval if start_time_index > 0 else 0.0
+        self.forward = self.time <= max_time
+        self.output = False
+
+    def update(self, maxulen):
+        if not self.forward:
+            return
+        self.step += 1
+        self.time = self.time + self.dt
+        self.forward = self.time < self.max_time
+        self.cfl = maxulen * self.dt
+        if self.cfl > 0:
+            top, bot = self.adjust
+            if self.cfl > top * self.max_cfl:
+                self.dt = self.dt * top
+                while maxulen * self.dt > top * self.max_cfl:
+                    self.dt = self.dt * top
+            elif self.cfl < bot * self.max_cfl:
+                self.dt = self.dt * 1.05
+            if (self.time + self.dt) / self.output_interval - self.output_index - 1 >= 0:
+                dttmp = (self.output_index + 1) * self.output_interval - self.time
+                self.dt = self.dt / 20 if dttmp < self.initial_dt / 100 else dttmp
+                self.output_index += 1
+                self.output = True
+            else:
+                self.output = False
+        if self.cfl > self.max_cfl * 1.1 or self.dt < self.initial_dt / 1000:
+            self.forward = False
+            self.output = True
+
+
 def run_steps(solver, stepper, nsteps):
-    """FlatChannelSolver::Execute's loop body, nsteps times (src/solver_channel_flat.cc:199-254)."""
+    """FlatChannelSolver::Execute's loop body, nsteps times (src/solver_channel_flat.cc:199-254);
+    a TimeModeStepper also hands its new dt to the solver (Phase::Update)."""
     feat = np.zeros(50)
+    timed = isinstance(stepper, TimeModeStepper)
     for _ in range(nsteps):
         if not stepper.forward:
             break
         stepper.update(feat[19])
+        if timed:
+            solver.set_dt(stepper.dt)
         solver.step(stepper.time, stepper.alternation)
         stepper.toggle()
         feat = solver.features()
