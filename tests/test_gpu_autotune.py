@@ -35,7 +35,10 @@ def test_every_variant_steps_like_the_default(precision, tol):
     cfg = small_config(precision=precision)
     ref = _steps(_solver(cfg, DEFAULT), cfg, 6)
     wide = [RT.Solver.VARIANT_BITS[k] for k in ("wide_yb", "wide_yf", "wide_xb", "wide_xf")]
-    for bits in list(range(8)) + [DEFAULT | w for w in wide] + [DEFAULT | RT.Solver.WIDE_FFT]:
+    swap = [RT.Solver.VARIANT_BITS[k] for k in ("swap_yb", "swap_yf", "swap_xb", "swap_xf")]
+    # 24 x 24: both lengths have a swapped radix order (4 x 6)
+    for bits in (list(range(8)) + [DEFAULT | w for w in wide] + [DEFAULT | RT.Solver.WIDE_FFT]
+                 + [DEFAULT | w for w in swap] + [DEFAULT | RT.Solver.WIDE_FFT | RT.Solver.SWAP_FFT]):
         g = _solver(cfg, bits)
         assert g.variant() == bits
         got = _steps(g, cfg, 6)
@@ -53,8 +56,9 @@ def test_autotune_keeps_state_and_the_fastest_variant():
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
     tried = [i for i in range(RT.Solver.N_VARIANTS) if not math.isnan(ms[i])]
-    # the eight step-level combinations, then each of the four widths
-    assert len(tried) == 12
+    # the eight step-level combinations, then each of the four widths and
+    # each of the four radix orders (24 x 24 has both)
+    assert len(tried) == 16
     assert all(i in tried for i in range(8))
     assert all(ms[i] > 0 for i in tried)
     assert bits == min(tried, key=lambda i: ms[i])
@@ -77,11 +81,20 @@ def test_environment_pins_a_switch(monkeypatch):
     tried = [i for i in range(RT.Solver.N_VARIANTS) if not math.isnan(ms[i])]
     assert not any(i & 1 for i in tried)
     # the four admissible step-level combinations (at the widths set above),
-    # then the four widths
-    assert {i & 7 for i in tried} == {0, 2, 4, 6} and len(tried) == 8
+    # then the four widths and the four radix orders
+    assert {i & 7 for i in tried} == {0, 2, 4, 6} and len(tried) == 12
 
 
 def test_set_variant_rejects_bad_bits():
     g = _solver(small_config())
     with pytest.raises(RT.TurbidError, match="out of range"):
         g.set_variant(RT.Solver.N_VARIANTS)
+
+
+def test_radix_order_bit_clear_where_no_swap():
+    # 32 x 32: the length 32 has the swapped order 4 x 8; 16 (4 x 4) has none
+    cfg = small_config(nx=16, ny=32, nz=17)
+    g = _solver(cfg)
+    g.set_variant(DEFAULT | RT.Solver.SWAP_FFT)
+    b = RT.Solver.VARIANT_BITS
+    assert g.variant() == DEFAULT | b["swap_yb"] | b["swap_yf"]

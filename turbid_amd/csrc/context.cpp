@@ -239,7 +239,6 @@ Context::Context(const tbd_config& c) : cfg(c) {
     fft_.wide_yb = fft_.wide_yf = fft_.wide_xb = fft_.wide_xf = std::string(e) == "1";
     pin_wide_ = true;
   }
-  if (const char* e = std::getenv("TBD_FFT_SLAB_MB")) slab_mb_ = std::max(0, std::atoi(e));
   if (const char* e = std::getenv("TBD_FUSED")) {
     fused_ = std::string(e) == "1";
     pin_fused_ = true;
@@ -483,6 +482,10 @@ Context::Variant Context::variant() const {
   v.wide[1] = fft_.wide_yf;
   v.wide[2] = fft_.wide_xb;
   v.wide[3] = fft_.wide_xf;
+  v.swap[0] = fft_.swap_yb;
+  v.swap[1] = fft_.swap_yf;
+  v.swap[2] = fft_.swap_xb;
+  v.swap[3] = fft_.swap_xf;
   return v;
 }
 
@@ -497,6 +500,12 @@ void Context::set_variant(const Variant& v) {
     fft_.wide_xb = v.wide[2];
     fft_.wide_xf = v.wide[3];
   }
+  // a swapped radix order exists only for some lengths; elsewhere the bit
+  // has no kernel to select and stays clear
+  fft_.swap_yb = v.swap[0] && fft_has_swap(d.ny);
+  fft_.swap_yf = v.swap[1] && fft_has_swap(d.ny);
+  fft_.swap_xb = v.swap[2] && fft_has_swap(d.nx);
+  fft_.swap_xf = v.swap[3] && fft_has_swap(d.nx);
   fuse_rhs_ = parity_ && d.nz % 2 == 1 && d.nz >= 7 && want_rhs_;
   fuse_p_ = fuse_rhs_ && hsplit_[3];
   drop_graphs();
@@ -524,8 +533,10 @@ int Context::autotune(int steps, double* ms8) {
       return false;
     for (int i = 0; i < 4; ++i) {
       if (pin_wide_ && v.wide[i] != cur.wide[i]) return false;
-      // the wide kernels exist only for the two-level transform family
+      // the wide and swapped kernels exist only for the two-level transform
+      // family, the swapped ones for lengths with two distinct factors
       if (v.wide[i] && !fft_.two_level) return false;
+      if (v.swap[i] && (!fft_.two_level || !fft_has_swap(i < 2 ? d.ny : d.nx))) return false;
     }
     // the fused right-hand side needs the even/odd operators; without them
     // both settings run the same kernels
@@ -558,13 +569,22 @@ int Context::autotune(int steps, double* ms8) {
   // every combination of the three step-level switches at the current widths
   for (int low = 0; low < 8; ++low) {
     Variant v = Variant::of(low);
-    for (int i = 0; i < 4; ++i) v.wide[i] = cur.wide[i];
+    for (int i = 0; i < 4; ++i) {
+      v.wide[i] = cur.wide[i];
+      v.swap[i] = cur.swap[i];
+    }
     if (admissible(v)) time_variant(v);
   }
   // then each transform width in turn, from the best so far
   for (int i = 0; i < 4; ++i) {
     Variant v = Variant::of(best);
     v.wide[i] = !v.wide[i];
+    if (admissible(v)) time_variant(v);
+  }
+  // then each pass's radix order
+  for (int i = 0; i < 4; ++i) {
+    Variant v = Variant::of(best);
+    v.swap[i] = !v.swap[i];
     if (admissible(v)) time_variant(v);
   }
   set_variant(Variant::of(best));

@@ -89,16 +89,20 @@ class Context {
   //   wide[4] two-level transform kernels of the y backward, y forward,
   //           x backward and x forward pass with twice the default workgroup
   //           size (twice the sequences per workgroup)
+  //   swap[4] the same four passes with the two factors of the length in
+  //           the other order (the radix split N2 x N1; lengths with two
+  //           distinct factors only)
   // A switch set in the environment at creation (TBD_FUSED, TBD_RHS_UNFUSED,
   // TBD_NO_OVERLAP; TBD_FFT_WIDE for the four widths) is pinned: set_variant
   // leaves it and autotune does not vary it.
-  static constexpr int kVariants = 128;
+  static constexpr int kVariants = 2048;
   struct Variant {
     bool fused = false, rhs = true, overlap = true;
     bool wide[4] = {false, false, false, false};
+    bool swap[4] = {false, false, false, false};
     int bits() const {
       int b = (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0);
-      for (int i = 0; i < 4; ++i) b |= wide[i] ? 8 << i : 0;
+      for (int i = 0; i < 4; ++i) b |= (wide[i] ? 8 << i : 0) | (swap[i] ? 128 << i : 0);
       return b;
     }
     static Variant of(int bits) {
@@ -106,7 +110,10 @@ class Context {
       v.fused = (bits & 1) != 0;
       v.rhs = (bits & 2) != 0;
       v.overlap = (bits & 4) != 0;
-      for (int i = 0; i < 4; ++i) v.wide[i] = (bits & (8 << i)) != 0;
+      for (int i = 0; i < 4; ++i) {
+        v.wide[i] = (bits & (8 << i)) != 0;
+        v.swap[i] = (bits & (128 << i)) != 0;
+      }
       return v;
     }
   };
@@ -115,8 +122,8 @@ class Context {
   // Times variants on a scratch context (same configuration, operators and
   // dt; this context's state uploaded to it before each) for `steps`
   // captured steps after three warm-up steps: every combination of fused,
-  // rhs and overlap, then each transform width in turn from the best so far
-  // (kept when faster), and keeps the fastest.
+  // rhs and overlap, then each transform width and then each radix order in
+  // turn from the best so far (kept when faster), and keeps the fastest.
   // ms[bits] = ms per step (NaN where not tried, kVariants entries); returns
   // the kept bits.
   int autotune(int steps, double* ms);
@@ -156,41 +163,18 @@ class Context {
   template <typename T, typename SpecT>
   void ypf(const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes, int scol = -1,
            const int* gate = nullptr);
-  // A field-transform is its two passes through the x-pencil intermediate.
-  // With a slab budget (slab_mb_ > 0) the planes go in slabs whose
-  // intermediate takes at most that many MiB, each slab's second pass right
-  // after its first, so the intermediate is read back while it is still in
-  // the device's last-level cache.
-  template <typename T>
-  int slab_planes(int nv, int nplanes) const {
-    if (slab_mb_ <= 0) return nplanes;
-    const double per = (double)nv * d.mx * d.ny * sizeof(C2<T>);
-    const int ns = (int)std::max(1.0, (double)slab_mb_ * 1048576.0 / per);
-    return std::min(ns, nplanes);
-  }
   template <typename T, typename SpecT>
   void fwd(const T* phys, long long pstride, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
            C2<T>* tmp, int scol = -1, const int* gate = nullptr) {
-    const int ns = slab_planes<T>(nv, nplanes);
-    const long long plane = (long long)d.nx * d.ny;
-    for (int z0 = 0; z0 < nplanes; z0 += ns) {
-      const int n = std::min(ns, nplanes - z0);
-      xpf<T>(phys + z0 * plane, pstride, tmp, nv, n, gate);
-      ypf<T, SpecT>(tmp, spec, sstride, nv, zoff + z0, n, scol, gate);
-    }
+    xpf<T>(phys, pstride, tmp, nv, nplanes, gate);
+    ypf<T, SpecT>(tmp, spec, sstride, nv, zoff, nplanes, scol, gate);
   }
   template <typename T, typename SpecT>
   void bwd(const SpecT* spec, long long sstride, T* phys, long long pstride, int nv, int zoff, int nplanes,
            C2<T>* tmp, int scol = -1) {
-    const int ns = slab_planes<T>(nv, nplanes);
-    const long long plane = (long long)d.nx * d.ny;
-    for (int z0 = 0; z0 < nplanes; z0 += ns) {
-      const int n = std::min(ns, nplanes - z0);
-      ypb<T, SpecT>(spec, sstride, tmp, nv, zoff + z0, n, scol);
-      xpb<T>(tmp, phys + z0 * plane, pstride, nv, n);
-    }
+    ypb<T, SpecT>(spec, sstride, tmp, nv, zoff, nplanes, scol);
+    xpb<T>(tmp, phys, pstride, nv, nplanes);
   }
-  int slab_mb_ = 0;         // slab budget of the transforms (MiB; 0: whole field at once)
   void* alloc(size_t bytes);
   void build_factorisations();
   void install_operators();

@@ -37,9 +37,16 @@ struct FftPlan {
   // forward) with twice the default workgroup size (twice the sequences per
   // workgroup); chosen per grid and pass by the autotuner
   bool wide_yb = false, wide_yf = false, wide_xb = false, wide_xf = false;
+  // the same passes with the two factors of the length in the other order
+  // (N2 x N1: level A the smaller DFT), for lengths with two distinct factors
+  // (fft_has_swap); chosen per grid and pass by the autotuner
+  bool swap_yb = false, swap_yf = false, swap_xb = false, swap_xf = false;
   void create(const Dims& dims);
   void destroy();
 };
+
+// Whether a two-level length has the swapped factor order (FftPlan::swap_*).
+bool fft_has_swap(int n);
 
 // Size of the x-pencil intermediate (complex elements) for nv components.
 inline long long fft_tmp_elems(const Dims& d, int nv) { return (long long)nv * d.nz * d.mx * d.ny; }

@@ -489,88 +489,99 @@ template <typename T> struct Nt2;
 template <> struct Nt2<double> { static constexpr int v = 128; };
 template <> struct Nt2<float> { static constexpr int v = 256; };
 
-// Supported two-level lengths; F(N1, N2) is expanded per length.
+// Supported two-level lengths; F(N, N1, N2) is expanded per length.
 #define TBD_FOR_LENGTHS(F)                                                                              \
   F(384, 24, 16) F(256, 16, 16) F(192, 16, 12) F(128, 16, 8) F(96, 12, 8) F(64, 8, 8) F(48, 8, 6)    \
   F(32, 8, 4) F(24, 6, 4) F(16, 4, 4) F(12, 4, 3) F(8, 4, 2) F(6, 3, 2) F(4, 4, 1) F(3, 3, 1) F(2, 2, 1)
+// The same lengths with the two factors in the other order (level A the
+// smaller one), an autotuner choice per pass (FftPlan::swap_*)
+#define TBD_FOR_SWAPPED(F)                                                                              \
+  F(384, 16, 24) F(192, 12, 16) F(128, 8, 16) F(96, 8, 12) F(48, 6, 8) F(32, 4, 8) F(24, 4, 6)       \
+  F(12, 3, 4) F(8, 2, 4) F(6, 2, 3)
 
+bool fft_has_swap(int n) {
+  switch (n) {
+#define TBD_CASE(NN, A, B) case NN: return true;
+    TBD_FOR_SWAPPED(TBD_CASE)
+#undef TBD_CASE
+    default:
+      return false;
+  }
+}
+
+template <typename T, typename SpecT, int NT, int A, int B>
+static void ypf2_launch(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff,
+                        int nplanes, hipStream_t st, int sc, T scale, const int* gate) {
+  constexpr int S = seqs2<A, B, NT>();
+  dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);
+  hipLaunchKernelGGL((k_ypf2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, tmp, spec, sstride, twy<T>(p), sc,
+                     p.d.mx, p.d.nyc, nplanes, zoff, scale, gate);
+}
 template <typename T, typename SpecT, int NT>
 static bool ypf2_nt(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
                  hipStream_t st, int scol, const int* gate) {
   const T scale = (T)(1.0 / ((double)p.d.nx * (double)p.d.ny));
   const int sc = scol > 0 ? scol : p.d.nz;
-  switch (p.d.ny) {
-#define TBD_CASE(NN, A, B)                                                                                  \
-  case NN: {                                                                                                \
-    constexpr int S = seqs2<A, B, NT>();                                                                    \
-    dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);                                                            \
-    hipLaunchKernelGGL((k_ypf2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, tmp, spec, sstride, twy<T>(p), sc, \
-                       p.d.mx, p.d.nyc, nplanes, zoff, scale, gate);                                        \
-    return true;                                                                                            \
-  }
-    TBD_FOR_LENGTHS(TBD_CASE)
+#define TBD_CASE(NN, A, B) \
+  case NN: ypf2_launch<T, SpecT, NT, A, B>(p, tmp, spec, sstride, nv, zoff, nplanes, st, sc, scale, gate); return true;
+  if (p.swap_yf) switch (p.d.ny) { TBD_FOR_SWAPPED(TBD_CASE) default: break; }
+  switch (p.d.ny) { TBD_FOR_LENGTHS(TBD_CASE) default: return false; }
 #undef TBD_CASE
-    default:
-      return false;
-  }
 }
 
+template <typename T, typename SpecT, int NT, int A, int B>
+static void ypb2_launch(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff,
+                        int nplanes, hipStream_t st, int sc) {
+  constexpr int S = seqs2<A, B, NT>();
+  dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);
+  hipLaunchKernelGGL((k_ypb2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, spec, sstride, tmp, twy<T>(p), sc,
+                     p.d.mx, p.d.nyc, nplanes, zoff);
+}
 template <typename T, typename SpecT, int NT>
 static bool ypb2_nt(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
                  hipStream_t st, int scol) {
   const int sc = scol > 0 ? scol : p.d.nz;
-  switch (p.d.ny) {
-#define TBD_CASE(NN, A, B)                                                                                  \
-  case NN: {                                                                                                \
-    constexpr int S = seqs2<A, B, NT>();                                                                    \
-    dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);                                                            \
-    hipLaunchKernelGGL((k_ypb2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, spec, sstride, tmp, twy<T>(p), sc, \
-                       p.d.mx, p.d.nyc, nplanes, zoff);                                                     \
-    return true;                                                                                            \
-  }
-    TBD_FOR_LENGTHS(TBD_CASE)
+#define TBD_CASE(NN, A, B) \
+  case NN: ypb2_launch<T, SpecT, NT, A, B>(p, spec, sstride, tmp, nv, zoff, nplanes, st, sc); return true;
+  if (p.swap_yb) switch (p.d.ny) { TBD_FOR_SWAPPED(TBD_CASE) default: break; }
+  switch (p.d.ny) { TBD_FOR_LENGTHS(TBD_CASE) default: return false; }
 #undef TBD_CASE
-    default:
-      return false;
-  }
 }
 
+template <typename T, int NT, int A, int B>
+static void xpf2_launch(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
+                        hipStream_t st, const int* gate) {
+  constexpr int S = seqs2<A, B, NT>();
+  dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);
+  hipLaunchKernelGGL((k_xpf2<T, A, B, NT>), grid, dim3(NT), 0, st, phys, pstride, tmp, twx<T>(p), p.d.ny, p.d.mx,
+                     nplanes, gate);
+}
 template <typename T, int NT>
 static bool xpf2_nt(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
                  hipStream_t st, const int* gate) {
-  switch (p.d.nx) {
-#define TBD_CASE(NN, A, B)                                                                              \
-  case NN: {                                                                                            \
-    constexpr int S = seqs2<A, B, NT>();                                                                \
-    dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);                                                   \
-    hipLaunchKernelGGL((k_xpf2<T, A, B, NT>), grid, dim3(NT), 0, st, phys, pstride, tmp, twx<T>(p), p.d.ny, \
-                       p.d.mx, nplanes, gate);                                                          \
-    return true;                                                                                        \
-  }
-    TBD_FOR_LENGTHS(TBD_CASE)
+#define TBD_CASE(NN, A, B) \
+  case NN: xpf2_launch<T, NT, A, B>(p, phys, pstride, tmp, nv, nplanes, st, gate); return true;
+  if (p.swap_xf) switch (p.d.nx) { TBD_FOR_SWAPPED(TBD_CASE) default: break; }
+  switch (p.d.nx) { TBD_FOR_LENGTHS(TBD_CASE) default: return false; }
 #undef TBD_CASE
-    default:
-      return false;
-  }
 }
 
+template <typename T, int NT, int A, int B>
+static void xpb2_launch(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
+                        hipStream_t st) {
+  constexpr int S = seqs2<A, B, NT>();
+  dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);
+  hipLaunchKernelGGL((k_xpb2<T, A, B, NT>), grid, dim3(NT), 0, st, tmp, phys, pstride, twx<T>(p), p.d.ny, p.d.mx,
+                     nplanes);
+}
 template <typename T, int NT>
 static bool xpb2_nt(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
                  hipStream_t st) {
-  switch (p.d.nx) {
-#define TBD_CASE(NN, A, B)                                                                              \
-  case NN: {                                                                                            \
-    constexpr int S = seqs2<A, B, NT>();                                                                \
-    dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);                                                   \
-    hipLaunchKernelGGL((k_xpb2<T, A, B, NT>), grid, dim3(NT), 0, st, tmp, phys, pstride, twx<T>(p), p.d.ny, \
-                       p.d.mx, nplanes);                                                                \
-    return true;                                                                                        \
-  }
-    TBD_FOR_LENGTHS(TBD_CASE)
+#define TBD_CASE(NN, A, B) \
+  case NN: xpb2_launch<T, NT, A, B>(p, tmp, phys, pstride, nv, nplanes, st); return true;
+  if (p.swap_xb) switch (p.d.nx) { TBD_FOR_SWAPPED(TBD_CASE) default: break; }
+  switch (p.d.nx) { TBD_FOR_LENGTHS(TBD_CASE) default: return false; }
 #undef TBD_CASE
-    default:
-      return false;
-  }
 }
 
 // The two-level kernels at the default workgroup size (Nt2: 16 sequences of
