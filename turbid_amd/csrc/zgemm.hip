@@ -1024,11 +1024,27 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
     }
   });
   __syncthreads();
-  gemm_panel<double, DEPTH>(a.d2p, a.ldp, pd, xs, [&](int row, int c, double2 v) { xs[c * ldx + row] = v; });
+  // D2 x through the even/odd halves (s rows 0..M, d rows M+1..), each slot
+  // emitted as v - q x_slot with q = s1 + kx^2 + ky^2 while the slot still
+  // holds its half of x (the buffered contraction emits after its last
+  // panel read, and every slot is read and written by the thread that emits
+  // it).  Then for the row pair (j, nz-1-j):
+  //   s1 x - ((D2 x - kx^2 x) - ky^2 x) = -(s'_j +- d'_j)
+  // and the right-hand side needs no second read of x.
+  gemm_panel<double, DEPTH>(a.d2p, a.ldp, pd, xs, [&](int row, int c, double2 v) {
+    if (row > 2 * M) {        // the split's last index holds no half of x
+      xs[c * ldx + row] = v;
+      return;
+    }
+    const double kx = kv[c].x, ky = kv[c].y;
+    const double q = a.s1 + kx * kx + ky * ky;
+    const double2 xv = xs[c * ldx + row];
+    xs[c * ldx + row] = make_double2(v.x - q * xv.x, v.y - q * xv.y);
+  });
   __syncthreads();
   // rhs per row pair (j, nz-1-j); rows 0 and nz-1 are the boundary rows, the
   // interior ones get the forward fix; written back to the two slots read
-  struct In { double2 xl, xh, hl, hh, al, ah, cl, ch; };
+  struct In { double2 hl, hh, al, ah, cl, ch; };
   auto adv_at = [&](long long e) -> double2 {
     if (a.adv32) {
       const float2 v = reinterpret_cast<const float2*>(F.adv)[e];
@@ -1058,14 +1074,10 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   };
   batched<2, In>(pd.ncol * (M + 1), [&](int idx) {
     const int c = idx / (M + 1), j = idx - c * (M + 1);
-    In r = {zero, zero, zero, zero, zero, zero, zero, zero};
+    In r = {zero, zero, zero, zero, zero, zero};
     if (c < ncol) {
       const long long o = (c0 + c) * nz;
       const bool upd = a.hupd != 0;
-      if (j > 0) {
-        r.xl = x[o + j];
-        if (j < M) r.xh = x[o + nz - 1 - j];
-      }
       if (j > 0 || upd) {
         r.hl = h[o + j];
         if (j < M) r.hh = h[o + nz - 1 - j];
@@ -1095,20 +1107,14 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
       }
     }
     if (j > 0) {
-      const long long col = c0 + c;
-      const double kx = kv[c].x, ky = kv[c].y;
-      const double k2x = -(kx * kx), k2y = -(ky * ky);
-      const double2 sv = xs[c * ldx + j], dv = xs[c * ldx + M + 1 + j];
-      auto rhs = [&](double2 v, double2 xv, double2 hv) {
-        const double2 lap = cadd<double>(cadd<double>(v, cscale<double>(k2x, xv)), cscale<double>(k2y, xv));
-        const double2 t = csub<double>(cadd<double>(cscale<double>(a.s2, hv), cscale<double>(a.s1, xv)), lap);
-        return t;
-      };
+      const double2 sv = xs[c * ldx + j];
+      const double2 dv = j < M ? xs[c * ldx + M + 1 + j] : zero;
+      auto rhs = [&](double2 v, double2 hv) { return csub<double>(cscale<double>(a.s2, hv), v); };
       auto fixed = [&](int i, double2 v) {
         return cadd<double>(v, cadd<double>(cscale<double>(fixs[i], bot), cscale<double>(fixs[nz + i], top)));
       };
-      lo = fixed(j, rhs(cadd<double>(sv, dv), r.xl, r.hl));
-      if (j < M) hi = fixed(nz - 1 - j, rhs(csub<double>(sv, dv), r.xh, r.hh));
+      lo = fixed(j, rhs(cadd<double>(sv, dv), r.hl));
+      if (j < M) hi = fixed(nz - 1 - j, rhs(csub<double>(sv, dv), r.hh));
     }
     if (j == M) {
       xs[c * ldx + M] = lo;
