@@ -25,7 +25,10 @@ algorithmic work.  Roofline: the kernel with the largest total time;
 transform pass or product kernel, or 4*Nz^2*P flops per contraction of one
 component) over its average launch duration.  step_roofline: the whole
 step's time at the roofline (minimal algorithmic transform bytes and
-contraction flops per step) over the measured time.  cpu_baseline: the FP64
+contraction flops per step) over the measured time.  spinup_gpu_s: the
+metric's second half, GPU seconds to stationary TKE and wall shear from the
+plain noisy-laminar start and from the learned proposal on a held-out seed
+(turbid_amd.spinup).  cpu_baseline: the FP64
 CPU restatement (oracle/) timed on the host's cores for one warm-up step and
 then --cpu-steps steps of the same workload.
 """
@@ -215,6 +218,35 @@ def step_roofline(cfg, mixed, ms_per_step):
             "frac": t_roof / ms_per_step}
 
 
+SPINUP_MODEL = os.path.join(ROOT, "turbid_amd", "spinup_model.npz")
+
+
+def spinup_leg(path, periods, noises):
+    """Second half of the metric: GPU seconds to stationary statistics (period
+    means of TKE and wall shear in band for two consecutive wave periods)
+    from the plain start and from the learned proposal (turbid_amd.spinup),
+    on the spin-up workload and a held-out seed, with the model trained
+    earlier on this device (its one-time data-generation and training
+    seconds come with it).  One entry per noise amplitude of the plain
+    start; the first is the headline (plain, surrogate)."""
+    from turbid_amd import spinup as SP
+    cfg = W.workload("spinup128", precision=1)
+    model, rec = SP.load_model(cfg, path)
+    starts = {}
+    for noise in noises:
+        res = SP.evaluate(cfg, model, rec["reference"], periods=periods, noise=noise,
+                          log_fn=lambda m: log("bench: " + m))
+        s = SP.summary(res, rec)
+        s["periods_run"] = {k: v["periods_run"] for k, v in res.items()}
+        starts[str(noise)] = s
+    head = starts[str(noises[0])]
+    return dict(plain=head["plain"], surrogate=head["surrogate"], plain_lower_bound=head["plain_lower_bound"],
+                ratio=head.get("ratio"), one_time_s=head["one_time_s"], break_even_runs=head.get("break_even_runs"),
+                starts=starts, periods_cap=periods, held_out_seed=7, data_seed=rec["seed_data"],
+                snapshots=rec["snapshots"], tolerance=SP.TOL,
+                workload="spinup128: Re_delta=1000, box 2x2x2, 128x128x129, dt=T/12000, mixed precision")
+
+
 def copy_calibration():
     """One plain device-to-device copy of 2^30 bytes after a warm-up copy: the
     FETCH_SIZE / WRITE_SIZE counters of its dispatch read 2^30 bytes each."""
@@ -245,6 +277,11 @@ def main():
     ap.add_argument("--variant", default=None,
                     help="run this kernel variant (tbd_set_variant bits) instead of autotuning: one value for "
                          "every precision, or FP64,MIXED")
+    ap.add_argument("--spinup-model", default=SPINUP_MODEL, help="trained proposal (python -m turbid_amd.spinup build)")
+    ap.add_argument("--spinup-periods", type=int, default=4, help="cap on each spin-up run; 0 skips the leg")
+    ap.add_argument("--spinup-noise", type=float, nargs="+", default=[0.01, 0.1],
+                    help="noise amplitudes of the plain starts (the first is the headline): the weakly "
+                         "perturbed laminar layer, and the start of the 256^3 workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -321,6 +358,12 @@ def main():
     line["autotune"] = {prec: r["tune"] for prec, r in results.items()}
     line["step_roofline"] = {prec: step_roofline(cfg64, prec == "mixed", r["ms_per_step"])
                              for prec, r in results.items()}
+    if args.spinup_periods > 0 and os.path.exists(args.spinup_model):
+        log("bench: spin-up, plain start vs proposal (noise %s, at most %d periods each)"
+            % (args.spinup_noise, args.spinup_periods))
+        line["spinup_gpu_s"] = spinup_leg(args.spinup_model, args.spinup_periods, args.spinup_noise)
+    else:
+        line["spinup_gpu_s"] = None
     if rank == 0 and not args.no_cpu_baseline:
         # all the host cores this process may use: the OpenMP thread count the
         # host sets for it, else every CPU

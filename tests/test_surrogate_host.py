@@ -58,3 +58,39 @@ def test_laminar_grid_and_input():
     x = SG.network_input(lam, 0.7)
     assert x.shape == (33 * 24 * 24, SG.CIN)
     np.testing.assert_allclose(x[:, SG.NFIELD], math.cos(0.7))
+
+
+def _hist(tke, tau, gpu_per_bin=1.0):
+    """Per-bin statistics of a run whose period means are tke[i], tau[i]."""
+    from turbid_amd import spinup as SP
+    out, t = [], 0.0
+    for a, b in zip(tke, tau):
+        for _ in range(SP.PHASE_BINS):
+            t += gpu_per_bin
+            out.append(dict(tke=a, tau=b, cmean=1.0, gpu_s=t))
+    return out
+
+
+def test_time_to_reference_needs_consecutive_periods_in_band():
+    from turbid_amd import spinup as SP
+    ref = dict(tke=1.0, tau=10.0)
+    # period 2 is in band, period 3 falls out, periods 4-5 are in band
+    h = _hist([0.1, 0.98, 0.5, 1.02, 0.99], [8.0, 10.1, 10.0, 9.9, 10.0])
+    gpu, per = SP.time_to_reference(h, ref, hold=2)
+    assert per == 4 and gpu == 4 * SP.PHASE_BINS
+    # the wall shear alone out of band keeps a period out
+    h = _hist([1.0, 1.0, 1.0], [10.0, 12.0, 10.0])
+    assert SP.time_to_reference(h, ref, hold=2) == (None, None)
+    assert SP.time_to_reference(h, ref, hold=1)[1] == 1
+
+
+def test_summary_break_even():
+    from turbid_amd import spinup as SP
+    rec = dict(data_gpu_s=100.0, train_s=50.0)
+    res = dict(plain=dict(gpu_s=40.0, gpu_s_run=60.0), proposal=dict(gpu_s=10.0, gpu_s_run=20.0))
+    s = SP.summary(res, rec)
+    assert s["ratio"] == 0.25 and s["break_even_runs"] == 5.0 and s["one_time_s"] == 150.0
+    # a plain start that never reaches the band: its run time is a lower bound
+    res["plain"]["gpu_s"] = None
+    s = SP.summary(res, rec)
+    assert not s["plain_reached"] and s["plain_lower_bound"] == 60.0 and s["ratio"] == 10.0 / 60.0

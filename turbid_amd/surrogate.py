@@ -152,6 +152,10 @@ class Proposal:
         self.mean = np.zeros(NFIELD)
         self.std = np.ones(NFIELD)
         self.noise_rms = 0.1
+        # developed-flow TKE by wave phase (12 bins from phase 0); when set,
+        # a proposal's fluctuations are scaled to it (see propose)
+        self.tke_phase = None
+        self.tke_lo_phase = None    # the part of it inside the reduced band
         self.net.set_params(init_params(self.net.nparams, grid, channels, layers, seed))
 
     def _norm_in(self, x):
@@ -209,7 +213,13 @@ class Proposal:
 
     def propose_grid(self, vals_in, phase):
         lam = laminar_grid(self.cfg, phase, self.grid)
-        x = self._norm_in(network_input(vals_in - lam, phase, self.grid)).astype(np.float32)[None]
+        # the start's deviation from the laminar state, scaled to the size the
+        # training inputs had (any noise amplitude of the plain start)
+        dev = vals_in - lam
+        rms = math.sqrt(float(np.mean(dev[..., :3] ** 2)))
+        if rms > 0:
+            dev = dev * (self.noise_rms / rms)
+        x = self._norm_in(network_input(dev, phase, self.grid)).astype(np.float32)[None]
         y = self.net.forward(x)[0]
         return lam + (y * self.std + self.mean).reshape(vals_in.shape)
 
@@ -224,12 +234,38 @@ class Proposal:
         out[-1, :, :, 2] = 0.0
         # the proposal holds the reduced modes only: the start's noise above
         # them is not part of a developed state
-        return expand_state(out, mx, nyc, nz, self.grid)
+        uvw_p, c_p = expand_state(out, mx, nyc, nz, self.grid)
+        if self.tke_phase is not None:
+            # A least-squares fit predicts the conditional mean of the
+            # fluctuations, with less energy than any one realisation, and
+            # the reduced grid holds none of the energy above its band.  The
+            # band's fluctuations (every mode but the mean) are scaled to the
+            # developed flow's band energy at this phase, and the modes above
+            # the band are the start's own, scaled to the developed flow's
+            # energy there (random phases, the start's wall window).
+            k = int(round(phase / (2 * math.pi / len(self.tke_phase)))) % len(self.tke_phase)
+            lo = self.tke_lo_phase[k] if self.tke_lo_phase is not None else self.tke_phase[k]
+            e_lo = fluctuation_energy(uvw_p)
+            if e_lo > 0:
+                mean = uvw_p[:, 0, 0, :].copy()
+                uvw_p *= math.sqrt(lo / e_lo)
+                uvw_p[:, 0, 0, :] = mean
+            if self.tke_lo_phase is not None:
+                hi = self.tke_phase[k] - lo
+                out_band = ~band_mask(mx, nyc, self.grid)
+                small = np.where(out_band[None, :, :, None], uvw, 0)
+                e_hi = fluctuation_energy(small)
+                if e_hi > 0 and hi > 0:
+                    uvw_p = uvw_p + small * math.sqrt(hi / e_hi)
+        return uvw_p, c_p
 
-    def save(self, path):
+    def save(self, path, record=None):
+        """Parameters and normalisation, plus `record` (JSON-able: how the
+        model was made) in the metadata."""
         np.savez(path, params=self.net.params(), mean=self.mean, std=self.std,
                  meta=json.dumps(dict(grid=self.grid, channels=self.channels, layers=self.layers,
-                                      noise_rms=self.noise_rms)))
+                                      noise_rms=self.noise_rms, tke_phase=self.tke_phase,
+                                      tke_lo_phase=self.tke_lo_phase, record=record)))
 
     def load(self, path):
         d = np.load(path, allow_pickle=False)
@@ -238,6 +274,8 @@ class Proposal:
         self.std = d["std"]
         meta = json.loads(str(d["meta"]))
         self.noise_rms = meta.get("noise_rms", self.noise_rms)
+        self.tke_phase = meta.get("tke_phase")
+        self.tke_lo_phase = meta.get("tke_lo_phase")
 
 
 def init_params(n, grid, C, L, seed):
@@ -270,13 +308,34 @@ def init_params(n, grid, C, L, seed):
 # ---------------------------------------------------------------------------
 # statistics used to assess a start
 # ---------------------------------------------------------------------------
+def fluctuation_energy(uvw):
+    """Turbulent kinetic energy of the non-mean modes of a spectral velocity
+    (Parseval on the retained modes, kx > 0 counted twice), averaged over the
+    wall-normal points."""
+    fl = uvw.copy()
+    fl[:, 0, 0, :] = 0
+    wgt = np.ones(uvw.shape[1])
+    wgt[1:] = 2.0
+    return 0.5 * float(np.sum(wgt[None, :, None, None] * np.abs(fl) ** 2)) / uvw.shape[-1]
+
+
+def band_mask(mx, nyc, grid=GRID):
+    """(mx, nyc) mask of the DNS modes the reduced grid holds (|kx|, |ky| <=
+    kmax)."""
+    kmax = min(grid["nx"], grid["ny"]) // 2 - 1
+    m = np.zeros((mx, nyc), dtype=bool)
+    m[:kmax + 1, _ky_index(nyc, kmax)] = True
+    return m
+
+
+def band_energy(uvw, grid=GRID):
+    """fluctuation_energy of the modes inside the reduced band."""
+    m = band_mask(uvw.shape[1], uvw.shape[2], grid)
+    return fluctuation_energy(np.where(m[None, :, :, None], uvw, 0))
+
+
 def flow_stats(uvw, c, feat):
     """Wall shear of the mean mode (dU/dz, dV/dz at the bottom), turbulent
     kinetic energy of the non-mean modes, mean concentration."""
-    fl = uvw.copy()
-    fl[:, 0, 0, :] = 0
-    # Parseval on the retained modes (kx > 0 counted twice)
-    wgt = np.ones(uvw.shape[1])
-    wgt[1:] = 2.0
-    tke = 0.5 * float(np.sum(wgt[None, :, None, None] * np.abs(fl) ** 2)) / uvw.shape[-1]
-    return dict(tau=float(math.hypot(feat[10], feat[11])), tke=tke, cmean=float(np.real(c[0, 0, 0, :]).mean()))
+    return dict(tau=float(math.hypot(feat[10], feat[11])), tke=fluctuation_energy(uvw),
+                tke_lo=band_energy(uvw), cmean=float(np.real(c[0, 0, 0, :]).mean()))

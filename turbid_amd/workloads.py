@@ -71,14 +71,15 @@ def workload(name, precision=0):
                         settling=5e-4, schmidt=0.5, precision=precision)
         c.update(noise_u=0.1, noise_c=0.01, seed=1)
         return c
-    if name == "spinup128":       # Re_delta = 800 physics of the reference's own example
+    if name == "spinup128":       # box and wave of the reference's Re_delta = 800 example
         # (examples/oscillatory_flow_re800/result/configuration.ini:41,96-97: box 2 x 2 x 2,
-        # Re = 24000, omega = 0.075) at 128x128x129; dt = T/12000 keeps u dt/dx near 0.5
+        # omega = 0.075) at Re_delta = 1000 (Re = 37500): on 128x128x129 the layer stays
+        # turbulent at 1000 and relaminarises at 800.  dt = T/12000 keeps u dt/dx near 0.5;
+        # zero-flux walls for the sediment (its mass is conserved)
         omega = 0.0535237234845831 * 2 * math.pi / 10.0 / 0.448399297311834
-        re = 0.448399297311834 * 0.0535237234845831 * 1e6
-        c = base_config(128, 128, 129, 2.0, 2.0, re, 2 * math.pi / omega / 12000.0, omega=omega,
-                        settling=5e-4, schmidt=0.5, precision=precision)
-        c.update(noise_u=0.1, noise_c=0.01, seed=1)
+        c = base_config(128, 128, 129, 2.0, 2.0, reynolds_from_stokes(1000.0, omega), 2 * math.pi / omega / 12000.0,
+                        omega=omega, settling=5e-4, schmidt=0.5, precision=precision)
+        c.update(noise_u=0.1, noise_c=0.01, seed=1, scalar_bc=0)
         return c
     if name == "train128":        # 128x128x129 surrogate-training resolution
         c = base_config(128, 128, 129, 20.0, 10.0, reynolds_from_stokes(800.0), dt,
