@@ -278,10 +278,10 @@ def main():
                     help="run this kernel variant (tbd_set_variant bits) instead of autotuning: one value for "
                          "every precision, or FP64,MIXED")
     ap.add_argument("--spinup-model", default=SPINUP_MODEL, help="trained proposal (python -m turbid_amd.spinup build)")
-    ap.add_argument("--spinup-periods", type=int, default=4, help="cap on each spin-up run; 0 skips the leg")
-    ap.add_argument("--spinup-noise", type=float, nargs="+", default=[0.01, 0.1],
-                    help="noise amplitudes of the plain starts (the first is the headline): the weakly "
-                         "perturbed laminar layer, and the start of the 256^3 workload")
+    ap.add_argument("--spinup-periods", type=int, default=6, help="cap on each spin-up run; 0 skips the leg")
+    ap.add_argument("--spinup-noise", type=float, nargs="+", default=[0.01],
+                    help="noise amplitudes of the plain starts (the first is the headline): 0.01, the weakly "
+                         "perturbed laminar layer; 0.1, the start of the 256^3 workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (reference for HBM byte counters)")
     args = ap.parse_args()

@@ -2,6 +2,7 @@
 start, accumulated on the device (tbd_stats_*), and their difference.
 
     python -m turbid_amd.phasestats --workload turb256 --periods 10 --bins 12
+    python -m turbid_amd.phasestats --workload spinup128 --skip 2 --periods 10
 
 For each precision the run starts from the same seeded state, advances
 `periods` wave periods with the reference's step loop, and at the end of
@@ -27,7 +28,7 @@ from . import workloads as W
 NAMES = ["U", "V", "W", "C", "uu", "vv", "ww", "cc"]
 
 
-def run(cfg, periods, bins, log):
+def run(cfg, periods, bins, log, skip=0):
     s = RT.Solver(cfg)
     s.upload(*W.initial_state(cfg))
     s.stats_reset()
@@ -36,6 +37,10 @@ def run(cfg, periods, bins, log):
     per_bin = spp // bins
     tau = np.zeros((bins, 2))
     t0 = time.perf_counter()
+    for p in range(skip):     # spin-up periods, not averaged
+        RT.run_steps(s, st, per_bin * bins)
+        log("phasestats: precision %d spin-up period %d done (%.1f s)" % (cfg["precision"], p + 1,
+                                                                         time.perf_counter() - t0))
     for p in range(periods):
         for b in range(bins):
             feat = RT.run_steps(s, st, per_bin)
@@ -52,13 +57,14 @@ def main(argv=None):
     ap.add_argument("--workload", default="turb256")
     ap.add_argument("--periods", type=int, default=10)
     ap.add_argument("--bins", type=int, default=12)
+    ap.add_argument("--skip", type=int, default=0, help="periods run before the averaging starts")
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     res = {}
     for prec in (0, 1):
         cfg = W.workload(args.workload, precision=prec)
-        res[prec] = run(cfg, args.periods, args.bins, log)
+        res[prec] = run(cfg, args.periods, args.bins, log, args.skip)
     p64, t64, s64 = res[0]
     p32, t32, s32 = res[1]
     diff = {}
@@ -70,7 +76,8 @@ def main(argv=None):
         diff[n] = float(np.max(np.abs(p32[:, i, :] - p64[:, i, :])) / scales[i]) if scales[i] > 0 else 0.0
     scale = np.max(np.abs(t64))
     diff["wall_shear"] = float(np.max(np.abs(t32 - t64)) / scale)
-    out = dict(workload=args.workload, periods=args.periods, bins=args.bins, seconds_fp64=s64, seconds_mixed=s32,
+    out = dict(workload=args.workload, periods=args.periods, skip=args.skip, bins=args.bins, seconds_fp64=s64,
+               seconds_mixed=s32,
                max_relative_difference=diff, max_over_quantities=max(diff.values()))
     line = json.dumps(out)
     if args.out:
