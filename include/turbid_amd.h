@@ -146,14 +146,16 @@ int tbd_set_factorisation(tbd_ctx* ctx, int which, const double* V, const double
  * twice their default workgroup size, 128 / 256 / 512 / 1024 the same passes
  * with the two factors of their length in the other order (radix split
  * N2 x N1; lengths with two distinct factors, e.g. 24, 96, 128, 384, else the
- * bit stays clear).  A switch set in the environment when the context was
+ * bit stays clear), 2048 the plain contractions (gradients, D1 of the
+ * nonlinear terms) on panels of twice the column groups where the LDS allows.
+ * A switch set in the environment when the context was
  * created (TBD_FUSED, TBD_RHS_UNFUSED, TBD_NO_OVERLAP; TBD_FFT_WIDE for the
  * four widths) is pinned.  tbd_autotune times every combination of bits 1, 2,
- * 4, then each width bit and then each radix-order bit in turn from the best
- * so far, for `steps` captured steps (after three warm-up steps) on a scratch
+ * 4, then each width bit, each radix-order bit and the panel-width bit in turn
+ * from the best so far, for `steps` captured steps (after three warm-up steps) on a scratch
  * context with the same configuration, operators, dt and current state (so it
  * needs the device memory of a second context; this context's state is not
- * changed) and keeps the fastest; ms_per_step[2048] (optional) receives each
+ * changed) and keeps the fastest; ms_per_step[4096] (optional) receives each
  * variant's time, NaN where not tried, and *bits (optional) the variant
  * kept. */
 int tbd_autotune(tbd_ctx* ctx, int steps, double* ms_per_step, int* bits);

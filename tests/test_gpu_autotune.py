@@ -38,7 +38,8 @@ def test_every_variant_steps_like_the_default(precision, tol):
     swap = [RT.Solver.VARIANT_BITS[k] for k in ("swap_yb", "swap_yf", "swap_xb", "swap_xf")]
     # 24 x 24: both lengths have a swapped radix order (4 x 6)
     for bits in (list(range(8)) + [DEFAULT | w for w in wide] + [DEFAULT | RT.Solver.WIDE_FFT]
-                 + [DEFAULT | w for w in swap] + [DEFAULT | RT.Solver.WIDE_FFT | RT.Solver.SWAP_FFT]):
+                 + [DEFAULT | w for w in swap] + [DEFAULT | RT.Solver.WIDE_FFT | RT.Solver.SWAP_FFT]
+                 + [DEFAULT | RT.Solver.VARIANT_BITS["wide_zgemm"], 1 | RT.Solver.VARIANT_BITS["wide_zgemm"]]):
         g = _solver(cfg, bits)
         assert g.variant() == bits
         got = _steps(g, cfg, 6)
@@ -56,9 +57,10 @@ def test_autotune_keeps_state_and_the_fastest_variant():
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
     tried = [i for i in range(RT.Solver.N_VARIANTS) if not math.isnan(ms[i])]
-    # the eight step-level combinations, then each of the four widths and
-    # each of the four radix orders (24 x 24 has both)
-    assert len(tried) == 16
+    # the eight step-level combinations, then each of the four widths, each
+    # of the four radix orders (24 x 24 has both) and the contraction panel
+    # width
+    assert len(tried) == 17
     assert all(i in tried for i in range(8))
     assert all(ms[i] > 0 for i in tried)
     assert bits == min(tried, key=lambda i: ms[i])
@@ -81,8 +83,8 @@ def test_environment_pins_a_switch(monkeypatch):
     tried = [i for i in range(RT.Solver.N_VARIANTS) if not math.isnan(ms[i])]
     assert not any(i & 1 for i in tried)
     # the four admissible step-level combinations (at the widths set above),
-    # then the four widths and the four radix orders
-    assert {i & 7 for i in tried} == {0, 2, 4, 6} and len(tried) == 12
+    # then the four widths, the four radix orders and the panel width
+    assert {i & 7 for i in tried} == {0, 2, 4, 6} and len(tried) == 13
 
 
 def test_set_variant_rejects_bad_bits():

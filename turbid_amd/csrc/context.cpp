@@ -171,15 +171,18 @@ void Context::solve(int mode, int n, const int* k, const double* sigma, const do
 }
 
 template <typename T, typename SpecT>
-void Context::ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol) {
-  const double b = (double)nv * nplanes * d.mx * (d.nyc * sizeof(SpecT) + (double)d.ny * sizeof(C2<T>));
+void Context::ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol,
+                  const int* gate) {
+  // a gated launch may do nothing: its work is not counted
+  const double b = gate ? 0.0 : (double)nv * nplanes * d.mx * (d.nyc * sizeof(SpecT) + (double)d.ny * sizeof(C2<T>));
   launch(is32<T>() ? KC_YB32 : KC_YB64, b,
-         [&] { fft_ypass_backward<T, SpecT>(fft_, spec, sstride, tmp, nv, zoff, nplanes, st_, scol); });
+         [&] { fft_ypass_backward<T, SpecT>(fft_, spec, sstride, tmp, nv, zoff, nplanes, st_, scol, gate); });
 }
 template <typename T>
-void Context::xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes) {
-  const double b = (double)nv * nplanes * d.ny * (d.mx * sizeof(C2<T>) + (double)d.nx * sizeof(T));
-  launch(is32<T>() ? KC_XB32 : KC_XB64, b, [&] { fft_xpass_backward<T>(fft_, tmp, phys, pstride, nv, nplanes, st_); });
+void Context::xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes, const int* gate) {
+  const double b = gate ? 0.0 : (double)nv * nplanes * d.ny * (d.mx * sizeof(C2<T>) + (double)d.nx * sizeof(T));
+  launch(is32<T>() ? KC_XB32 : KC_XB64, b,
+         [&] { fft_xpass_backward<T>(fft_, tmp, phys, pstride, nv, nplanes, st_, gate); });
 }
 template <typename T>
 void Context::xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes, const int* gate) {
@@ -339,6 +342,8 @@ Context::Context(const tbd_config& c) : cfg(c) {
   diagpart_ = (double*)alloc(sizeof(double) * 10 * kDiagMaxBlocks);
   force_ = (double*)alloc(sizeof(double) * 2);
   filt_ = (int*)alloc(sizeof(int) * 2);
+  need_ = (int*)alloc(sizeof(int) * 2);
+  mmpart_ = (double*)alloc(sizeof(double) * 1024);
   stats_ = (double*)alloc(sizeof(double) * kStatBins * kPhaseStats * d.nz);
   if (const char* e = std::getenv("TBD_GRAPH")) use_graphs_ = std::string(e) != "0";
   TBD_HIP(hipStreamSynchronize(st_));
@@ -486,6 +491,7 @@ Context::Variant Context::variant() const {
   v.swap[1] = fft_.swap_yf;
   v.swap[2] = fft_.swap_xb;
   v.swap[3] = fft_.swap_xf;
+  v.zwide = zwide_;
   return v;
 }
 
@@ -506,6 +512,7 @@ void Context::set_variant(const Variant& v) {
   fft_.swap_yf = v.swap[1] && fft_has_swap(d.ny);
   fft_.swap_xb = v.swap[2] && fft_has_swap(d.nx);
   fft_.swap_xf = v.swap[3] && fft_has_swap(d.nx);
+  zwide_ = v.zwide;
   fuse_rhs_ = parity_ && d.nz % 2 == 1 && d.nz >= 7 && want_rhs_;
   fuse_p_ = fuse_rhs_ && hsplit_[3];
   drop_graphs();
@@ -586,6 +593,12 @@ int Context::autotune(int steps, double* ms8) {
     Variant v = Variant::of(best);
     v.swap[i] = !v.swap[i];
     if (admissible(v)) time_variant(v);
+  }
+  // then the panel width of the plain contractions
+  {
+    Variant v = Variant::of(best);
+    v.zwide = !v.zwide;
+    time_variant(v);
   }
   set_variant(Variant::of(best));
   return variant().bits();
@@ -765,6 +778,7 @@ void Context::forces(double time, double& f1, double& f2) const {
 template <typename T>
 void Context::set_d1(ZArgs& a) const {
   const bool f = std::is_same<T, float>::value;
+  a.ncg_mult = zwide_ ? 2 : 1;
   if (parity_) {
     a.op = f ? (const void*)d1pf_ : (const void*)d1p_;
     a.lda = ldp_;
@@ -776,6 +790,7 @@ void Context::set_d1(ZArgs& a) const {
   }
 }
 void Context::set_d2(ZArgs& a) const {
+  a.ncg_mult = zwide_ ? 2 : 1;
   a.op = parity_ ? d2p_ : d2_;
   a.lda = parity_ ? ldp_ : lda_;
   a.parity = parity_ ? 1 : 0;
@@ -1101,13 +1116,18 @@ void Context::scalar_correct(int stage, int alternation) {
       solve(0, 1, &k, &sigma, &x, &y);
     }
   }
-  // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478)
-  bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)stmp_);
-  // the filter raises filt_ when it changes a plane; otherwise the transform
-  // back would reproduce c except for the kx = 0 plane's projection, which is
-  // applied directly
+  // Bartnicki positivity filter: physical c, 5 plane iterations, back (:419-478).
+  // The filter leaves a field without negative values unchanged, so an FP32
+  // transform of c and a min/max check first decide whether the exact FP64
+  // transform and filter have anything to do (need_); the filter raises
+  // filt_ when it changes a plane; otherwise the transform back would
+  // reproduce c except for the kx = 0 plane's projection, which is applied
+  // directly
+  bwd<float, double2>(c_, S, (float*)saphys_, R, 1, 0, d.nz, (float2*)stmp_);
+  launch(KC_OTHER, 0.0, [&] { negativity_check<float>((const float*)saphys_, R, mmpart_, need_, st_); });
+  bwd<double, double2>(c_, S, cphys_, R, 1, 0, d.nz, (double2*)stmp_, -1, need_);
   fill_zero(filt_, sizeof(int) * 2, st_);
-  launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, filt_, st_); });
+  launch(KC_OTHER, 0.0, [&] { bartnicki_filter(cphys_, d.nx, d.ny, d.nz, filt_, st_, need_); });
   fwd<double, double2>(cphys_, R, c_, S, 1, 0, d.nz, (double2*)stmp_, -1, filt_);
   launch(KC_OTHER, 0.0, [&] { kx0_round_trip(c_, d.nyc, d.nz, filt_, st_); });
   launch(KC_OTHER, 0.0, [&] { trapz_mean(c_, gridz_, d.nz, fl_ + 21, st_); });

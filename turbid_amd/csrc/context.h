@@ -92,16 +92,20 @@ class Context {
   //   swap[4] the same four passes with the two factors of the length in
   //           the other order (the radix split N2 x N1; lengths with two
   //           distinct factors only)
+  //   zwide   plain contractions (gradients, D1 of the nonlinear terms,
+  //           unfused rhs and projection) with twice the column groups per
+  //           panel where the LDS allows
   // A switch set in the environment at creation (TBD_FUSED, TBD_RHS_UNFUSED,
   // TBD_NO_OVERLAP; TBD_FFT_WIDE for the four widths) is pinned: set_variant
   // leaves it and autotune does not vary it.
-  static constexpr int kVariants = 2048;
+  static constexpr int kVariants = 4096;
   struct Variant {
     bool fused = false, rhs = true, overlap = true;
     bool wide[4] = {false, false, false, false};
     bool swap[4] = {false, false, false, false};
+    bool zwide = false;
     int bits() const {
-      int b = (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0);
+      int b = (fused ? 1 : 0) | (rhs ? 2 : 0) | (overlap ? 4 : 0) | (zwide ? 2048 : 0);
       for (int i = 0; i < 4; ++i) b |= (wide[i] ? 8 << i : 0) | (swap[i] ? 128 << i : 0);
       return b;
     }
@@ -110,6 +114,7 @@ class Context {
       v.fused = (bits & 1) != 0;
       v.rhs = (bits & 2) != 0;
       v.overlap = (bits & 4) != 0;
+      v.zwide = (bits & 2048) != 0;
       for (int i = 0; i < 4; ++i) {
         v.wide[i] = (bits & (8 << i)) != 0;
         v.swap[i] = (bits & (128 << i)) != 0;
@@ -122,8 +127,9 @@ class Context {
   // Times variants on a scratch context (same configuration, operators and
   // dt; this context's state uploaded to it before each) for `steps`
   // captured steps after three warm-up steps: every combination of fused,
-  // rhs and overlap, then each transform width and then each radix order in
-  // turn from the best so far (kept when faster), and keeps the fastest.
+  // rhs and overlap, then each transform width, each radix order and the
+  // contraction panel width in turn from the best so far (kept when
+  // faster), and keeps the fastest.
   // ms[bits] = ms per step (NaN where not tried, kVariants entries); returns
   // the kept bits.
   int autotune(int steps, double* ms);
@@ -155,9 +161,10 @@ class Context {
   void solve(int mode, int n, const int* k, const double* sigma, const double2* const* x, double2* const* y);
   // profiled transform passes (one launch each)
   template <typename T, typename SpecT>
-  void ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol = -1);
+  void ypb(const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes, int scol = -1,
+           const int* gate = nullptr);
   template <typename T>
-  void xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes);
+  void xpb(const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes, const int* gate = nullptr);
   template <typename T>
   void xpf(const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes, const int* gate = nullptr);
   template <typename T, typename SpecT>
@@ -171,9 +178,9 @@ class Context {
   }
   template <typename T, typename SpecT>
   void bwd(const SpecT* spec, long long sstride, T* phys, long long pstride, int nv, int zoff, int nplanes,
-           C2<T>* tmp, int scol = -1) {
-    ypb<T, SpecT>(spec, sstride, tmp, nv, zoff, nplanes, scol);
-    xpb<T>(tmp, phys, pstride, nv, nplanes);
+           C2<T>* tmp, int scol = -1, const int* gate = nullptr) {
+    ypb<T, SpecT>(spec, sstride, tmp, nv, zoff, nplanes, scol, gate);
+    xpb<T>(tmp, phys, pstride, nv, nplanes, gate);
   }
   void* alloc(size_t bytes);
   void build_factorisations();
@@ -216,6 +223,7 @@ class Context {
   bool fuse_rhs_ = false;          // rhs + boundary rows + Helmholtz fused (TBD_RHS_UNFUSED=1 disables)
   bool want_rhs_ = true;           // the variant switch (fuse_rhs_ also needs the even/odd operators)
   bool pin_fused_ = false, pin_rhs_ = false, pin_overlap_ = false, pin_wide_ = false;
+  bool zwide_ = false;             // plain contractions on panels of twice the column groups
   bool fuse_p_ = false;            // divergence + pressure solve + projection fused (same switch)
   // The scalar's Correct of stage s reads the carrier's physical velocity of
   // stage s and writes only scalar state; without buoyancy the carrier never
@@ -289,6 +297,8 @@ class Context {
   bool capturing_ = false;
   double* force_ = nullptr;    // mean-mode force read by the captured H update
   int* filt_ = nullptr;        // set by the positivity filter when it changes c
+  int* need_ = nullptr;        // set when physical c may hold a negative value (the filter's gate)
+  double* mmpart_ = nullptr;   // per-block min/max of the negativity check
   double* stats_ = nullptr;    // [kStatBins][kPhaseStats][nz] sums
   long long stat_count_[64] = {};
   void drop_graphs();

@@ -66,11 +66,11 @@ void fft_forward(const FftPlan& p, const T* phys, long long phys_stride, SpecT* 
                  long long spec_stride, int nv, int zoff, int nplanes, C2<T>* tmp,
                  hipStream_t st, int scol = -1, const int* gate = nullptr);
 
-// spectral -> physical
+// spectral -> physical (gate as for the forward direction)
 template <typename T, typename SpecT>
 void fft_backward(const FftPlan& p, const SpecT* spec, long long spec_stride, T* phys,
                   long long phys_stride, int nv, int zoff, int nplanes, C2<T>* tmp,
-                  hipStream_t st, int scol = -1);
+                  hipStream_t st, int scol = -1, const int* gate = nullptr);
 
 // Individual passes (exposed for fused callers and tests).
 template <typename T>
@@ -82,9 +82,10 @@ void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long lon
                        const int* gate = nullptr);
 template <typename T, typename SpecT>
 void fft_ypass_backward(const FftPlan& p, const SpecT* spec, long long spec_stride, C2<T>* tmp,
-                        int nv, int zoff, int nplanes, hipStream_t st, int scol = -1);
+                        int nv, int zoff, int nplanes, hipStream_t st, int scol = -1,
+                        const int* gate = nullptr);
 template <typename T>
 void fft_xpass_backward(const FftPlan& p, const C2<T>* tmp, T* phys, long long phys_stride,
-                        int nv, int nplanes, hipStream_t st);
+                        int nv, int nplanes, hipStream_t st, const int* gate = nullptr);
 
 }  // namespace tbd

@@ -118,7 +118,8 @@ __global__ __launch_bounds__(NT) void k_xpass_fwd(const T* __restrict__ phys, lo
 template <typename T, int NT, int MAXB>
 __global__ __launch_bounds__(NT) void k_xpass_bwd(const C2<T>* __restrict__ tmp, T* __restrict__ phys,
                                                   long long pstride, FftLen L, const C2<T>* twg,
-                                                  int ny, int mx, int nplanes, int nseq) {
+                                                  int ny, int mx, int nplanes, int nseq, const int* gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ char smem_raw[];
   C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
   const int N = L.n;
@@ -200,7 +201,9 @@ __global__ __launch_bounds__(NT) void k_ypass_fwd(const C2<T>* __restrict__ tmp,
 template <typename T, typename SpecT, int NT, int MAXB>
 __global__ __launch_bounds__(NT) void k_ypass_bwd(const SpecT* __restrict__ spec, long long sstride,
                                                   C2<T>* __restrict__ tmp, FftLen L, const C2<T>* twg,
-                                                  int nz, int mx, int nyc, int nplanes, int zoff, int zt) {
+                                                  int nz, int mx, int nyc, int nplanes, int zoff, int zt,
+                                                  const int* gate) {
+  if (gate && *gate == 0) return;
   extern __shared__ char smem_raw[];
   C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
   const int N = L.n;
@@ -309,7 +312,8 @@ __global__ __launch_bounds__(NT) void k_ypf2(const C2<T>* __restrict__ tmp, Spec
 template <typename T, typename SpecT, int N1, int N2, int NT>
 __global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, long long sstride,
                                              C2<T>* __restrict__ tmp, const C2<T>* __restrict__ tw, int scol, int mx,
-                                             int nyc, int nplanes, int zoff) {
+                                             int nyc, int nplanes, int zoff, const int* gate) {
+  if (gate && *gate == 0) return;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   // exchange through one real plane, real then imaginary parts (see k_ypf2)
@@ -414,7 +418,9 @@ __global__ __launch_bounds__(NT) void k_xpf2(const T* __restrict__ phys, long lo
 // x pass, backward: T[pl][kx][y] (kx < mx) -> rows, halfcomplex c2r
 template <typename T, int N1, int N2, int NT>
 __global__ __launch_bounds__(NT) void k_xpb2(const C2<T>* __restrict__ tmp, T* __restrict__ phys, long long pstride,
-                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes) {
+                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes,
+                                             const int* gate) {
+  if (gate && *gate == 0) return;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(true)];
@@ -531,18 +537,18 @@ static bool ypf2_nt(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long s
 
 template <typename T, typename SpecT, int NT, int A, int B>
 static void ypb2_launch(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff,
-                        int nplanes, hipStream_t st, int sc) {
+                        int nplanes, hipStream_t st, int sc, const int* gate) {
   constexpr int S = seqs2<A, B, NT>();
   dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);
   hipLaunchKernelGGL((k_ypb2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, spec, sstride, tmp, twy<T>(p), sc,
-                     p.d.mx, p.d.nyc, nplanes, zoff);
+                     p.d.mx, p.d.nyc, nplanes, zoff, gate);
 }
 template <typename T, typename SpecT, int NT>
 static bool ypb2_nt(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
-                 hipStream_t st, int scol) {
+                 hipStream_t st, int scol, const int* gate) {
   const int sc = scol > 0 ? scol : p.d.nz;
 #define TBD_CASE(NN, A, B) \
-  case NN: ypb2_launch<T, SpecT, NT, A, B>(p, spec, sstride, tmp, nv, zoff, nplanes, st, sc); return true;
+  case NN: ypb2_launch<T, SpecT, NT, A, B>(p, spec, sstride, tmp, nv, zoff, nplanes, st, sc, gate); return true;
   if (p.swap_yb) switch (p.d.ny) { TBD_FOR_SWAPPED(TBD_CASE) default: break; }
   switch (p.d.ny) { TBD_FOR_LENGTHS(TBD_CASE) default: return false; }
 #undef TBD_CASE
@@ -568,17 +574,17 @@ static bool xpf2_nt(const FftPlan& p, const T* phys, long long pstride, C2<T>* t
 
 template <typename T, int NT, int A, int B>
 static void xpb2_launch(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
-                        hipStream_t st) {
+                        hipStream_t st, const int* gate) {
   constexpr int S = seqs2<A, B, NT>();
   dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);
   hipLaunchKernelGGL((k_xpb2<T, A, B, NT>), grid, dim3(NT), 0, st, tmp, phys, pstride, twx<T>(p), p.d.ny, p.d.mx,
-                     nplanes);
+                     nplanes, gate);
 }
 template <typename T, int NT>
 static bool xpb2_nt(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
-                 hipStream_t st) {
+                 hipStream_t st, const int* gate) {
 #define TBD_CASE(NN, A, B) \
-  case NN: xpb2_launch<T, NT, A, B>(p, tmp, phys, pstride, nv, nplanes, st); return true;
+  case NN: xpb2_launch<T, NT, A, B>(p, tmp, phys, pstride, nv, nplanes, st, gate); return true;
   if (p.swap_xb) switch (p.d.nx) { TBD_FOR_SWAPPED(TBD_CASE) default: break; }
   switch (p.d.nx) { TBD_FOR_LENGTHS(TBD_CASE) default: return false; }
 #undef TBD_CASE
@@ -595,9 +601,9 @@ static bool ypf2(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstr
 }
 template <typename T, typename SpecT>
 static bool ypb2(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
-                 hipStream_t st, int scol) {
-  return p.wide_yb ? ypb2_nt<T, SpecT, 2 * Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol)
-                : ypb2_nt<T, SpecT, Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol);
+                 hipStream_t st, int scol, const int* gate) {
+  return p.wide_yb ? ypb2_nt<T, SpecT, 2 * Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol, gate)
+                : ypb2_nt<T, SpecT, Nt2<T>::v>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol, gate);
 }
 template <typename T>
 static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
@@ -607,9 +613,9 @@ static bool xpf2(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp,
 }
 template <typename T>
 static bool xpb2(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
-                 hipStream_t st) {
-  return p.wide_xb ? xpb2_nt<T, 2 * Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st)
-                : xpb2_nt<T, Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st);
+                 hipStream_t st, const int* gate) {
+  return p.wide_xb ? xpb2_nt<T, 2 * Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st, gate)
+                : xpb2_nt<T, Nt2<T>::v>(p, tmp, phys, pstride, nv, nplanes, st, gate);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,8 +660,8 @@ void fft_xpass_forward(const FftPlan& p, const T* phys, long long pstride, C2<T>
 
 template <typename T>
 void fft_xpass_backward(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,
-                        hipStream_t st) {
-  if (p.two_level && xpb2<T>(p, tmp, phys, pstride, nv, nplanes, st)) {
+                        hipStream_t st, const int* gate) {
+  if (p.two_level && xpb2<T>(p, tmp, phys, pstride, nv, nplanes, st, gate)) {
     TBD_HIP(hipGetLastError());
     return;
   }
@@ -663,7 +669,7 @@ void fft_xpass_backward(const FftPlan& p, const C2<T>* tmp, T* phys, long long p
   const int nseq = std::max(1, std::min(seq_per_block<T>(n), (p.d.ny + 1) / 2));
   dim3 grid(ceil_div(p.d.ny, 2 * nseq), nv * nplanes);
   hipLaunchKernelGGL((k_xpass_bwd<T, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, nseq), st, tmp, phys,
-                     pstride, p.lx, twx<T>(p), p.d.ny, p.d.mx, nplanes, nseq);
+                     pstride, p.lx, twx<T>(p), p.d.ny, p.d.mx, nplanes, nseq, gate);
   TBD_HIP(hipGetLastError());
 }
 
@@ -686,8 +692,8 @@ void fft_ypass_forward(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long lon
 
 template <typename T, typename SpecT>
 void fft_ypass_backward(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff,
-                        int nplanes, hipStream_t st, int scol) {
-  if (p.two_level && ypb2<T, SpecT>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol)) {
+                        int nplanes, hipStream_t st, int scol, const int* gate) {
+  if (p.two_level && ypb2<T, SpecT>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol, gate)) {
     TBD_HIP(hipGetLastError());
     return;
   }
@@ -695,7 +701,7 @@ void fft_ypass_backward(const FftPlan& p, const SpecT* spec, long long sstride, 
   const int zt = std::max(1, std::min(seq_per_block<T>(n), nplanes));
   dim3 grid(ceil_div(nplanes, zt), p.d.mx, nv);
   hipLaunchKernelGGL((k_ypass_bwd<T, SpecT, kNT, kMaxB>), grid, dim3(kNT), lds_bytes<T>(n, zt), st, spec, sstride,
-                     tmp, p.ly, twy<T>(p), scol > 0 ? scol : p.d.nz, p.d.mx, p.d.nyc, nplanes, zoff, zt);
+                     tmp, p.ly, twy<T>(p), scol > 0 ? scol : p.d.nz, p.d.mx, p.d.nyc, nplanes, zoff, zt, gate);
   TBD_HIP(hipGetLastError());
 }
 
@@ -708,20 +714,20 @@ void fft_forward(const FftPlan& p, const T* phys, long long pstride, SpecT* spec
 
 template <typename T, typename SpecT>
 void fft_backward(const FftPlan& p, const SpecT* spec, long long sstride, T* phys, long long pstride, int nv,
-                  int zoff, int nplanes, C2<T>* tmp, hipStream_t st, int scol) {
-  fft_ypass_backward<T, SpecT>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol);
-  fft_xpass_backward<T>(p, tmp, phys, pstride, nv, nplanes, st);
+                  int zoff, int nplanes, C2<T>* tmp, hipStream_t st, int scol, const int* gate) {
+  fft_ypass_backward<T, SpecT>(p, spec, sstride, tmp, nv, zoff, nplanes, st, scol, gate);
+  fft_xpass_backward<T>(p, tmp, phys, pstride, nv, nplanes, st, gate);
 }
 
 #define TBD_FFT_INST(T, S)                                                                              \
   template void fft_forward<T, S>(const FftPlan&, const T*, long long, S*, long long, int, int, int,  \
                                   C2<T>*, hipStream_t, int, const int*);                              \
   template void fft_backward<T, S>(const FftPlan&, const S*, long long, T*, long long, int, int, int, \
-                                   C2<T>*, hipStream_t, int);                                         \
+                                   C2<T>*, hipStream_t, int, const int*);                             \
   template void fft_ypass_forward<T, S>(const FftPlan&, const C2<T>*, S*, long long, int, int, int,   \
                                         hipStream_t, int, const int*);                                \
   template void fft_ypass_backward<T, S>(const FftPlan&, const S*, long long, C2<T>*, int, int, int,  \
-                                         hipStream_t, int);
+                                         hipStream_t, int, const int*);
 
 TBD_FFT_INST(double, double2)
 TBD_FFT_INST(float, float2)
@@ -730,7 +736,9 @@ template void fft_xpass_forward<double>(const FftPlan&, const double*, long long
                                         const int*);
 template void fft_xpass_forward<float>(const FftPlan&, const float*, long long, float2*, int, int, hipStream_t,
                                        const int*);
-template void fft_xpass_backward<double>(const FftPlan&, const double2*, double*, long long, int, int, hipStream_t);
-template void fft_xpass_backward<float>(const FftPlan&, const float2*, float*, long long, int, int, hipStream_t);
+template void fft_xpass_backward<double>(const FftPlan&, const double2*, double*, long long, int, int, hipStream_t,
+                                         const int*);
+template void fft_xpass_backward<float>(const FftPlan&, const float2*, float*, long long, int, int, hipStream_t,
+                                        const int*);
 
 }  // namespace tbd

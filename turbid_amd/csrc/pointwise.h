@@ -62,7 +62,16 @@ void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re
 
 // Bartnicki positivity filter, 5 iterations per z-plane (phase_scalar_flat.cc:427-475)
 // touched (optional, device int): set to 1 when any plane is changed
-void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st);
+// gate (optional, device int): the filter does nothing when *gate == 0
+void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st, const int* gate = nullptr);
+
+// *need (device int) = 1 when some x[i] may be negative: min x <= 1e-4 max x
+// (or max x <= 0), else 0.  part: 1024 doubles of scratch.  The filter above
+// leaves a field with no negative value unchanged, so a check on an FP32
+// transform of c (whose error is far below 1e-4 of its largest value)
+// decides whether the exact FP64 filter has anything to do.
+template <typename T>
+void negativity_check(const T* x, long long n, double* part, int* need, hipStream_t st);
 
 // What DftForward(DftBackward(s)) does to a truncated spectrum: the
 // halfcomplex inverse keeps the real part of the kx = 0 sequences, so the

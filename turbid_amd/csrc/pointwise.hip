@@ -1,6 +1,8 @@
 // Elementwise and reduction kernels of the substage (see pointwise.h).
 #include "pointwise.h"
 
+#include <algorithm>
+
 namespace tbd {
 
 static constexpr int kNT = 256;
@@ -221,7 +223,9 @@ void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re
 // One workgroup per z-plane; the plane statistics are FP64 sums.  A plane
 // with no negative entries is left untouched after the first reduction (each
 // further pass would add 0 to the positive entries and zero the zeros).
-__global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long long plane, int* touched) {
+__global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long long plane, int* touched,
+                                                 const int* gate) {
+  if (gate && *gate == 0) return;
   __shared__ double red[4][16];
   double* pl = c + (long long)blockIdx.x * plane;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -259,10 +263,61 @@ __global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long lo
     __syncthreads();
   }
 }
-void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st) {
-  hipLaunchKernelGGL(k_filter, dim3(nz), dim3(1024), 0, st, c, (long long)nx * ny, touched);
+void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st, const int* gate) {
+  hipLaunchKernelGGL(k_filter, dim3(nz), dim3(1024), 0, st, c, (long long)nx * ny, touched, gate);
   TBD_HIP(hipGetLastError());
 }
+
+// Smallest and largest value of x: per-block partials, then one block that
+// sets *need = 1 when the smallest is not above kNegMargin times the largest
+// (or the largest is not positive), else 0.
+static constexpr int kMinMaxBlocks = 512;
+static constexpr double kNegMargin = 1e-4;
+template <typename T>
+__global__ __launch_bounds__(256) void k_minmax_part(const T* __restrict__ x, long long n, double* part) {
+  __shared__ double lo[4], hi[4];
+  double a = 1e300, b = -1e300;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double v = (double)x[i];
+    a = fmin(a, v);
+    b = fmax(b, v);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmin(a, __shfl_xor(a, o));
+    b = fmax(b, __shfl_xor(b, o));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { lo[wave] = a; hi[wave] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) { a = fmin(a, lo[w]); b = fmax(b, hi[w]); }
+    part[2 * blockIdx.x] = fmin(a, lo[0]);
+    part[2 * blockIdx.x + 1] = fmax(b, hi[0]);
+  }
+}
+__global__ __launch_bounds__(64) void k_minmax_final(const double* __restrict__ part, int nb, int* need) {
+  double a = 1e300, b = -1e300;
+  for (int i = threadIdx.x; i < nb; i += 64) {
+    a = fmin(a, part[2 * i]);
+    b = fmax(b, part[2 * i + 1]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmin(a, __shfl_xor(a, o));
+    b = fmax(b, __shfl_xor(b, o));
+  }
+  if (threadIdx.x == 0) *need = (b <= 0.0 || a <= kNegMargin * b) ? 1 : 0;
+}
+template <typename T>
+void negativity_check(const T* x, long long n, double* part, int* need, hipStream_t st) {
+  const int nb = (int)std::min<long long>(kMinMaxBlocks, std::max<long long>(1, (n + 255) / 256));
+  hipLaunchKernelGGL(k_minmax_part<T>, dim3(nb), dim3(256), 0, st, x, n, part);
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, part, nb, need);
+  TBD_HIP(hipGetLastError());
+}
+template void negativity_check<float>(const float*, long long, double*, int*, hipStream_t);
+template void negativity_check<double>(const double*, long long, double*, int*, hipStream_t);
 
 __global__ void k_kx0_round_trip(double2* __restrict__ s, int nyc, int nz, const int* gate) {
   if (gate && *gate != 0) return;

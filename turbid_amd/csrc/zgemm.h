@@ -50,6 +50,10 @@ struct ZArgs {
   double2* wall_bot = nullptr; double2* wall_top = nullptr;
   double2* wall2_bot = nullptr; double2* wall2_top = nullptr;
   int open_top = 0;
+  // panel width: the column groups of 16 a workgroup takes are multiplied
+  // by this where the LDS and the workgroup's tail-row pass allow it (an
+  // autotuner choice; 1 keeps the width make_panel picks)
+  int ncg_mult = 1;
 };
 
 // Even/odd split of a centro-(anti)symmetric nz x nz operator A (nz = 2M+1,

@@ -58,6 +58,7 @@ __device__ __forceinline__ V4<float> ld4(const float* p) {
 static constexpr int kNT = 512;           // threads per workgroup
 static constexpr int kWaves = kNT / 64;
 static constexpr int kMaxU = 2;           // units (and VALU tail outputs) per thread in buffered mode
+static constexpr int kSplitMaxCol = 64;   // columns of a panel at most (per-column LDS tables)
 
 // A contraction segment: output rows [0, rows) of the operator block stored
 // at op + opofs*lda, contracted with panel entries [kofs, kofs + K) of every
@@ -88,7 +89,7 @@ __host__ inline Seg make_seg(int rows, int K, int kofs, int opofs, int rbase) {
 
 // parity != 0 needs nz odd: the operator is stored as the (M+1) x (M+1)
 // even block followed by the (M+1) x M odd block (M = (nz-1)/2)
-__host__ inline Panel make_panel(int nz, size_t elem, size_t lds_limit, int parity = 0) {
+__host__ inline Panel make_panel(int nz, size_t elem, size_t lds_limit, int parity = 0, int ncg_mult = 1) {
   Panel pn;
   pn.nz = nz;
   pn.parity = parity;
@@ -111,6 +112,8 @@ __host__ inline Panel make_panel(int nz, size_t elem, size_t lds_limit, int pari
   }
   int ncg = pairs > 0 ? (kWaves + pairs - 1) / pairs : 1;
   if (ncg > 4) ncg = 4;
+  ncg *= ncg_mult;
+  if (ncg > kSplitMaxCol / 16) ncg = kSplitMaxCol / 16;
   while (ncg > 1 && (size_t)16 * ncg * pn.ldx * elem > lds_limit) --ncg;
   // keep the rows past the last full tile within what one pass of the
   // workgroup (or the register-buffered mode) holds
@@ -518,7 +521,6 @@ __device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, 
 }
 
 static constexpr size_t kLdsLimit = 80 * 1024;   // two workgroups per CU
-static constexpr int kSplitMaxCol = 64;
 
 // (kx, ky) of spectral column col (columns past P, the padding of the last
 // panel, read column 0); 32-bit index arithmetic (P < 2^31)
@@ -633,7 +635,7 @@ void zgemm(const ZArgs& a, int nfield, hipStream_t st) {
   TBD_CHECK(a.nz >= 4 && a.lda % 4 == 0 && a.P > 0, "zgemm > bad shape");
   TBD_CHECK(a.parity == 0 ? a.lda >= a.nz : (a.nz % 2 == 1 && a.lda >= (a.nz + 1) / 2),
             "zgemm > operator layout does not match the column length");
-  const Panel pn = make_panel(a.nz, sizeof(C2<T>), kLdsLimit, a.parity);
+  const Panel pn = make_panel(a.nz, sizeof(C2<T>), kLdsLimit, a.parity, a.ncg_mult);
   const size_t lds = sizeof(C2<T>) * pn.ncol * pn.ldx;
   TBD_CHECK(pn.ncol <= kSplitMaxCol, "zgemm > too many columns per panel");
   TBD_CHECK(lds + sizeof(double2) * kSplitMaxCol <= 160 * 1024, "zgemm > nz too large for one LDS panel");
@@ -975,7 +977,11 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   const long long c0 = (long long)blockIdx.x * pd.ncol;
   const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
   __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
-  for (int c = threadIdx.x; c < pd.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
+  __shared__ double qv[kSplitMaxCol];    // s1 + kx^2 + ky^2 of each panel column
+  for (int c = threadIdx.x; c < pd.ncol; c += kNT) {
+    kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
+    qv[c] = a.s1 + kv[c].x * kv[c].x + kv[c].y * kv[c].y;
+  }
   const double2 zero = make_double2(0.0, 0.0);
   // boundary rows of every column of the panel
   if ((int)threadIdx.x < pd.ncol) {
@@ -1032,12 +1038,8 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   //   s1 x - ((D2 x - kx^2 x) - ky^2 x) = -(s'_j +- d'_j)
   // and the right-hand side needs no second read of x.
   gemm_panel<double, DEPTH>(a.d2p, a.ldp, pd, xs, [&](int row, int c, double2 v) {
-    if (row > 2 * M) {        // the split's last index holds no half of x
-      xs[c * ldx + row] = v;
-      return;
-    }
-    const double kx = kv[c].x, ky = kv[c].y;
-    const double q = a.s1 + kx * kx + ky * ky;
+    // the split's last index holds no half of x
+    const double q = row > 2 * M ? 0.0 : qv[c];
     const double2 xv = xs[c * ldx + row];
     xs[c * ldx + row] = make_double2(v.x - q * xv.x, v.y - q * xv.y);
   });

@@ -339,10 +339,11 @@ class Solver:
     # y forward / x backward / x forward transform kernels at twice their
     # workgroup size
     VARIANT_BITS = {"fused": 1, "rhs": 2, "overlap": 4, "wide_yb": 8, "wide_yf": 16, "wide_xb": 32,
-                    "wide_xf": 64, "swap_yb": 128, "swap_yf": 256, "swap_xb": 512, "swap_xf": 1024}
+                    "wide_xf": 64, "swap_yb": 128, "swap_yf": 256, "swap_xb": 512, "swap_xf": 1024,
+                    "wide_zgemm": 2048}
     WIDE_FFT = 8 | 16 | 32 | 64
     SWAP_FFT = 128 | 256 | 512 | 1024
-    N_VARIANTS = 2048
+    N_VARIANTS = 4096
 
     def autotune(self, steps=5):
         """Times the kernel variants on a scratch copy and keeps the fastest;
