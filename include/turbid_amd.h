@@ -162,6 +162,31 @@ int tbd_autotune(tbd_ctx* ctx, int steps, double* ms_per_step, int* bits);
 int tbd_get_variant(tbd_ctx* ctx, int* bits);
 int tbd_set_variant(tbd_ctx* ctx, int bits);
 
+/* Ensemble of independent contexts advanced together (the reference runs one
+ * FlatChannelSolver per realisation, solver_channel_flat.cc:117-137 being the
+ * step loop of each; it has no batched form).  The members (at most 32, not
+ * owned, created with tbd_create and kept alive while the ensemble exists)
+ * may differ in grid, Reynolds number, settling velocity and the rest of
+ * their configuration.  tbd_ensemble_step advances every member by one RK3
+ * step (member m at time[m] with alternation[m], as tbd_step): each member's
+ * step is recorded once per (alternation, wall-row slot, dt, previous dt) of
+ * all members, the members' launches are merged into one launch per kernel
+ * and launch shape at each position of the step (the workgroups of all
+ * members in one grid, each with its own arguments), and the merged
+ * sequence is captured into one HIP graph and replayed.  It is ordered after
+ * the work already issued on the members and before what is issued on them
+ * afterwards (uploads, downloads, tbd_features).  Members must run with the
+ * overlap and fused-transform variant bits clear (tbd_set_variant) and the
+ * register transforms; otherwise the step fails with an error.
+ * tbd_ensemble_launches: grouped launches per step and the launches the
+ * members would issue one by one, of the last recorded step. */
+typedef struct tbd_ens tbd_ens;
+int tbd_ensemble_create(tbd_ctx* const* members, int n, tbd_ens** out);
+int tbd_ensemble_step(tbd_ens* e, const double* time, const int* alternation);
+int tbd_ensemble_launches(tbd_ens* e, int* grouped, int* member);
+int tbd_ensemble_sync(tbd_ens* e);
+void tbd_ensemble_destroy(tbd_ens* e);
+
 /* Stand-alone device kernels on host buffers (parity tests of single pieces).
  * DftForward / DftBackward of nv components (src/cube_real_cplx.cc:525-555);
  * fp32 != 0 runs the FP32 transform on FP64 data. */

@@ -62,3 +62,81 @@ def test_member_matches_oracle():
     tol = 1e-5 if cfg["precision"] == 1 else 1e-9
     assert rel_l2(ug, uo) < tol
     assert rel_l2(cg, co) < tol
+
+
+def _alone(cfg, nsteps):
+    s = RT.Solver(cfg)
+    s.upload(*W.initial_state(cfg))
+    RT.run_steps(s, RT.Stepper(cfg["dt"]), nsteps)
+    out = s.download()
+    s.close()
+    return out
+
+
+def _grouped_members():
+    # grids of the register transforms' lengths (36 is not one of them)
+    cfgs = []
+    for m, (n, nz, re_d, ws) in enumerate([(24, 33, 400.0, 1e-3), (48, 17, 700.0, 2e-4), (16, 65, 1000.0, 5e-4),
+                                           (24, 33, 900.0, 1e-4)]):
+        c = W.base_config(n, n, nz, 20.0, 10.0, W.reynolds_from_stokes(re_d), W.TWO_PI / 2000.0, settling=ws,
+                          schmidt=0.5, precision=m % 2)
+        c.update(noise_u=0.1, noise_c=0.01, seed=60 + m)
+        cfgs.append(c)
+    return cfgs
+
+
+def test_grouped_members_match_alone():
+    # FP64 and mixed members of three grids in one ensemble: their launch
+    # sequences differ, so the merge groups some positions and not others;
+    # 5 steps replay both recorded (alternation, wall-row slot) programs
+    cfgs = _grouped_members()
+    ens = E.Ensemble(cfgs, grouped=True)
+    for _ in range(5):
+        ens.step_grouped()
+    g, m = ens.group.launches()
+    got = [s.download() for s in ens.solvers]
+    feats = [f.copy() for f in ens.feats]
+    ens.close()
+    print("grouped launches per step %d, member launches %d" % (g, m))
+    assert 0 < g < m
+    for cfg, (ug, pg, cg), fg in zip(cfgs, got, feats):
+        ua, pa, ca = _alone(cfg, 5)
+        assert np.array_equal(ug, ua)
+        assert np.array_equal(pg, pa)
+        assert np.array_equal(cg, ca)
+        assert np.all(np.isfinite(fg[:20]))
+
+
+def test_grouped_ensemble_of_the_sixteen_members_matches_streams():
+    # the sixteen members of the ensemble workload (64..128 squared by
+    # 65..129, mixed precision): grouped and stream-concurrent runs agree bit
+    # for bit, and alike members merge position by position
+    cfgs = E.member_configs(16, seed=0, precision=1)
+    ens = E.Ensemble(cfgs, grouped=True)
+    for _ in range(3):
+        ens.step_grouped()
+    g, m = ens.group.launches()
+    got = [s.download() for s in ens.solvers]
+    ens.close()
+    print("grouped launches per step %d, member launches %d" % (g, m))
+    assert 3 * g < m
+    ref = E.Ensemble(cfgs)
+    for _ in range(3):
+        ref.step()
+    want = [s.download() for s in ref.solvers]
+    ref.close()
+    for (ug, pg, cg), (ua, pa, ca) in zip(got, want):
+        assert np.array_equal(ug, ua)
+        assert np.array_equal(cg, ca)
+
+
+def test_grouped_step_refuses_an_overlapping_member():
+    cfg = _grouped_members()[0]
+    s = RT.Solver(cfg)
+    s.upload(*W.initial_state(cfg))
+    s.set_variant(s.variant() | RT.Solver.VARIANT_BITS["overlap"])
+    grp = RT.GroupedEnsemble([s])
+    with pytest.raises(RT.TurbidError, match="overlap"):
+        grp.step([cfg["dt"]], [0])
+    grp.close()
+    s.close()

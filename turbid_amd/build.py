@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libturbid_amd.so")
 ARCH = os.environ.get("TURBID_AMD_ARCH", "gfx950")
 
-SOURCES = ["fft.hip", "nonlinear.hip", "zgemm.hip", "pointwise.hip", "sno.hip", "noalloc.hip", "setup.cpp", "context.cpp", "api.cpp"]
+SOURCES = ["fft.hip", "nonlinear.hip", "zgemm.hip", "pointwise.hip", "sno.hip", "noalloc.hip", "setup.cpp", "context.cpp", "ensemble.cpp", "api.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

@@ -7,6 +7,7 @@
 
 #include "../../include/turbid_amd.h"
 #include "context.h"
+#include "ensemble.h"
 #include "fft.h"
 #include "pointwise.h"
 #include "zgemm.h"
@@ -14,6 +15,9 @@
 
 struct tbd_ctx {
   tbd::Context* c;
+};
+struct tbd_ens {
+  tbd::Ensemble* e;
 };
 
 namespace tbd {
@@ -102,6 +106,54 @@ int tbd_substage(tbd_ctx* ctx, int stage, double time, int alternation) {
     TBD_CHECK(ctx, "tbd_substage > null context");
     ctx->c->substage(stage, time, alternation);
   });
+}
+
+int tbd_ensemble_create(tbd_ctx* const* members, int n, tbd_ens** out) {
+  return guard([&] {
+    TBD_CHECK(members && out && n >= 1, "tbd_ensemble_create > null argument or no members");
+    *out = nullptr;
+    std::vector<tbd::Context*> m;
+    for (int i = 0; i < n; ++i) {
+      TBD_CHECK(members[i], "tbd_ensemble_create > null member");
+      m.push_back(members[i]->c);
+    }
+    tbd_ens* e = new tbd_ens;
+    try {
+      e->e = new tbd::Ensemble(m);
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+int tbd_ensemble_step(tbd_ens* e, const double* time, const int* alternation) {
+  return guard([&] {
+    TBD_CHECK(e && time && alternation, "tbd_ensemble_step > null argument");
+    e->e->step(time, alternation);
+  });
+}
+
+int tbd_ensemble_launches(tbd_ens* e, int* grouped, int* member) {
+  return guard([&] {
+    TBD_CHECK(e, "tbd_ensemble_launches > null ensemble");
+    if (grouped) *grouped = e->e->grouped_launches();
+    if (member) *member = e->e->member_launches();
+  });
+}
+
+int tbd_ensemble_sync(tbd_ens* e) {
+  return guard([&] {
+    TBD_CHECK(e, "tbd_ensemble_sync > null ensemble");
+    e->e->sync();
+  });
+}
+
+void tbd_ensemble_destroy(tbd_ens* e) {
+  if (!e) return;
+  try { delete e->e; } catch (...) {}
+  delete e;
 }
 
 int tbd_step(tbd_ctx* ctx, double time, int alternation) {

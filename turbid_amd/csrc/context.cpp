@@ -691,6 +691,45 @@ void Context::step(double time, int alternation) {
   TBD_HIP(hipGraphLaunch(g.exec, st_));
 }
 
+void Context::record_step(double time, int alternation, Recorder& r) {
+  TBD_CHECK(!prof.on, "Context::record_step > not while profiling");
+  TBD_CHECK(Recorder::active() == nullptr, "Context::record_step > a recording is already active on this thread");
+  last_dt_ = dt_;
+  last_olddt_ = olddt_;
+  // a capture on the context's own stream catches any work not routed
+  // through the recorder: it must come out empty
+  TBD_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+  Recorder::active() = &r;
+  capturing_ = true;
+  try {
+    stages(time, alternation);
+  } catch (...) {
+    Recorder::active() = nullptr;
+    capturing_ = false;
+    hipGraph_t broken = nullptr;
+    (void)hipStreamEndCapture(st_, &broken);
+    if (broken) (void)hipGraphDestroy(broken);
+    throw;
+  }
+  Recorder::active() = nullptr;
+  capturing_ = false;
+  hipGraph_t g = nullptr;
+  TBD_HIP(hipStreamEndCapture(st_, &g));
+  size_t nodes = 0;
+  const hipError_t e = hipGraphGetNodes(g, nullptr, &nodes);
+  (void)hipGraphDestroy(g);
+  TBD_HIP(e);
+  TBD_CHECK(nodes == 0, "Context::record_step > the step issued work outside the recorded launches "
+                        "(fused x transforms, LDS Stockham transforms or a second stream)");
+}
+
+void Context::replayed_step() {
+  last_dt_ = dt_;
+  last_olddt_ = olddt_;
+  wslot_ = 1 - wslot_;   // three toggles
+  olddt_ = dt_;
+}
+
 void Context::upload(const double* uvw, const double* p, const double* c) {
   const size_t S16 = sizeof(double2) * d.S;
   TBD_HIP(hipMemcpyAsync(uvw_, uvw, 3 * S16, hipMemcpyHostToDevice, st_));
@@ -839,8 +878,8 @@ void Context::carrier_predict(int stage, double time, int alternation) {
       fwd<T, CT>(aphys, R, adv, S, 3, 0, d.nz, tmp);
       if (erosion_planes) {
         const size_t pl = (size_t)d.nx * d.ny * sizeof(T);
-        TBD_HIP(hipMemcpyAsync(dudz_bot_[stage & 1], gphys + 2 * R, pl, hipMemcpyDeviceToDevice, st_));
-        TBD_HIP(hipMemcpyAsync(dvdz_bot_[stage & 1], gphys + 5 * R, pl, hipMemcpyDeviceToDevice, st_));
+        copy_words(dudz_bot_[stage & 1], gphys + 2 * R, pl, st_);
+        copy_words(dvdz_bot_[stage & 1], gphys + 5 * R, pl, st_);
       }
     }
   } else {
@@ -997,11 +1036,10 @@ void Context::carrier_correct(int stage) {
     launch(KC_OTHER, 0.0, [&] {
       diagnostics<T>((const T*)uvwnl_, R, d.nx, d.ny, d.nz, glx_, gly_, glz_, fl_, diagpart_, st_);
     });
-    TBD_HIP(hipMemcpyAsync(fg_, fl_, sizeof(double) * 50, hipMemcpyDeviceToDevice, st_));
-    // 10..15: dU/dz, dV/dz, dW/dz on the mean column at bottom, then top
+    // the step's features: fl_, then 10..15: dU/dz, dV/dz, dW/dz on the mean
+    // column at the bottom, then the top
     const int src[6] = {0, 2, 4, 1, 3, 5};
-    for (int i = 0; i < 6; ++i)
-      TBD_HIP(hipMemcpyAsync(fg_ + 10 + i, wallmean_ + src[i], sizeof(double), hipMemcpyDeviceToDevice, st_));
+    collect_features(fg_, fl_, 50, wallmean_, src, st_);
   }
 }
 
@@ -1090,7 +1128,7 @@ void Context::scalar_correct(int stage, int alternation) {
                 st_);
     });
   if (erosion) {   // feature 28: mean erosion flux, 29: mean deposition flux (:377-380)
-    TBD_HIP(hipMemcpyAsync(fl_ + 28, ehat_, sizeof(double), hipMemcpyDeviceToDevice, st_));
+    copy_words(fl_ + 28, ehat_, sizeof(double), st_);
     launch(KC_OTHER, 0.0, [&] { scaled_real(c_, wsc, cfg.averaged_concentration, fl_ + 29, st_); });
   }
   // Helmholtz (:399, :640-661)

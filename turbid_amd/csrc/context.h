@@ -9,6 +9,7 @@
 #include "../../include/turbid_amd.h"
 #include "common.h"
 #include "fft.h"
+#include "group.h"
 #include "setup.h"
 #include "zgemm.h"
 
@@ -133,6 +134,28 @@ class Context {
   // ms[bits] = ms per step (NaN where not tried, kVariants entries); returns
   // the kept bits.
   int autotune(int steps, double* ms);
+
+  // One step's launches recorded into r instead of issued (the grouped
+  // launches of an ensemble, group.h), with the host-side state a step
+  // advances (wall-row slot, previous dt) advanced as by step().  The
+  // forcing is read from the device pair force_ (written before each replay).
+  // Throws if anything of the step was issued on the context's stream
+  // rather than recorded (fused x transforms, the LDS Stockham transforms).
+  void record_step(double time, int alternation, Recorder& r);
+  // the host-side state advance of a step replayed from a recording
+  void replayed_step();
+  // whether record_step can record this context's step: no second stream,
+  // no fused x transforms, the register transforms in x and y, no profiling
+  bool groupable() const {
+    return !overlap_ && !fused_ && !prof.on && fft_.two_level && fft_has_two_level(d.nx) &&
+           fft_has_two_level(d.ny);
+  }
+  double* force_pair() const { return force_; }
+  void forcing(double time, double& f1, double& f2) const { forces(time, f1, f2); }
+  hipStream_t stream() const { return st_; }
+  int wall_slot() const { return wslot_; }
+  double dt() const { return dt_; }
+  double olddt() const { return olddt_; }
 
   tbd_config cfg;
   Dims d;

@@ -46,6 +46,8 @@ struct FftPlan {
 };
 
 // Whether a two-level length has the swapped factor order (FftPlan::swap_*).
+// true for the lengths the two-level register kernels cover
+bool fft_has_two_level(int n);
 bool fft_has_swap(int n);
 
 // Size of the x-pencil intermediate (complex elements) for nv components.

@@ -9,6 +9,7 @@
 // global loads/stores of the passes.
 #include "fft.h"
 #include "fft_device.h"
+#include "group.h"
 #include "regfft.h"
 
 #include <cmath>
@@ -256,18 +257,34 @@ __host__ __device__ constexpr int lds2_elems(bool with_rows) {
 }
 
 // y pass, forward: T[d][z][kx][y] -> spectral (z fastest), keep nyc, scale
-template <typename T, typename SpecT, int N1, int N2, int NT>
-__global__ __launch_bounds__(NT) void k_ypf2(const C2<T>* __restrict__ tmp, SpecT* __restrict__ spec,
-                                             long long sstride, const C2<T>* __restrict__ tw, int scol, int mx,
-                                             int nyc, int nplanes, int zoff, T scale, const int* gate) {
-  if (gate && *gate == 0) return;
+template <typename T, typename SpecT>
+struct YfA {
+  const C2<T>* tmp;
+  SpecT* spec;
+  long long sstride;
+  const C2<T>* tw;
+  int scol, mx, nyc, nplanes, zoff;
+  T scale;
+  const int* gate;
+};
+template <typename T, typename SpecT, int N1, int N2, int NT, bool G>
+__global__ __launch_bounds__(NT) void k_ypf2(YfA<T, SpecT> a0, const Group* grp) {
+  Blk bk;
+  const YfA<T, SpecT>& A = member<G>(a0, grp, bk);
+  if (A.gate && *A.gate == 0) return;
+  const C2<T>* __restrict__ tmp = A.tmp;
+  SpecT* __restrict__ spec = A.spec;
+  const long long sstride = A.sstride;
+  const C2<T>* __restrict__ tw = A.tw;
+  const int scol = A.scol, mx = A.mx, nyc = A.nyc, nplanes = A.nplanes, zoff = A.zoff;
+  const T scale = A.scale;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   // level-A -> level-B exchange through one real plane: real parts, then
   // imaginary parts (half the LDS of a complex exchange, twice the occupancy)
   __shared__ __align__(16) char raw[sizeof(T) * lds2_elems<T, N1, N2, NT>(false)];
   T* E = reinterpret_cast<T*>(raw);
-  const int z0 = blockIdx.x * S, k = blockIdx.y, d = blockIdx.z;
+  const int z0 = bk.x * S, k = bk.y, d = bk.z;
   const int ns = min(S, nplanes - z0);
   const int t = threadIdx.x;
   const int n2 = t % N2, sa = t / N2;
@@ -309,17 +326,31 @@ __global__ __launch_bounds__(NT) void k_ypf2(const C2<T>* __restrict__ tmp, Spec
 }
 
 // y pass, backward: spectral (z fastest) -> T[d][z][kx][y], zero padded
-template <typename T, typename SpecT, int N1, int N2, int NT>
-__global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, long long sstride,
-                                             C2<T>* __restrict__ tmp, const C2<T>* __restrict__ tw, int scol, int mx,
-                                             int nyc, int nplanes, int zoff, const int* gate) {
-  if (gate && *gate == 0) return;
+template <typename T, typename SpecT>
+struct YbA {
+  const SpecT* spec;
+  long long sstride;
+  C2<T>* tmp;
+  const C2<T>* tw;
+  int scol, mx, nyc, nplanes, zoff;
+  const int* gate;
+};
+template <typename T, typename SpecT, int N1, int N2, int NT, bool G>
+__global__ __launch_bounds__(NT) void k_ypb2(YbA<T, SpecT> a0, const Group* grp) {
+  Blk bk;
+  const YbA<T, SpecT>& A = member<G>(a0, grp, bk);
+  if (A.gate && *A.gate == 0) return;
+  const SpecT* __restrict__ spec = A.spec;
+  const long long sstride = A.sstride;
+  C2<T>* __restrict__ tmp = A.tmp;
+  const C2<T>* __restrict__ tw = A.tw;
+  const int scol = A.scol, mx = A.mx, nyc = A.nyc, nplanes = A.nplanes, zoff = A.zoff;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   // exchange through one real plane, real then imaginary parts (see k_ypf2)
   __shared__ __align__(16) char raw[sizeof(T) * lds2_elems<T, N1, N2, NT>(false)];
   T* E = reinterpret_cast<T*>(raw);
-  const int z0 = blockIdx.x * S, k = blockIdx.y, d = blockIdx.z;
+  const int z0 = bk.x * S, k = bk.y, d = bk.z;
   const int ns = min(S, nplanes - z0);
   const int t = threadIdx.x;
   const int sa = t % S, n2 = t / S;
@@ -361,19 +392,33 @@ __global__ __launch_bounds__(NT) void k_ypb2(const SpecT* __restrict__ spec, lon
 }
 
 // x pass, forward: rows (y0 .. y0+2S) of plane pl -> T[pl][kx][y]
-template <typename T, int N1, int N2, int NT>
-__global__ __launch_bounds__(NT) void k_xpf2(const T* __restrict__ phys, long long pstride, C2<T>* __restrict__ tmp,
-                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes,
-                                             const int* gate) {
-  if (gate && *gate == 0) return;
+template <typename T>
+struct XfA {
+  const T* phys;
+  long long pstride;
+  C2<T>* tmp;
+  const C2<T>* tw;
+  int ny, mx, nplanes;
+  const int* gate;
+};
+template <typename T, int N1, int N2, int NT, bool G>
+__global__ __launch_bounds__(NT) void k_xpf2(XfA<T> a0, const Group* grp) {
+  Blk bk;
+  const XfA<T>& A = member<G>(a0, grp, bk);
+  if (A.gate && *A.gate == 0) return;
+  const T* __restrict__ phys = A.phys;
+  const long long pstride = A.pstride;
+  C2<T>* __restrict__ tmp = A.tmp;
+  const C2<T>* __restrict__ tw = A.tw;
+  const int ny = A.ny, mx = A.mx, nplanes = A.nplanes;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(true)];
   C2<T>* E = reinterpret_cast<C2<T>*>(raw);
   C2<T>* Z = E;   // natural-order spectra after level B (aliases E)
-  const int pl = blockIdx.y;
+  const int pl = bk.y;
   const int d = pl / nplanes, zl = pl - d * nplanes;
-  const int y0 = blockIdx.x * 2 * S;
+  const int y0 = bk.x * 2 * S;
   const T* base = phys + (long long)d * pstride + (long long)zl * ny * N;
   const int t = threadIdx.x;
   if (t < S * N2) {
@@ -416,19 +461,33 @@ __global__ __launch_bounds__(NT) void k_xpf2(const T* __restrict__ phys, long lo
 }
 
 // x pass, backward: T[pl][kx][y] (kx < mx) -> rows, halfcomplex c2r
-template <typename T, int N1, int N2, int NT>
-__global__ __launch_bounds__(NT) void k_xpb2(const C2<T>* __restrict__ tmp, T* __restrict__ phys, long long pstride,
-                                             const C2<T>* __restrict__ tw, int ny, int mx, int nplanes,
-                                             const int* gate) {
-  if (gate && *gate == 0) return;
+template <typename T>
+struct XbA {
+  const C2<T>* tmp;
+  T* phys;
+  long long pstride;
+  const C2<T>* tw;
+  int ny, mx, nplanes;
+  const int* gate;
+};
+template <typename T, int N1, int N2, int NT, bool G>
+__global__ __launch_bounds__(NT) void k_xpb2(XbA<T> a0, const Group* grp) {
+  Blk bk;
+  const XbA<T>& A = member<G>(a0, grp, bk);
+  if (A.gate && *A.gate == 0) return;
+  const C2<T>* __restrict__ tmp = A.tmp;
+  T* __restrict__ phys = A.phys;
+  const long long pstride = A.pstride;
+  const C2<T>* __restrict__ tw = A.tw;
+  const int ny = A.ny, mx = A.mx, nplanes = A.nplanes;
   constexpr int N = N1 * N2;
   constexpr int S = seqs2<N1, N2, NT>();
   __shared__ __align__(16) char raw[sizeof(C2<T>) * lds2_elems<T, N1, N2, NT>(true)];
   C2<T>* E = reinterpret_cast<C2<T>*>(raw);
   C2<T>* Z = E;   // packed two-row spectra (aliases E)
-  const int pl = blockIdx.y;
+  const int pl = bk.y;
   const int d = pl / nplanes, zl = pl - d * nplanes;
-  const int y0 = blockIdx.x * 2 * S;
+  const int y0 = bk.x * 2 * S;
   const int t = threadIdx.x;
   const C2<T>* in = tmp + (long long)pl * mx * ny;
   // all of a thread's spectra are loaded before any is packed into LDS
@@ -505,6 +564,16 @@ template <> struct Nt2<float> { static constexpr int v = 256; };
   F(384, 16, 24) F(192, 12, 16) F(128, 8, 16) F(96, 8, 12) F(48, 6, 8) F(32, 4, 8) F(24, 4, 6)       \
   F(12, 3, 4) F(8, 2, 4) F(6, 2, 3)
 
+bool fft_has_two_level(int n) {
+  switch (n) {
+#define TBD_CASE(NN, A, B) case NN: return true;
+    TBD_FOR_LENGTHS(TBD_CASE)
+#undef TBD_CASE
+    default:
+      return false;
+  }
+}
+
 bool fft_has_swap(int n) {
   switch (n) {
 #define TBD_CASE(NN, A, B) case NN: return true;
@@ -520,8 +589,8 @@ static void ypf2_launch(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long lo
                         int nplanes, hipStream_t st, int sc, T scale, const int* gate) {
   constexpr int S = seqs2<A, B, NT>();
   dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);
-  hipLaunchKernelGGL((k_ypf2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, tmp, spec, sstride, twy<T>(p), sc,
-                     p.d.mx, p.d.nyc, nplanes, zoff, scale, gate);
+  glaunch(k_ypf2<T, SpecT, A, B, NT, false>, k_ypf2<T, SpecT, A, B, NT, true>, grid, dim3(NT), 0, st,
+          YfA<T, SpecT>{tmp, spec, sstride, twy<T>(p), sc, p.d.mx, p.d.nyc, nplanes, zoff, scale, gate});
 }
 template <typename T, typename SpecT, int NT>
 static bool ypf2_nt(const FftPlan& p, const C2<T>* tmp, SpecT* spec, long long sstride, int nv, int zoff, int nplanes,
@@ -540,8 +609,8 @@ static void ypb2_launch(const FftPlan& p, const SpecT* spec, long long sstride, 
                         int nplanes, hipStream_t st, int sc, const int* gate) {
   constexpr int S = seqs2<A, B, NT>();
   dim3 grid(ceil_div(nplanes, S), p.d.mx, nv);
-  hipLaunchKernelGGL((k_ypb2<T, SpecT, A, B, NT>), grid, dim3(NT), 0, st, spec, sstride, tmp, twy<T>(p), sc,
-                     p.d.mx, p.d.nyc, nplanes, zoff, gate);
+  glaunch(k_ypb2<T, SpecT, A, B, NT, false>, k_ypb2<T, SpecT, A, B, NT, true>, grid, dim3(NT), 0, st,
+          YbA<T, SpecT>{spec, sstride, tmp, twy<T>(p), sc, p.d.mx, p.d.nyc, nplanes, zoff, gate});
 }
 template <typename T, typename SpecT, int NT>
 static bool ypb2_nt(const FftPlan& p, const SpecT* spec, long long sstride, C2<T>* tmp, int nv, int zoff, int nplanes,
@@ -559,8 +628,8 @@ static void xpf2_launch(const FftPlan& p, const T* phys, long long pstride, C2<T
                         hipStream_t st, const int* gate) {
   constexpr int S = seqs2<A, B, NT>();
   dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);
-  hipLaunchKernelGGL((k_xpf2<T, A, B, NT>), grid, dim3(NT), 0, st, phys, pstride, tmp, twx<T>(p), p.d.ny, p.d.mx,
-                     nplanes, gate);
+  glaunch(k_xpf2<T, A, B, NT, false>, k_xpf2<T, A, B, NT, true>, grid, dim3(NT), 0, st,
+          XfA<T>{phys, pstride, tmp, twx<T>(p), p.d.ny, p.d.mx, nplanes, gate});
 }
 template <typename T, int NT>
 static bool xpf2_nt(const FftPlan& p, const T* phys, long long pstride, C2<T>* tmp, int nv, int nplanes,
@@ -577,8 +646,8 @@ static void xpb2_launch(const FftPlan& p, const C2<T>* tmp, T* phys, long long p
                         hipStream_t st, const int* gate) {
   constexpr int S = seqs2<A, B, NT>();
   dim3 grid(ceil_div(p.d.ny, 2 * S), nv * nplanes);
-  hipLaunchKernelGGL((k_xpb2<T, A, B, NT>), grid, dim3(NT), 0, st, tmp, phys, pstride, twx<T>(p), p.d.ny, p.d.mx,
-                     nplanes, gate);
+  glaunch(k_xpb2<T, A, B, NT, false>, k_xpb2<T, A, B, NT, true>, grid, dim3(NT), 0, st,
+          XbA<T>{tmp, phys, pstride, twx<T>(p), p.d.ny, p.d.mx, nplanes, gate});
 }
 template <typename T, int NT>
 static bool xpb2_nt(const FftPlan& p, const C2<T>* tmp, T* phys, long long pstride, int nv, int nplanes,

@@ -32,6 +32,13 @@ struct HUpd {
   const double* fdev = nullptr;   // if set: the force is read from fdev[0], fdev[1]
   int nz = 0;
 };
+// dst[i] = src[i] for i < n (n <= 64), then dst[10 + j] = wall[src6[j]], j < 6
+void collect_features(double* dst, const double* src, int n, const double* wall, const int* src6, hipStream_t st);
+// device copy of `bytes` (a multiple of 4) as a kernel
+void copy_words(void* dst, const void* src, size_t bytes, hipStream_t st);
+// dst[i][0] = v[2 i], dst[i][1] = v[2 i + 1], i < n (n <= 32): the forcing
+// pairs of an ensemble's members, one launch
+void set_pairs(double* const* dst, const double* v, int n, hipStream_t st);
 // dst[0] = a, dst[1] = b (values that a captured step graph reads at replay)
 void set_pair(double* dst, double a, double b, hipStream_t st);
 template <typename TA>

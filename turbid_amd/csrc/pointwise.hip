@@ -1,9 +1,15 @@
 // Elementwise and reduction kernels of the substage (see pointwise.h).
 #include "pointwise.h"
+#include "group.h"
 
 #include <algorithm>
 
 namespace tbd {
+
+Recorder*& Recorder::active() {
+  thread_local Recorder* r = nullptr;
+  return r;
+}
 
 static constexpr int kNT = 256;
 
@@ -15,10 +21,26 @@ static inline dim3 grid1d(long long n, int per = 1) {
 
 #define GRID_STRIDE(i, n) \
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+// the same over the member's own grid (group.h)
+#define MEMBER_STRIDE(i, n) \
+  for (long long i = (long long)b.x * blockDim.x + threadIdx.x; i < (n); i += (long long)b.gx * blockDim.x)
 
 template <typename T>
-__global__ void k_products_conv(const T* __restrict__ U, const T* __restrict__ g, T* __restrict__ adv, long long R) {
-  GRID_STRIDE(e, R) {
+struct PConvA {
+  const T* U;
+  const T* g;
+  T* adv;
+  long long R;
+};
+template <typename T, bool G>
+__global__ void k_products_conv(PConvA<T> a0, const Group* grp) {
+  Blk b;
+  const PConvA<T>& A = member<G>(a0, grp, b);
+  const T* __restrict__ U = A.U;
+  const T* __restrict__ g = A.g;
+  T* __restrict__ adv = A.adv;
+  const long long R = A.R;
+  MEMBER_STRIDE(e, R) {
     const T a = U[e], b = U[R + e], w = U[2 * R + e];
     adv[e] = a * g[e] + b * g[R + e] + w * g[2 * R + e];
     adv[R + e] = a * g[3 * R + e] + b * g[4 * R + e] + w * g[5 * R + e];
@@ -27,13 +49,23 @@ __global__ void k_products_conv(const T* __restrict__ U, const T* __restrict__ g
 }
 template <typename T>
 void products_conv(const T* uvw, const T* g, T* adv, long long R, hipStream_t st) {
-  hipLaunchKernelGGL(k_products_conv<T>, grid1d(R), dim3(kNT), 0, st, uvw, g, adv, R);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_products_conv<T, false>, k_products_conv<T, true>, grid1d(R), dim3(kNT), 0, st, PConvA<T>{uvw, g, adv, R});
 }
 
 template <typename T>
-__global__ void k_products_div(const T* __restrict__ U, T* __restrict__ o, long long R) {
-  GRID_STRIDE(e, R) {
+struct PDivA {
+  const T* U;
+  T* o;
+  long long R;
+};
+template <typename T, bool G>
+__global__ void k_products_div(PDivA<T> a0, const Group* grp) {
+  Blk b;
+  const PDivA<T>& A = member<G>(a0, grp, b);
+  const T* __restrict__ U = A.U;
+  T* __restrict__ o = A.o;
+  const long long R = A.R;
+  MEMBER_STRIDE(e, R) {
     const T a = U[e], b = U[R + e], w = U[2 * R + e];
     o[e] = a * a;
     o[R + e] = a * b;
@@ -45,27 +77,54 @@ __global__ void k_products_div(const T* __restrict__ U, T* __restrict__ o, long 
 }
 template <typename T>
 void products_div(const T* uvw, T* prod6, long long R, hipStream_t st) {
-  hipLaunchKernelGGL(k_products_div<T>, grid1d(R), dim3(kNT), 0, st, uvw, prod6, R);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_products_div<T, false>, k_products_div<T, true>, grid1d(R), dim3(kNT), 0, st, PDivA<T>{uvw, prod6, R});
 }
 
 template <typename T>
-__global__ void k_scalar_conv(const T* __restrict__ U, T su, T sv, T sw, const T* __restrict__ dc, T* __restrict__ adv,
-                              long long R) {
-  GRID_STRIDE(e, R) {
+struct SConvA {
+  const T* U;
+  T su, sv, sw;
+  const T* dc;
+  T* adv;
+  long long R;
+};
+template <typename T, bool G>
+__global__ void k_scalar_conv(SConvA<T> a0, const Group* grp) {
+  Blk b;
+  const SConvA<T>& A = member<G>(a0, grp, b);
+  const T* __restrict__ U = A.U;
+  const T* __restrict__ dc = A.dc;
+  T* __restrict__ adv = A.adv;
+  const T su = A.su, sv = A.sv, sw = A.sw;
+  const long long R = A.R;
+  MEMBER_STRIDE(e, R) {
     adv[e] = (U[e] + su) * dc[e] + (U[R + e] + sv) * dc[R + e] + (U[2 * R + e] + sw) * dc[2 * R + e];
   }
 }
 template <typename T>
 void scalar_products_conv(const T* uvw, T su, T sv, T sw, const T* dc, T* adv, long long R, hipStream_t st) {
-  hipLaunchKernelGGL(k_scalar_conv<T>, grid1d(R), dim3(kNT), 0, st, uvw, su, sv, sw, dc, adv, R);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_scalar_conv<T, false>, k_scalar_conv<T, true>, grid1d(R), dim3(kNT), 0, st,
+          SConvA<T>{uvw, su, sv, sw, dc, adv, R});
 }
 
 template <typename T, typename TC>
-__global__ void k_scalar_div(const T* __restrict__ U, T su, T sv, T sw, const TC* __restrict__ c, T* __restrict__ o,
-                             long long R) {
-  GRID_STRIDE(e, R) {
+struct SDivA {
+  const T* U;
+  T su, sv, sw;
+  const TC* c;
+  T* o;
+  long long R;
+};
+template <typename T, typename TC, bool G>
+__global__ void k_scalar_div(SDivA<T, TC> a0, const Group* grp) {
+  Blk b;
+  const SDivA<T, TC>& A = member<G>(a0, grp, b);
+  const T* __restrict__ U = A.U;
+  const TC* __restrict__ c = A.c;
+  T* __restrict__ o = A.o;
+  const T su = A.su, sv = A.sv, sw = A.sw;
+  const long long R = A.R;
+  MEMBER_STRIDE(e, R) {
     const T cv = (T)c[e];
     o[e] = cv * (U[e] + su);
     o[R + e] = cv * (U[R + e] + sv);
@@ -74,14 +133,30 @@ __global__ void k_scalar_div(const T* __restrict__ U, T su, T sv, T sw, const TC
 }
 template <typename T, typename TC>
 void scalar_products_div(const T* uvw, T su, T sv, T sw, const TC* c, T* out3, long long R, hipStream_t st) {
-  hipLaunchKernelGGL((k_scalar_div<T, TC>), grid1d(R), dim3(kNT), 0, st, uvw, su, sv, sw, c, out3, R);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_scalar_div<T, TC, false>, k_scalar_div<T, TC, true>, grid1d(R), dim3(kNT), 0, st,
+          SDivA<T, TC>{uvw, su, sv, sw, c, out3, R});
 }
 
 template <typename TA>
-__global__ void k_h_update(double2* __restrict__ h, const TA* __restrict__ adv, const double2* __restrict__ c,
-                           HUpd u, int nf, long long S) {
-  GRID_STRIDE(i, nf * S) {
+struct HUpdA {
+  double2* h;
+  const TA* adv;
+  const double2* c;
+  HUpd u;
+  int nf;
+  long long S;
+};
+template <typename TA, bool G>
+__global__ void k_h_update(HUpdA<TA> a0, const Group* grp) {
+  Blk b;
+  const HUpdA<TA>& A = member<G>(a0, grp, b);
+  double2* __restrict__ h = A.h;
+  const TA* __restrict__ adv = A.adv;
+  const double2* __restrict__ c = A.c;
+  const HUpd& u = A.u;
+  const int nf = A.nf;
+  const long long S = A.S;
+  MEMBER_STRIDE(i, nf * S) {
     const int f = (int)(i / S);
     const long long e = i - (long long)f * S;
     double2 hv = h[i];
@@ -108,8 +183,8 @@ __global__ void k_h_update(double2* __restrict__ h, const TA* __restrict__ adv, 
 }
 template <typename TA>
 void h_update(double2* h, const TA* adv, const double2* c, const HUpd& u, int nf, long long S, hipStream_t st) {
-  hipLaunchKernelGGL(k_h_update<TA>, grid1d(nf * S), dim3(kNT), 0, st, h, adv, c, u, nf, S);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_h_update<TA, false>, k_h_update<TA, true>, grid1d(nf * S), dim3(kNT), 0, st,
+          HUpdA<TA>{h, adv, c, u, nf, S});
 }
 
 // lambda^3 - I1 lambda^2 + I2 lambda - I3 = 0; with lambda = t + I1/3:
@@ -157,8 +232,42 @@ void set_pair(double* dst, double a, double b, hipStream_t st) {
   TBD_HIP(hipGetLastError());
 }
 
-__global__ void k_bc_carrier(double2* __restrict__ rhs, BcArgs a) {
-  GRID_STRIDE(j, 3 * a.P) {
+struct BcCA {
+  double2* rhs;
+  BcArgs a;
+};
+struct PairsA {
+  double* dst[kMaxMembers];
+  double v[2 * kMaxMembers];
+  int n;
+};
+__global__ void k_set_pairs(PairsA a) {
+  const int i = threadIdx.x;
+  if (i < a.n) {
+    a.dst[i][0] = a.v[2 * i];
+    a.dst[i][1] = a.v[2 * i + 1];
+  }
+}
+void set_pairs(double* const* dst, const double* v, int n, hipStream_t st) {
+  TBD_CHECK(n >= 1 && n <= kMaxMembers, "set_pairs > between 1 and kMaxMembers pairs");
+  PairsA a;
+  for (int i = 0; i < n; ++i) {
+    a.dst[i] = dst[i];
+    a.v[2 * i] = v[2 * i];
+    a.v[2 * i + 1] = v[2 * i + 1];
+  }
+  a.n = n;
+  hipLaunchKernelGGL(k_set_pairs, dim3(1), dim3(64), 0, st, a);
+  TBD_HIP(hipGetLastError());
+}
+
+template <bool G>
+__global__ void k_bc_carrier(BcCA a0, const Group* grp) {
+  Blk b;
+  const BcCA& A = member<G>(a0, grp, b);
+  double2* __restrict__ rhs = A.rhs;
+  const BcArgs& a = A.a;
+  MEMBER_STRIDE(j, 3 * a.P) {
     const int f = (int)(j / a.P);
     const long long col = j - (long long)f * a.P;
     const double s = a.s, k = a.k;
@@ -176,37 +285,67 @@ __global__ void k_bc_carrier(double2* __restrict__ rhs, BcArgs a) {
   }
 }
 void bc_carrier(double2* rhs, const BcArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_bc_carrier, grid1d(3 * a.P), dim3(kNT), 0, st, rhs, a);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_bc_carrier<false>, k_bc_carrier<true>, grid1d(3 * a.P), dim3(kNT), 0, st, BcCA{rhs, a});
 }
 
-__global__ void k_bc_scalar(double2* __restrict__ rhs, int nz, long long P, const double2* __restrict__ ehat,
-                            const double2* __restrict__ c, double ws, double cbar, double re, double sc, int erosion) {
-  GRID_STRIDE(col, P) {
+struct BcSA {
+  double2* rhs;
+  int nz;
+  long long P;
+  const double2* ehat;
+  const double2* c;
+  double ws, cbar, re, sc;
+  int erosion;
+};
+template <bool G>
+__global__ void k_bc_scalar(BcSA a0, const Group* grp) {
+  Blk b;
+  const BcSA& A = member<G>(a0, grp, b);
+  double2* __restrict__ rhs = A.rhs;
+  const double2* __restrict__ ehat = A.ehat;
+  const double2* __restrict__ c = A.c;
+  const int nz = A.nz, erosion = A.erosion;
+  const long long P = A.P;
+  const double ws = A.ws, cbar = A.cbar, re = A.re, sc = A.sc;
+  MEMBER_STRIDE(col, P) {
     double2* r = rhs + col * nz;
-    double2 b = make_double2(0.0, 0.0);
+    double2 bv = make_double2(0.0, 0.0);
     if (erosion) {
       const double2 e = ehat[col], cv = c[col * nz];
       double2 v = make_double2(e.x - ws * cv.x * cbar, e.y - ws * cv.y * cbar);
       const double scale = 10E+6;   // BOUNDARY_COEF_SCALE (common.h:55)
       v = make_double2(-1.0 * v.x * re * sc / cbar * scale, -1.0 * v.y * re * sc / cbar * scale);
-      b = v;
+      bv = v;
     }
-    r[0] = b;
+    r[0] = bv;
     r[nz - 1] = make_double2(0.0, 0.0);
   }
 }
 void bc_scalar(double2* rhs, int nz, long long P, const double2* ehat, const double2* c, double ws, double cbar,
                double re, double sc, int erosion, hipStream_t st) {
-  hipLaunchKernelGGL(k_bc_scalar, grid1d(P), dim3(kNT), 0, st, rhs, nz, P, ehat, c, ws, cbar, re, sc, erosion);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_bc_scalar<false>, k_bc_scalar<true>, grid1d(P), dim3(kNT), 0, st,
+          BcSA{rhs, nz, P, ehat, c, ws, cbar, re, sc, erosion});
 }
 
 template <typename T>
-__global__ void k_erosion(const T* __restrict__ a, const T* __restrict__ b, double* __restrict__ e, long long n,
-                          double re, double tc, double M) {
-  GRID_STRIDE(i, n) {
-    const double x = (double)a[i], y = (double)b[i];
+struct ErosA {
+  const T* a;
+  const T* b;
+  double* e;
+  long long n;
+  double re, tc, M;
+};
+template <typename T, bool G>
+__global__ void k_erosion(ErosA<T> a0, const Group* grp) {
+  Blk b;
+  const ErosA<T>& A = member<G>(a0, grp, b);
+  const T* __restrict__ pa = A.a;
+  const T* __restrict__ pb = A.b;
+  double* __restrict__ e = A.e;
+  const long long n = A.n;
+  const double re = A.re, tc = A.tc, M = A.M;
+  MEMBER_STRIDE(i, n) {
+    const double x = (double)pa[i], y = (double)pb[i];
     double val = sqrt(x * x + y * y) / re;
     if (val < tc) val = 0;
     else val = M * (val / tc - 1);
@@ -216,18 +355,28 @@ __global__ void k_erosion(const T* __restrict__ a, const T* __restrict__ b, doub
 template <typename T>
 void erosion_map(const T* dudz, const T* dvdz, double* e, long long n, double re, double tc, double M,
                  hipStream_t st) {
-  hipLaunchKernelGGL(k_erosion<T>, grid1d(n), dim3(kNT), 0, st, dudz, dvdz, e, n, re, tc, M);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_erosion<T, false>, k_erosion<T, true>, grid1d(n), dim3(kNT), 0, st,
+          ErosA<T>{dudz, dvdz, e, n, re, tc, M});
 }
 
 // One workgroup per z-plane; the plane statistics are FP64 sums.  A plane
 // with no negative entries is left untouched after the first reduction (each
 // further pass would add 0 to the positive entries and zero the zeros).
-__global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long long plane, int* touched,
-                                                 const int* gate) {
-  if (gate && *gate == 0) return;
+struct FiltA {
+  double* c;
+  long long plane;
+  int* touched;
+  const int* gate;
+};
+template <bool G>
+__global__ __launch_bounds__(1024) void k_filter(FiltA a0, const Group* grp) {
+  Blk b;
+  const FiltA& A = member<G>(a0, grp, b);
+  const long long plane = A.plane;
+  int* touched = A.touched;
+  if (A.gate && *A.gate == 0) return;
   __shared__ double red[4][16];
-  double* pl = c + (long long)blockIdx.x * plane;
+  double* pl = A.c + (long long)b.x * plane;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int it = 0; it < 5; ++it) {
     double pos = 0, npos = 0, neg = 0, nneg = 0;
@@ -264,8 +413,7 @@ __global__ __launch_bounds__(1024) void k_filter(double* __restrict__ c, long lo
   }
 }
 void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream_t st, const int* gate) {
-  hipLaunchKernelGGL(k_filter, dim3(nz), dim3(1024), 0, st, c, (long long)nx * ny, touched, gate);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_filter<false>, k_filter<true>, dim3(nz), dim3(1024), 0, st, FiltA{c, (long long)nx * ny, touched, gate});
 }
 
 // Smallest and largest value of x: per-block partials, then one block that
@@ -274,10 +422,21 @@ void bartnicki_filter(double* c, int nx, int ny, int nz, int* touched, hipStream
 static constexpr int kMinMaxBlocks = 512;
 static constexpr double kNegMargin = 1e-4;
 template <typename T>
-__global__ __launch_bounds__(256) void k_minmax_part(const T* __restrict__ x, long long n, double* part) {
+struct MmA {
+  const T* x;
+  long long n;
+  double* part;
+};
+template <typename T, bool G>
+__global__ __launch_bounds__(256) void k_minmax_part(MmA<T> a0, const Group* grp) {
+  Blk bk;
+  const MmA<T>& A = member<G>(a0, grp, bk);
+  const T* __restrict__ x = A.x;
+  const long long n = A.n;
+  double* part = A.part;
   __shared__ double lo[4], hi[4];
   double a = 1e300, b = -1e300;
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+  for (long long i = bk.x * 256LL + threadIdx.x; i < n; i += (long long)bk.gx * 256) {
     const double v = (double)x[i];
     a = fmin(a, v);
     b = fmax(b, v);
@@ -292,11 +451,22 @@ __global__ __launch_bounds__(256) void k_minmax_part(const T* __restrict__ x, lo
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < 4; ++w) { a = fmin(a, lo[w]); b = fmax(b, hi[w]); }
-    part[2 * blockIdx.x] = fmin(a, lo[0]);
-    part[2 * blockIdx.x + 1] = fmax(b, hi[0]);
+    part[2 * bk.x] = fmin(a, lo[0]);
+    part[2 * bk.x + 1] = fmax(b, hi[0]);
   }
 }
-__global__ __launch_bounds__(64) void k_minmax_final(const double* __restrict__ part, int nb, int* need) {
+struct MmFA {
+  const double* part;
+  int nb;
+  int* need;
+};
+template <bool G>
+__global__ __launch_bounds__(64) void k_minmax_final(MmFA a0, const Group* grp) {
+  Blk bk;
+  const MmFA& A = member<G>(a0, grp, bk);
+  const double* __restrict__ part = A.part;
+  const int nb = A.nb;
+  int* need = A.need;
   double a = 1e300, b = -1e300;
   for (int i = threadIdx.x; i < nb; i += 64) {
     a = fmin(a, part[2 * i]);
@@ -312,17 +482,26 @@ __global__ __launch_bounds__(64) void k_minmax_final(const double* __restrict__ 
 template <typename T>
 void negativity_check(const T* x, long long n, double* part, int* need, hipStream_t st) {
   const int nb = (int)std::min<long long>(kMinMaxBlocks, std::max<long long>(1, (n + 255) / 256));
-  hipLaunchKernelGGL(k_minmax_part<T>, dim3(nb), dim3(256), 0, st, x, n, part);
-  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, part, nb, need);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_minmax_part<T, false>, k_minmax_part<T, true>, dim3(nb), dim3(256), 0, st, MmA<T>{x, n, part});
+  glaunch(k_minmax_final<false>, k_minmax_final<true>, dim3(1), dim3(64), 0, st, MmFA{part, nb, need});
 }
 template void negativity_check<float>(const float*, long long, double*, int*, hipStream_t);
 template void negativity_check<double>(const double*, long long, double*, int*, hipStream_t);
 
-__global__ void k_kx0_round_trip(double2* __restrict__ s, int nyc, int nz, const int* gate) {
-  if (gate && *gate != 0) return;
+struct Kx0A {
+  double2* s;
+  int nyc, nz;
+  const int* gate;
+};
+template <bool G>
+__global__ void k_kx0_round_trip(Kx0A a0, const Group* grp) {
+  Blk bk;
+  const Kx0A& A = member<G>(a0, grp, bk);
+  if (A.gate && *A.gate != 0) return;
+  double2* __restrict__ s = A.s;
+  const int nyc = A.nyc, nz = A.nz;
   const int half = nyc / 2;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (half + 1) * nz; i += gridDim.x * blockDim.x) {
+  for (int i = bk.x * blockDim.x + threadIdx.x; i < (half + 1) * nz; i += bk.gx * blockDim.x) {
     const int iy = i / nz, z = i - iy * nz;
     double2* a = s + (long long)iy * nz + z;
     if (iy == 0) {
@@ -340,8 +519,8 @@ __global__ void k_kx0_round_trip(double2* __restrict__ s, int nyc, int nz, const
   }
 }
 void kx0_round_trip(double2* s, int nyc, int nz, const int* gate, hipStream_t st) {
-  hipLaunchKernelGGL(k_kx0_round_trip, dim3(((nyc / 2 + 1) * nz + 255) / 256), dim3(256), 0, st, s, nyc, nz, gate);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_kx0_round_trip<false>, k_kx0_round_trip<true>, dim3(((nyc / 2 + 1) * nz + 255) / 256), dim3(256), 0, st,
+          Kx0A{s, nyc, nz, gate});
 }
 
 // Max-norm and CFL diagnostics: per-block partial maxima (z planes strided
@@ -353,15 +532,30 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 template <typename T>
-__global__ __launch_bounds__(kNT) void k_diag_part(const T* __restrict__ U, long long R, int plane, int nz, double glx,
-                                                    double gly, const double* __restrict__ glz,
-                                                    double* __restrict__ part) {
+struct DiagA {
+  const T* U;
+  long long R;
+  int plane, nz;
+  double glx, gly;
+  const double* glz;
+  double* part;
+};
+template <typename T, bool G>
+__global__ __launch_bounds__(kNT) void k_diag_part(DiagA<T> a0, const Group* grp) {
+  Blk bk;
+  const DiagA<T>& A = member<G>(a0, grp, bk);
+  const T* __restrict__ U = A.U;
+  const long long R = A.R;
+  const int plane = A.plane, nz = A.nz;
+  const double glx = A.glx, gly = A.gly;
+  const double* __restrict__ glz = A.glz;
+  double* __restrict__ part = A.part;
   double m[kDiagVals] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int z = blockIdx.y; z < nz; z += gridDim.y) {
+  for (int z = bk.y; z < nz; z += bk.gy) {
     const double gz = glz[z];
     const T* u = U + (long long)z * plane;
     double a0 = 0, a1 = 0, a2 = 0, cf = 0;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < plane; e += gridDim.x * blockDim.x) {
+    for (int e = bk.x * blockDim.x + threadIdx.x; e < plane; e += bk.gx * blockDim.x) {
       const double a = fabs((double)u[e]), b = fabs((double)u[R + e]), w = fabs((double)u[2 * R + e]);
       a0 = fmax(a0, a);
       a1 = fmax(a1, b);
@@ -385,10 +579,21 @@ __global__ __launch_bounds__(kNT) void k_diag_part(const T* __restrict__ U, long
   if (threadIdx.x < kDiagVals) {
     double v = 0;
     for (int w = 0; w < kNT / 64; ++w) v = fmax(v, red[threadIdx.x][w]);
-    part[(long long)(blockIdx.y * gridDim.x + blockIdx.x) * kDiagVals + threadIdx.x] = v;
+    part[(long long)(bk.y * bk.gx + bk.x) * kDiagVals + threadIdx.x] = v;
   }
 }
-__global__ __launch_bounds__(kNT) void k_diag_final(const double* __restrict__ part, int nblk, double* feat) {
+struct DiagFA {
+  const double* part;
+  int nblk;
+  double* feat;
+};
+template <bool G>
+__global__ __launch_bounds__(kNT) void k_diag_final(DiagFA a0, const Group* grp) {
+  Blk bk;
+  const DiagFA& A = member<G>(a0, grp, bk);
+  const double* __restrict__ part = A.part;
+  const int nblk = A.nblk;
+  double* feat = A.feat;
   __shared__ double red[kDiagVals][kNT / 64];
   for (int i = 0; i < kDiagVals; ++i) {
     double v = 0;
@@ -411,10 +616,9 @@ void diagnostics(const T* uvw, long long R, int nx, int ny, int nz, double glx, 
   nbx = nbx < 1 ? 1 : (nbx > 32 ? 32 : nbx);
   int nby = kDiagMaxBlocks / nbx;
   nby = nby > nz ? nz : (nby < 1 ? 1 : nby);
-  hipLaunchKernelGGL(k_diag_part<T>, dim3(nbx, nby), dim3(kNT), 0, st, uvw, R, plane, nz, glx, gly, glz, part);
-  TBD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kNT), 0, st, (const double*)part, nbx * nby, feat);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_diag_part<T, false>, k_diag_part<T, true>, dim3(nbx, nby), dim3(kNT), 0, st,
+          DiagA<T>{uvw, R, plane, nz, glx, gly, glz, part});
+  glaunch(k_diag_final<false>, k_diag_final<true>, dim3(1), dim3(kNT), 0, st, DiagFA{part, nbx * nby, feat});
 }
 
 // one workgroup per z: reductions over the P spectral columns of that plane
@@ -460,7 +664,20 @@ void phase_stats(const double2* uvw, const double2* c, long long S, int nz, int 
   TBD_HIP(hipGetLastError());
 }
 
-__global__ void k_trapz(const double2* __restrict__ c, const double* __restrict__ z, int nz, double* out) {
+struct TrapzA {
+  const double2* c;
+  const double* z;
+  int nz;
+  double* out;
+};
+template <bool G>
+__global__ void k_trapz(TrapzA a0, const Group* grp) {
+  Blk bk;
+  const TrapzA& A = member<G>(a0, grp, bk);
+  const double2* __restrict__ c = A.c;
+  const double* __restrict__ z = A.z;
+  const int nz = A.nz;
+  double* out = A.out;
   __shared__ double part[64];
   double s = 0.0;
   for (int i = threadIdx.x; i + 1 < nz; i += blockDim.x) s += (z[i + 1] - z[i]) * (c[i].x + c[i + 1].x) / 2;
@@ -474,8 +691,7 @@ __global__ void k_trapz(const double2* __restrict__ c, const double* __restrict_
   }
 }
 void trapz_mean(const double2* c, const double* gridz, int nz, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_trapz, dim3(1), dim3(256), 0, st, c, gridz, nz, out);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_trapz<false>, k_trapz<true>, dim3(1), dim3(256), 0, st, TrapzA{c, gridz, nz, out});
 }
 
 __global__ void k_gather_real(const double2* __restrict__ src, const long long* __restrict__ idx, int n,
@@ -488,27 +704,89 @@ void gather_real(const double2* src, const long long* idx, int n, double* dst, h
   TBD_HIP(hipGetLastError());
 }
 
-__global__ void k_zero(unsigned long long* p, long long n) {
-  GRID_STRIDE(i, n) p[i] = 0ULL;
-}
-__global__ void k_zero_bytes(unsigned char* p, long long n) {
-  GRID_STRIDE(i, n) p[i] = 0;
+template <typename W>
+struct ZeroA {
+  W* p;
+  long long n;
+};
+template <typename W, bool G>
+__global__ void k_zero(ZeroA<W> a0, const Group* grp) {
+  Blk b;
+  const ZeroA<W>& A = member<G>(a0, grp, b);
+  W* p = A.p;
+  MEMBER_STRIDE(i, A.n) p[i] = (W)0;
 }
 void fill_zero(void* p, size_t bytes, hipStream_t st) {
+  typedef unsigned long long u64;
+  typedef unsigned char u8;
   if (bytes % 8 == 0) {
-    hipLaunchKernelGGL(k_zero, grid1d(bytes / 8), dim3(kNT), 0, st, (unsigned long long*)p, (long long)(bytes / 8));
+    glaunch(k_zero<u64, false>, k_zero<u64, true>, grid1d(bytes / 8), dim3(kNT), 0, st,
+            ZeroA<u64>{(u64*)p, (long long)(bytes / 8)});
   } else {
-    hipLaunchKernelGGL(k_zero_bytes, grid1d(bytes), dim3(kNT), 0, st, (unsigned char*)p, (long long)bytes);
+    glaunch(k_zero<u8, false>, k_zero<u8, true>, grid1d(bytes), dim3(kNT), 0, st, ZeroA<u8>{(u8*)p, (long long)bytes});
   }
-  TBD_HIP(hipGetLastError());
 }
 
-__global__ void k_scaled_real(const double2* __restrict__ src, double a, double b, double* out) {
-  if (threadIdx.x == 0) *out = (a * src[0].x) * b;
+struct ScRA {
+  const double2* src;
+  double a, b;
+  double* out;
+};
+template <bool G>
+__global__ void k_scaled_real(ScRA a0, const Group* grp) {
+  Blk bk;
+  const ScRA& A = member<G>(a0, grp, bk);
+  if (threadIdx.x == 0) *A.out = (A.a * A.src[0].x) * A.b;
 }
 void scaled_real(const double2* src, double a, double b, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_scaled_real, dim3(1), dim3(64), 0, st, src, a, b, out);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_scaled_real<false>, k_scaled_real<true>, dim3(1), dim3(64), 0, st, ScRA{src, a, b, out});
+}
+
+// dst[i] = src[i] for i < n, then dst[10 + j] = wall[src6[j]] for j < 6 (the
+// step's features: the diagnostics' 50 values and the mean-mode wall gradients)
+struct FeatA {
+  double* dst;
+  const double* src;
+  int n;
+  const double* wall;
+  int src6[6];
+};
+template <bool G>
+__global__ void k_features(FeatA a0, const Group* grp) {
+  Blk bk;
+  const FeatA& A = member<G>(a0, grp, bk);
+  const int i = threadIdx.x;
+  if (i < A.n) A.dst[i] = A.src[i];
+  __syncthreads();
+  if (i < 6) A.dst[10 + i] = A.wall[A.src6[i]];
+}
+void collect_features(double* dst, const double* src, int n, const double* wall, const int* src6, hipStream_t st) {
+  TBD_CHECK(n >= 16 && n <= 64, "collect_features > n must be in 16..64");
+  FeatA a;
+  a.dst = dst;
+  a.src = src;
+  a.n = n;
+  a.wall = wall;
+  for (int j = 0; j < 6; ++j) a.src6[j] = src6[j];
+  glaunch(k_features<false>, k_features<true>, dim3(1), dim3(64), 0, st, a);
+}
+
+// dst = src, n bytes (a multiple of 4)
+struct CopyA {
+  unsigned int* dst;
+  const unsigned int* src;
+  long long n;
+};
+template <bool G>
+__global__ void k_copy4(CopyA a0, const Group* grp) {
+  Blk b;
+  const CopyA& A = member<G>(a0, grp, b);
+  MEMBER_STRIDE(i, A.n) A.dst[i] = A.src[i];
+}
+void copy_words(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  TBD_CHECK(bytes % 4 == 0, "copy_words > size must be a multiple of 4 bytes");
+  glaunch(k_copy4<false>, k_copy4<true>, grid1d(bytes / 4), dim3(kNT), 0, st,
+          CopyA{(unsigned int*)dst, (const unsigned int*)src, (long long)(bytes / 4)});
 }
 
 __global__ void k_copy_d2(double2* __restrict__ d, const double2* __restrict__ s, long long n) {

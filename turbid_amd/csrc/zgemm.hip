@@ -18,6 +18,7 @@
 // overwritten with them (the fused Helmholtz solve needs one LDS panel, not
 // two, and two workgroups fit per CU).
 #include "zgemm.h"
+#include "group.h"
 
 namespace tbd {
 
@@ -533,14 +534,22 @@ __device__ __forceinline__ double2 kcol(const double* kx, const double* ky, long
 // Panel load, buffered contraction written over the panel, then the epilogue
 // as one coalesced pass (row fastest) that re-reads the input column where the
 // mode needs it.
-template <typename T, typename TX, typename TY, int DEPTH>
-__global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH == 1 ? 4 : 2)) void k_zgemm(ZArgs a, Panel pn) {
+struct ZL {
+  ZArgs a;
+  Panel pn;
+};
+template <typename T, typename TX, typename TY, int DEPTH, bool G>
+__global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH == 1 ? 4 : 2)) void k_zgemm(ZL l0, const Group* grp) {
+  Blk bk;
+  const ZL& L = member<G>(l0, grp, bk);
+  const ZArgs& a = L.a;
+  const Panel& pn = L.pn;
   extern __shared__ char smem_raw[];
   C2<T>* xs = reinterpret_cast<C2<T>*>(smem_raw);
   const int nz = a.nz;
   const int ldx = pn.ldx;
-  const int f = blockIdx.y;
-  const long long c0 = (long long)blockIdx.x * pn.ncol;
+  const int f = bk.y;
+  const long long c0 = (long long)bk.x * pn.ncol;
   const int ncol = (int)min((long long)pn.ncol, a.P - c0);
   const TX* x = zx<TX>(a, f) + c0 * nz;
   const C2<T> zero = mk<T>((T)0, (T)0);
@@ -643,10 +652,9 @@ void zgemm(const ZArgs& a, int nfield, hipStream_t st) {
   TBD_CHECK(depth <= kMaxU, "zgemm > nz too large for the register-buffered contraction");
   dim3 grid(ceil_div(a.P, pn.ncol), nfield);
   if (depth <= 1)
-    hipLaunchKernelGGL((k_zgemm<T, TX, TY, 1>), grid, dim3(kNT), lds, st, a, pn);
+    glaunch(k_zgemm<T, TX, TY, 1, false>, k_zgemm<T, TX, TY, 1, true>, grid, dim3(kNT), lds, st, ZL{a, pn});
   else
-    hipLaunchKernelGGL((k_zgemm<T, TX, TY, 2>), grid, dim3(kNT), lds, st, a, pn);
-  TBD_HIP(hipGetLastError());
+    glaunch(k_zgemm<T, TX, TY, 2, false>, k_zgemm<T, TX, TY, 2, true>, grid, dim3(kNT), lds, st, ZL{a, pn});
 }
 
 template void zgemm<double, double2, double2>(const ZArgs&, int, hipStream_t);
@@ -656,13 +664,21 @@ template void zgemm<float, float2, float2>(const ZArgs&, int, hipStream_t);
 // ---------------------------------------------------------------------------
 // fused Helmholtz: one LDS panel, both contractions in buffered mode
 // ---------------------------------------------------------------------------
-template <int DEPTH>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HArgs a, Panel pn) {
+struct HL {
+  HArgs a;
+  Panel pn;
+};
+template <int DEPTH, bool G>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HL l0, const Group* grp) {
+  Blk bk;
+  const HL& L = member<G>(l0, grp, bk);
+  const HArgs& a = L.a;
+  const Panel& pn = L.pn;
   extern __shared__ char smem_raw[];
   const int nz = a.nz;
   const int ldx = pn.ldx;
   double2* xs = reinterpret_cast<double2*>(smem_raw);
-  const int fi = blockIdx.y;
+  const int fi = bk.y;
   const double* vinv = fi == 0 ? a.f[0].vinv : (fi == 1 ? a.f[1].vinv : a.f[2].vinv);
   const double* vnorm = fi == 0 ? a.f[0].vnorm : (fi == 1 ? a.f[1].vnorm : a.f[2].vnorm);
   const double* lam = fi == 0 ? a.f[0].lambda : (fi == 1 ? a.f[1].lambda : a.f[2].lambda);
@@ -671,7 +687,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HArgs a, 
   const int nullrow = fi == 0 ? a.f[0].nullrow : (fi == 1 ? a.f[1].nullrow : a.f[2].nullrow);
   const double2* x = fi == 0 ? a.f[0].x : (fi == 1 ? a.f[1].x : a.f[2].x);
   double2* y = fi == 0 ? a.f[0].y : (fi == 1 ? a.f[1].y : a.f[2].y);
-  const long long c0 = (long long)blockIdx.x * pn.ncol;
+  const long long c0 = (long long)bk.x * pn.ncol;
   const int ncol = (int)min((long long)pn.ncol, (long long)a.P - c0);
   __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
   for (int c = threadIdx.x; c < pn.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
@@ -734,10 +750,9 @@ void helmholtz(const HArgs& a, int nfield, hipStream_t st) {
   TBD_CHECK(lds + sizeof(double2) * kSplitMaxCol <= 160 * 1024, "helmholtz > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, pn.ncol), nfield);
   if (depth <= 1)
-    hipLaunchKernelGGL(k_helmholtz<1>, grid, dim3(kNT), lds, st, a, pn);
+    glaunch(k_helmholtz<1, false>, k_helmholtz<1, true>, grid, dim3(kNT), lds, st, HL{a, pn});
   else
-    hipLaunchKernelGGL(k_helmholtz<2>, grid, dim3(kNT), lds, st, a, pn);
-  TBD_HIP(hipGetLastError());
+    glaunch(k_helmholtz<2, false>, k_helmholtz<2, true>, grid, dim3(kNT), lds, st, HL{a, pn});
 }
 
 // ---------------------------------------------------------------------------
@@ -844,8 +859,17 @@ __device__ __forceinline__ void store_columns(const double2* xs, int ldx, int nc
   }
 }
 
-template <int DEPTH>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HArgs a, Panel p1, Panel p2) {
+struct HSL {
+  HArgs a;
+  Panel p1, p2;
+};
+template <int DEPTH, bool G>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HSL l0, const Group* grp) {
+  Blk bk;
+  const HSL& L = member<G>(l0, grp, bk);
+  const HArgs& a = L.a;
+  const Panel& p1 = L.p1;
+  const Panel& p2 = L.p2;
   extern __shared__ char smem_raw[];
   __shared__ double2 tfix[2][kSplitMaxCol];
   const int nz = a.nz;
@@ -856,7 +880,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HAr
   const double* lam = a.f[0].lambda;
   const double sigma = a.f[0].sigma;
   const int nullrow = a.f[0].nullrow;
-  const long long c0 = (long long)blockIdx.x * p1.ncol;
+  const long long c0 = (long long)bk.x * p1.ncol;
   const int ncol = (int)min((long long)p1.ncol, (long long)a.P - c0);
   __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
   for (int c = threadIdx.x; c < p1.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
@@ -928,9 +952,9 @@ void helmholtz_split(const HArgs& a, hipStream_t st) {
   TBD_CHECK(lds + sizeof(double2) * 3 * kSplitMaxCol <= 160 * 1024, "helmholtz_split > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, p1.ncol), 1);
   if (depth <= 1)
-    hipLaunchKernelGGL(k_helmholtz_split<1>, grid, dim3(kNT), lds, st, a, p1, p2);
+    glaunch(k_helmholtz_split<1, false>, k_helmholtz_split<1, true>, grid, dim3(kNT), lds, st, HSL{a, p1, p2});
   else
-    hipLaunchKernelGGL(k_helmholtz_split<2>, grid, dim3(kNT), lds, st, a, p1, p2);
+    glaunch(k_helmholtz_split<2, false>, k_helmholtz_split<2, true>, grid, dim3(kNT), lds, st, HSL{a, p1, p2});
   TBD_HIP(hipGetLastError());
 }
 
@@ -951,8 +975,18 @@ __host__ inline Panel with_ncg(Panel pn, int ncg, int ldx) {
   return pn;
 }
 
-template <int DEPTH, bool SPLIT>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArgs a, Panel pd, Panel p1, Panel p2) {
+struct RHL {
+  RHArgs a;
+  Panel pd, p1, p2;
+};
+template <int DEPTH, bool SPLIT, bool G>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHL l0, const Group* grp) {
+  Blk bk;
+  const RHL& L = member<G>(l0, grp, bk);
+  const RHArgs& a = L.a;
+  const Panel& pd = L.pd;
+  const Panel& p1 = L.p1;
+  const Panel& p2 = L.p2;
   extern __shared__ char smem_raw[];
   __shared__ double2 bcv[2][kSplitMaxCol];
   __shared__ double2 tfix[2][kSplitMaxCol];
@@ -960,7 +994,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   const int M = (nz - 1) / 2;
   const int ldx = pd.ldx;
   double2* xs = reinterpret_cast<double2*>(smem_raw);
-  const int fi = blockIdx.y;
+  const int fi = bk.y;
   const RHField& F = fi == 0 ? a.f[0] : (fi == 1 ? a.f[1] : a.f[2]);
   const double2* x = F.x;
   double2* h = F.h;
@@ -974,7 +1008,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHArg
   for (int i = threadIdx.x; i < nz + 1; i += kNT) lam[i] = i < nz ? F.lambda[i] : 0.0;
   for (int i = threadIdx.x; i < 2 * nz; i += kNT) fixs[i] = fix[i];
   const double sigma = F.sigma;
-  const long long c0 = (long long)blockIdx.x * pd.ncol;
+  const long long c0 = (long long)bk.x * pd.ncol;
   const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
   __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
   __shared__ double qv[kSplitMaxCol];    // s1 + kx^2 + ky^2 of each panel column
@@ -1174,20 +1208,37 @@ void rhs_helmholtz(const RHArgs& a, int nfield, hipStream_t st) {
   TBD_CHECK(lds + sizeof(double2) * 5 * kSplitMaxCol <= 160 * 1024, "rhs_helmholtz > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, pd.ncol), nfield);
   if (a.split) {
-    if (depth <= 1) hipLaunchKernelGGL((k_rhs_helmholtz<1, true>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
-    else hipLaunchKernelGGL((k_rhs_helmholtz<2, true>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
+    if (depth <= 1)
+      glaunch(k_rhs_helmholtz<1, true, false>, k_rhs_helmholtz<1, true, true>, grid, dim3(kNT), lds, st,
+              RHL{a, pd, p1, p2});
+    else
+      glaunch(k_rhs_helmholtz<2, true, false>, k_rhs_helmholtz<2, true, true>, grid, dim3(kNT), lds, st,
+              RHL{a, pd, p1, p2});
   } else {
-    if (depth <= 1) hipLaunchKernelGGL((k_rhs_helmholtz<1, false>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
-    else hipLaunchKernelGGL((k_rhs_helmholtz<2, false>), grid, dim3(kNT), lds, st, a, pd, p1, p2);
+    if (depth <= 1)
+      glaunch(k_rhs_helmholtz<1, false, false>, k_rhs_helmholtz<1, false, true>, grid, dim3(kNT), lds, st,
+              RHL{a, pd, p1, p2});
+    else
+      glaunch(k_rhs_helmholtz<2, false, false>, k_rhs_helmholtz<2, false, true>, grid, dim3(kNT), lds, st,
+              RHL{a, pd, p1, p2});
   }
-  TBD_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
 // divergence + pressure solve + projection in one kernel
 // ---------------------------------------------------------------------------
-template <int DEPTH>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArgs a, Panel pd, Panel p1, Panel p2) {
+struct PL {
+  PArgs a;
+  Panel pd, p1, p2;
+};
+template <int DEPTH, bool G>
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PL l0, const Group* grp) {
+  Blk bk;
+  const PL& L = member<G>(l0, grp, bk);
+  const PArgs& a = L.a;
+  const Panel& pd = L.pd;
+  const Panel& p1 = L.p1;
+  const Panel& p2 = L.p2;
   extern __shared__ char smem_raw[];
   __shared__ double2 tfix[2][kSplitMaxCol];
   const int nz = a.nz;
@@ -1195,7 +1246,7 @@ __global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PArg
   const int ldx = pd.ldx;
   const long long S = a.S;
   double2* xs = reinterpret_cast<double2*>(smem_raw);
-  const long long c0 = (long long)blockIdx.x * pd.ncol;
+  const long long c0 = (long long)bk.x * pd.ncol;
   const int ncol = (int)min((long long)pd.ncol, (long long)a.P - c0);
   __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
   for (int c = threadIdx.x; c < pd.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
@@ -1375,9 +1426,10 @@ void pressure_chain(const PArgs& a, hipStream_t st) {
   const size_t lds = sizeof(double2) * pd.ncol * ldx;
   TBD_CHECK(lds + sizeof(double2) * 3 * kSplitMaxCol <= 160 * 1024, "pressure_chain > nz too large for one LDS panel");
   dim3 grid(ceil_div(a.P, pd.ncol), 1);
-  if (depth <= 1) hipLaunchKernelGGL(k_pressure_chain<1>, grid, dim3(kNT), lds, st, a, pd, p1, p2);
-  else hipLaunchKernelGGL(k_pressure_chain<2>, grid, dim3(kNT), lds, st, a, pd, p1, p2);
-  TBD_HIP(hipGetLastError());
+  if (depth <= 1)
+    glaunch(k_pressure_chain<1, false>, k_pressure_chain<1, true>, grid, dim3(kNT), lds, st, PL{a, pd, p1, p2});
+  else
+    glaunch(k_pressure_chain<2, false>, k_pressure_chain<2, true>, grid, dim3(kNT), lds, st, PL{a, pd, p1, p2});
 }
 
 double pressure_chain_flops(int nz, long long P) {
@@ -1412,11 +1464,26 @@ double helmholtz_split_flops(int nz, long long P) {
 // ---------------------------------------------------------------------------
 // wall rows of M x: one wave per column
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_zrows_wall(const double* __restrict__ op, int lda, int nz,
-                                                   const double2* __restrict__ x, long long P,
-                                                   double2* __restrict__ bot, double2* __restrict__ top) {
+struct ZRowsA {
+  const double* op;
+  int lda, nz;
+  const double2* x;
+  long long P;
+  double2* bot;
+  double2* top;
+};
+template <bool G>
+__global__ __launch_bounds__(kNT) void k_zrows_wall(ZRowsA a0, const Group* grp) {
+  Blk bk;
+  const ZRowsA& A = member<G>(a0, grp, bk);
+  const double* __restrict__ op = A.op;
+  const int lda = A.lda, nz = A.nz;
+  const double2* __restrict__ x = A.x;
+  const long long P = A.P;
+  double2* __restrict__ bot = A.bot;
+  double2* __restrict__ top = A.top;
   const int lane = threadIdx.x & 63;
-  const long long col = (long long)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6);
+  const long long col = (long long)bk.x * (kNT / 64) + (threadIdx.x >> 6);
   if (col >= P) return;
   const double* r0 = op;
   const double* r1 = op + (long long)(nz - 1) * lda;
@@ -1444,13 +1511,26 @@ __global__ __launch_bounds__(kNT) void k_zrows_wall(const double* __restrict__ o
 void zrows_wall(const double* op, int lda, int nz, const double2* x, long long P, double2* bottom, double2* top,
                 hipStream_t st) {
   dim3 grid(ceil_div(P, kNT / 64));
-  hipLaunchKernelGGL(k_zrows_wall, grid, dim3(kNT), 0, st, op, lda, nz, x, P, bottom, top);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_zrows_wall<false>, k_zrows_wall<true>, grid, dim3(kNT), 0, st, ZRowsA{op, lda, nz, x, P, bottom, top});
 }
 
 // one wave per (field, wall): 3 fields x 2 rows
-__global__ __launch_bounds__(384) void k_wall_mean(const double* __restrict__ op, int lda, int nz,
-                                                   const double2* __restrict__ x, long long S, double* out) {
+struct WallMeanA {
+  const double* op;
+  int lda, nz;
+  const double2* x;
+  long long S;
+  double* out;
+};
+template <bool G>
+__global__ __launch_bounds__(384) void k_wall_mean(WallMeanA a0, const Group* grp) {
+  Blk bk;
+  const WallMeanA& A = member<G>(a0, grp, bk);
+  const double* __restrict__ op = A.op;
+  const int lda = A.lda, nz = A.nz;
+  const double2* __restrict__ x = A.x;
+  const long long S = A.S;
+  double* out = A.out;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;        // 0..5
   const int f = w >> 1, top = w & 1;
@@ -1464,8 +1544,7 @@ __global__ __launch_bounds__(384) void k_wall_mean(const double* __restrict__ op
 }
 
 void wall_mean_rows(const double* op, int lda, int nz, const double2* x, long long S, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_wall_mean, dim3(1), dim3(384), 0, st, op, lda, nz, x, S, out);
-  TBD_HIP(hipGetLastError());
+  glaunch(k_wall_mean<false>, k_wall_mean<true>, dim3(1), dim3(384), 0, st, WallMeanA{op, lda, nz, x, S, out});
 }
 
 template <typename T>

@@ -40,7 +40,8 @@ EXPORTS = [
     "tbd_set_variant", "tbd_dft_forward", "tbd_dft_forward_noalloc", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
     "tbd_project", "tbd_swirling_strength", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
     "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
-    "tbd_bf16_gemm",
+    "tbd_bf16_gemm", "tbd_ensemble_create", "tbd_ensemble_step", "tbd_ensemble_launches", "tbd_ensemble_sync",
+    "tbd_ensemble_destroy",
 ]
 
 # profiler classes (one per kernel): name -> (index, work kind)
@@ -134,6 +135,12 @@ def lib():
         L.tbd_sno_loss_grad.argtypes = [vp, fp, fp, fp, ctypes.c_int, dp]
         L.tbd_sno_train_step.argtypes = [vp, fp, fp, fp, ctypes.c_int, ctypes.c_float, dp]
         L.tbd_bf16_gemm.argtypes = [ctypes.c_int] * 3 + [fp, fp, fp]
+        L.tbd_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.tbd_ensemble_step.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.tbd_ensemble_launches.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.tbd_ensemble_sync.argtypes = [vp]
+        L.tbd_ensemble_destroy.argtypes = [vp]
+        L.tbd_ensemble_destroy.restype = None
         _lib = L
     return _lib
 
@@ -367,6 +374,47 @@ class Solver:
         y = np.zeros_like(xx)
         _check(lib().tbd_helmholtz(self._h, which, stage, mode, _p(xx.view(np.float64)), _p(y.view(np.float64))))
         return y
+
+
+class GroupedEnsemble:
+    """Solvers advanced together through grouped launches (tbd_ensemble_*):
+    one launch per kernel and launch shape at each position of the step over
+    all members, captured in one HIP graph.  The solvers stay usable one by
+    one (upload, download, features) and must outlive this object."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        hs = (ctypes.c_void_p * len(self.solvers))(*[s._h for s in self.solvers])
+        h = ctypes.c_void_p()
+        _check(lib().tbd_ensemble_create(hs, len(self.solvers), ctypes.byref(h)))
+        self._h = h
+
+    def step(self, times, alternations):
+        n = len(self.solvers)
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        a = (ctypes.c_int * n)(*[int(x) for x in alternations])
+        assert t.shape == (n,)
+        _check(lib().tbd_ensemble_step(self._h, _p(t), a))
+
+    def launches(self):
+        """(grouped launches, launches of the members one by one) per step."""
+        g, m = ctypes.c_int(), ctypes.c_int()
+        _check(lib().tbd_ensemble_launches(self._h, ctypes.byref(g), ctypes.byref(m)))
+        return g.value, m.value
+
+    def sync(self):
+        _check(lib().tbd_ensemble_sync(self._h))
+
+    def close(self):
+        if self._h:
+            lib().tbd_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Stepper:
