@@ -179,11 +179,14 @@ int tbd_set_variant(tbd_ctx* ctx, int bits);
  * overlap and fused-transform variant bits clear (tbd_set_variant) and the
  * register transforms; otherwise the step fails with an error.
  * tbd_ensemble_launches: grouped launches per step and the launches the
- * members would issue one by one, of the last recorded step. */
+ * members would issue one by one, of the last recorded step.
+ * tbd_ensemble_features: every member's 50 features (tbd_features) after the
+ * last step, member m at out + 50 m, waiting for the ensemble's work. */
 typedef struct tbd_ens tbd_ens;
 int tbd_ensemble_create(tbd_ctx* const* members, int n, tbd_ens** out);
 int tbd_ensemble_step(tbd_ens* e, const double* time, const int* alternation);
 int tbd_ensemble_launches(tbd_ens* e, int* grouped, int* member);
+int tbd_ensemble_features(tbd_ens* e, double* out);
 int tbd_ensemble_sync(tbd_ens* e);
 void tbd_ensemble_destroy(tbd_ens* e);
 

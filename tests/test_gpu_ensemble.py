@@ -96,6 +96,9 @@ def test_grouped_members_match_alone():
     g, m = ens.group.launches()
     got = [s.download() for s in ens.solvers]
     feats = [f.copy() for f in ens.feats]
+    gathered = ens.group.features()
+    for k, s in enumerate(ens.solvers):
+        assert np.array_equal(gathered[k], s.features())
     ens.close()
     print("grouped launches per step %d, member launches %d" % (g, m))
     assert 0 < g < m

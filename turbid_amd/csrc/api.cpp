@@ -143,6 +143,13 @@ int tbd_ensemble_launches(tbd_ens* e, int* grouped, int* member) {
   });
 }
 
+int tbd_ensemble_features(tbd_ens* e, double* out) {
+  return guard([&] {
+    TBD_CHECK(e && out, "tbd_ensemble_features > null argument");
+    e->e->features(out);
+  });
+}
+
 int tbd_ensemble_sync(tbd_ens* e) {
   return guard([&] {
     TBD_CHECK(e, "tbd_ensemble_sync > null ensemble");

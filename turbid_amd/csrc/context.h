@@ -151,6 +151,7 @@ class Context {
            fft_has_two_level(d.ny);
   }
   double* force_pair() const { return force_; }
+  const double* features_device() const { return fg_; }
   void forcing(double time, double& f1, double& f2) const { forces(time, f1, f2); }
   hipStream_t stream() const { return st_; }
   int wall_slot() const { return wslot_; }

@@ -18,10 +18,15 @@ Ensemble::Ensemble(const std::vector<Context*>& members) : m_(members) {
   TBD_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
   mev_.resize(m_.size(), nullptr);
   for (auto& e : mev_) TBD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  TBD_HIP(hipMalloc((void**)&feat_dev_, sizeof(double) * 50 * m_.size()));
+  TBD_HIP(hipHostMalloc((void**)&feat_host_, sizeof(double) * 50 * m_.size(), hipHostMallocDefault));
 }
 
 Ensemble::~Ensemble() {
   drop();
+  if (st_) (void)hipStreamSynchronize(st_);
+  if (feat_dev_) (void)hipFree(feat_dev_);
+  if (feat_host_) (void)hipHostFree(feat_host_);
   for (auto e : mev_) if (e) (void)hipEventDestroy(e);
   if (ev_) (void)hipEventDestroy(ev_);
   if (st_) (void)hipStreamDestroy(st_);
@@ -39,6 +44,16 @@ void Ensemble::drop() {
 }
 
 void Ensemble::sync() { TBD_HIP(hipStreamSynchronize(st_)); }
+
+void Ensemble::features(double* out) {
+  const int n = (int)m_.size();
+  std::vector<const double*> src(n);
+  for (int m = 0; m < n; ++m) src[m] = m_[m]->features_device();
+  gather_rows(src.data(), n, 50, feat_dev_, st_);
+  TBD_HIP(hipMemcpyAsync(feat_host_, feat_dev_, sizeof(double) * 50 * n, hipMemcpyDeviceToHost, st_));
+  TBD_HIP(hipStreamSynchronize(st_));
+  std::memcpy(out, feat_host_, sizeof(double) * 50 * n);
+}
 
 // Records every member's step, merges the launch sequences and captures the
 // merged sequence.  Merge by position: the i-th launches of all members that
@@ -114,7 +129,7 @@ void Ensemble::build(Program& p, const double* time, const int* alternation) {
   TBD_HIP(hipMalloc((void**)&p.arena, image.empty() ? 256 : image.size()));
   for (size_t i = 0; i < launches.size(); ++i) {
     Group* g = reinterpret_cast<Group*>(image.data() + launches[i].group_off);
-    g->args = p.arena + args_off[i];
+    g->args = (unsigned long long)(p.arena + args_off[i]);
   }
   TBD_HIP(hipMemcpy(p.arena, image.data(), image.size(), hipMemcpyHostToDevice));
   // the graph's edges follow the members: a grouped launch waits for the

@@ -31,6 +31,9 @@ class Ensemble {
   int grouped_launches() const { return last_grouped_; }
   int member_launches() const { return last_member_; }
   void sync();
+  // the 50 features of every member after the last step (member m at
+  // out + 50 m): one gather and one copy, then a wait for the ensemble
+  void features(double* out);
 
  private:
   struct Launch {
@@ -60,6 +63,8 @@ class Ensemble {
   std::vector<Program> progs_;
   static constexpr size_t kMaxPrograms = 8;
   int last_grouped_ = 0, last_member_ = 0;
+  double* feat_dev_ = nullptr;   // n x 50
+  double* feat_host_ = nullptr;  // pinned copy
 };
 
 }  // namespace tbd

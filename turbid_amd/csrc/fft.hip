@@ -268,7 +268,7 @@ struct YfA {
   const int* gate;
 };
 template <typename T, typename SpecT, int N1, int N2, int NT, bool G>
-__global__ __launch_bounds__(NT) void k_ypf2(YfA<T, SpecT> a0, const Group* grp) {
+__global__ __launch_bounds__(NT) void k_ypf2(YfA<T, SpecT> a0, GroupP grp) {
   Blk bk;
   const YfA<T, SpecT>& A = member<G>(a0, grp, bk);
   if (A.gate && *A.gate == 0) return;
@@ -336,7 +336,7 @@ struct YbA {
   const int* gate;
 };
 template <typename T, typename SpecT, int N1, int N2, int NT, bool G>
-__global__ __launch_bounds__(NT) void k_ypb2(YbA<T, SpecT> a0, const Group* grp) {
+__global__ __launch_bounds__(NT) void k_ypb2(YbA<T, SpecT> a0, GroupP grp) {
   Blk bk;
   const YbA<T, SpecT>& A = member<G>(a0, grp, bk);
   if (A.gate && *A.gate == 0) return;
@@ -402,7 +402,7 @@ struct XfA {
   const int* gate;
 };
 template <typename T, int N1, int N2, int NT, bool G>
-__global__ __launch_bounds__(NT) void k_xpf2(XfA<T> a0, const Group* grp) {
+__global__ __launch_bounds__(NT) void k_xpf2(XfA<T> a0, GroupP grp) {
   Blk bk;
   const XfA<T>& A = member<G>(a0, grp, bk);
   if (A.gate && *A.gate == 0) return;
@@ -471,7 +471,7 @@ struct XbA {
   const int* gate;
 };
 template <typename T, int N1, int N2, int NT, bool G>
-__global__ __launch_bounds__(NT) void k_xpb2(XbA<T> a0, const Group* grp) {
+__global__ __launch_bounds__(NT) void k_xpb2(XbA<T> a0, GroupP grp) {
   Blk bk;
   const XbA<T>& A = member<G>(a0, grp, bk);
   if (A.gate && *A.gate == 0) return;

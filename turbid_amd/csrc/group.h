@@ -3,7 +3,7 @@
 // argument struct.
 //
 // Every kernel of the substage is written once, as `template <..., bool G>
-// __global__ void k(A a, const Group* g)`.  With G = false it is the plain
+// __global__ void k(A a, GroupP g)`.  With G = false it is the plain
 // launch (grid and arguments its own, g unused).  With G = true one launch
 // covers the members of a Group: member m owns the linear workgroups
 // [start[m], start[m + 1]) and sees, through `member`, the workgroup index
@@ -26,12 +26,17 @@ namespace tbd {
 constexpr int kMaxMembers = 32;
 
 struct Group {
-  const char* args;                 // member m's argument struct: args + m * stride
+  unsigned long long args;          // device address of member 0's argument struct; member m's at args + m * stride
   int stride;
   int n;                            // members in this launch
   int start[kMaxMembers + 1];       // first linear workgroup of each member, start[n] = total
   int gx[kMaxMembers], gy[kMaxMembers], gz[kMaxMembers];
 };
+
+// Kernels see the Group, and through it the argument table, in the constant
+// address space, as they see their own arguments: the compiler keeps what it
+// reads there in scalar registers and treats the pointers it holds as global.
+typedef const __attribute__((address_space(4))) Group* GroupP;
 
 // a workgroup's index and grid as the member's own launch would give them
 struct Blk {
@@ -39,7 +44,7 @@ struct Blk {
 };
 
 template <bool G, class A>
-__device__ __forceinline__ const A& member(const A& own, const Group* __restrict__ g, Blk& b) {
+__device__ __forceinline__ const A& member(const A& own, GroupP g, Blk& b) {
   if constexpr (G) {
     const int lin = (int)blockIdx.x;
     int m = 0;
@@ -52,11 +57,8 @@ __device__ __forceinline__ const A& member(const A& own, const Group* __restrict
     l /= b.gx;
     b.y = l % b.gy;
     b.z = l / b.gy;
-    // the argument table is read through the constant address space: the
-    // compiler may then keep it in scalar registers (uniform, invariant
-    // loads) as it does the kernel's own arguments
     typedef const __attribute__((address_space(4))) char* ConstP;
-    const ConstP p = (ConstP)(g->args + (size_t)m * g->stride);
+    const ConstP p = (ConstP)g->args + (size_t)m * g->stride;
     return *reinterpret_cast<const A*>((const char*)p);
   } else {
     b.x = (int)blockIdx.x;
@@ -84,7 +86,7 @@ struct Recorder {
 };
 
 template <class A>
-void glaunch(void (*single)(A, const Group*), void (*grouped)(A, const Group*), dim3 grid, dim3 block, size_t lds,
+void glaunch(void (*single)(A, GroupP), void (*grouped)(A, GroupP), dim3 grid, dim3 block, size_t lds,
              hipStream_t st, const A& a) {
   if (Recorder* r = Recorder::active()) {
     Rec rec;
@@ -96,7 +98,7 @@ void glaunch(void (*single)(A, const Group*), void (*grouped)(A, const Group*), 
     r->ops.push_back(std::move(rec));
     return;
   }
-  hipLaunchKernelGGL(single, grid, block, lds, st, a, (const Group*)nullptr);
+  hipLaunchKernelGGL(single, grid, block, lds, st, a, (GroupP)nullptr);
   TBD_HIP(hipGetLastError());
 }
 

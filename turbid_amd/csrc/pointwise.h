@@ -39,6 +39,8 @@ void copy_words(void* dst, const void* src, size_t bytes, hipStream_t st);
 // dst[i][0] = v[2 i], dst[i][1] = v[2 i + 1], i < n (n <= 32): the forcing
 // pairs of an ensemble's members, one launch
 void set_pairs(double* const* dst, const double* v, int n, hipStream_t st);
+// dst[m * count + i] = src[m][i], m < n (n <= 32), i < count
+void gather_rows(const double* const* src, int n, int count, double* dst, hipStream_t st);
 // dst[0] = a, dst[1] = b (values that a captured step graph reads at replay)
 void set_pair(double* dst, double a, double b, hipStream_t st);
 template <typename TA>

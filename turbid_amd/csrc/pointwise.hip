@@ -33,7 +33,7 @@ struct PConvA {
   long long R;
 };
 template <typename T, bool G>
-__global__ void k_products_conv(PConvA<T> a0, const Group* grp) {
+__global__ void k_products_conv(PConvA<T> a0, GroupP grp) {
   Blk b;
   const PConvA<T>& A = member<G>(a0, grp, b);
   const T* __restrict__ U = A.U;
@@ -59,7 +59,7 @@ struct PDivA {
   long long R;
 };
 template <typename T, bool G>
-__global__ void k_products_div(PDivA<T> a0, const Group* grp) {
+__global__ void k_products_div(PDivA<T> a0, GroupP grp) {
   Blk b;
   const PDivA<T>& A = member<G>(a0, grp, b);
   const T* __restrict__ U = A.U;
@@ -89,7 +89,7 @@ struct SConvA {
   long long R;
 };
 template <typename T, bool G>
-__global__ void k_scalar_conv(SConvA<T> a0, const Group* grp) {
+__global__ void k_scalar_conv(SConvA<T> a0, GroupP grp) {
   Blk b;
   const SConvA<T>& A = member<G>(a0, grp, b);
   const T* __restrict__ U = A.U;
@@ -116,7 +116,7 @@ struct SDivA {
   long long R;
 };
 template <typename T, typename TC, bool G>
-__global__ void k_scalar_div(SDivA<T, TC> a0, const Group* grp) {
+__global__ void k_scalar_div(SDivA<T, TC> a0, GroupP grp) {
   Blk b;
   const SDivA<T, TC>& A = member<G>(a0, grp, b);
   const T* __restrict__ U = A.U;
@@ -147,7 +147,7 @@ struct HUpdA {
   long long S;
 };
 template <typename TA, bool G>
-__global__ void k_h_update(HUpdA<TA> a0, const Group* grp) {
+__global__ void k_h_update(HUpdA<TA> a0, GroupP grp) {
   Blk b;
   const HUpdA<TA>& A = member<G>(a0, grp, b);
   double2* __restrict__ h = A.h;
@@ -261,8 +261,27 @@ void set_pairs(double* const* dst, const double* v, int n, hipStream_t st) {
   TBD_HIP(hipGetLastError());
 }
 
+struct GatherA {
+  const double* src[kMaxMembers];
+  double* dst;
+  int count;
+};
+__global__ void k_gather_rows(GatherA a) {
+  const int m = blockIdx.x;
+  for (int i = threadIdx.x; i < a.count; i += blockDim.x) a.dst[(long long)m * a.count + i] = a.src[m][i];
+}
+void gather_rows(const double* const* src, int n, int count, double* dst, hipStream_t st) {
+  TBD_CHECK(n >= 1 && n <= kMaxMembers, "gather_rows > between 1 and kMaxMembers rows");
+  GatherA a;
+  for (int i = 0; i < n; ++i) a.src[i] = src[i];
+  a.dst = dst;
+  a.count = count;
+  hipLaunchKernelGGL(k_gather_rows, dim3(n), dim3(64), 0, st, a);
+  TBD_HIP(hipGetLastError());
+}
+
 template <bool G>
-__global__ void k_bc_carrier(BcCA a0, const Group* grp) {
+__global__ void k_bc_carrier(BcCA a0, GroupP grp) {
   Blk b;
   const BcCA& A = member<G>(a0, grp, b);
   double2* __restrict__ rhs = A.rhs;
@@ -298,7 +317,7 @@ struct BcSA {
   int erosion;
 };
 template <bool G>
-__global__ void k_bc_scalar(BcSA a0, const Group* grp) {
+__global__ void k_bc_scalar(BcSA a0, GroupP grp) {
   Blk b;
   const BcSA& A = member<G>(a0, grp, b);
   double2* __restrict__ rhs = A.rhs;
@@ -336,7 +355,7 @@ struct ErosA {
   double re, tc, M;
 };
 template <typename T, bool G>
-__global__ void k_erosion(ErosA<T> a0, const Group* grp) {
+__global__ void k_erosion(ErosA<T> a0, GroupP grp) {
   Blk b;
   const ErosA<T>& A = member<G>(a0, grp, b);
   const T* __restrict__ pa = A.a;
@@ -369,7 +388,7 @@ struct FiltA {
   const int* gate;
 };
 template <bool G>
-__global__ __launch_bounds__(1024) void k_filter(FiltA a0, const Group* grp) {
+__global__ __launch_bounds__(1024) void k_filter(FiltA a0, GroupP grp) {
   Blk b;
   const FiltA& A = member<G>(a0, grp, b);
   const long long plane = A.plane;
@@ -428,7 +447,7 @@ struct MmA {
   double* part;
 };
 template <typename T, bool G>
-__global__ __launch_bounds__(256) void k_minmax_part(MmA<T> a0, const Group* grp) {
+__global__ __launch_bounds__(256) void k_minmax_part(MmA<T> a0, GroupP grp) {
   Blk bk;
   const MmA<T>& A = member<G>(a0, grp, bk);
   const T* __restrict__ x = A.x;
@@ -461,7 +480,7 @@ struct MmFA {
   int* need;
 };
 template <bool G>
-__global__ __launch_bounds__(64) void k_minmax_final(MmFA a0, const Group* grp) {
+__global__ __launch_bounds__(64) void k_minmax_final(MmFA a0, GroupP grp) {
   Blk bk;
   const MmFA& A = member<G>(a0, grp, bk);
   const double* __restrict__ part = A.part;
@@ -494,7 +513,7 @@ struct Kx0A {
   const int* gate;
 };
 template <bool G>
-__global__ void k_kx0_round_trip(Kx0A a0, const Group* grp) {
+__global__ void k_kx0_round_trip(Kx0A a0, GroupP grp) {
   Blk bk;
   const Kx0A& A = member<G>(a0, grp, bk);
   if (A.gate && *A.gate != 0) return;
@@ -541,7 +560,7 @@ struct DiagA {
   double* part;
 };
 template <typename T, bool G>
-__global__ __launch_bounds__(kNT) void k_diag_part(DiagA<T> a0, const Group* grp) {
+__global__ __launch_bounds__(kNT) void k_diag_part(DiagA<T> a0, GroupP grp) {
   Blk bk;
   const DiagA<T>& A = member<G>(a0, grp, bk);
   const T* __restrict__ U = A.U;
@@ -588,7 +607,7 @@ struct DiagFA {
   double* feat;
 };
 template <bool G>
-__global__ __launch_bounds__(kNT) void k_diag_final(DiagFA a0, const Group* grp) {
+__global__ __launch_bounds__(kNT) void k_diag_final(DiagFA a0, GroupP grp) {
   Blk bk;
   const DiagFA& A = member<G>(a0, grp, bk);
   const double* __restrict__ part = A.part;
@@ -671,7 +690,7 @@ struct TrapzA {
   double* out;
 };
 template <bool G>
-__global__ void k_trapz(TrapzA a0, const Group* grp) {
+__global__ void k_trapz(TrapzA a0, GroupP grp) {
   Blk bk;
   const TrapzA& A = member<G>(a0, grp, bk);
   const double2* __restrict__ c = A.c;
@@ -710,7 +729,7 @@ struct ZeroA {
   long long n;
 };
 template <typename W, bool G>
-__global__ void k_zero(ZeroA<W> a0, const Group* grp) {
+__global__ void k_zero(ZeroA<W> a0, GroupP grp) {
   Blk b;
   const ZeroA<W>& A = member<G>(a0, grp, b);
   W* p = A.p;
@@ -733,7 +752,7 @@ struct ScRA {
   double* out;
 };
 template <bool G>
-__global__ void k_scaled_real(ScRA a0, const Group* grp) {
+__global__ void k_scaled_real(ScRA a0, GroupP grp) {
   Blk bk;
   const ScRA& A = member<G>(a0, grp, bk);
   if (threadIdx.x == 0) *A.out = (A.a * A.src[0].x) * A.b;
@@ -752,7 +771,7 @@ struct FeatA {
   int src6[6];
 };
 template <bool G>
-__global__ void k_features(FeatA a0, const Group* grp) {
+__global__ void k_features(FeatA a0, GroupP grp) {
   Blk bk;
   const FeatA& A = member<G>(a0, grp, bk);
   const int i = threadIdx.x;
@@ -778,7 +797,7 @@ struct CopyA {
   long long n;
 };
 template <bool G>
-__global__ void k_copy4(CopyA a0, const Group* grp) {
+__global__ void k_copy4(CopyA a0, GroupP grp) {
   Blk b;
   const CopyA& A = member<G>(a0, grp, b);
   MEMBER_STRIDE(i, A.n) A.dst[i] = A.src[i];

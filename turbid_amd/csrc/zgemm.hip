@@ -539,7 +539,7 @@ struct ZL {
   Panel pn;
 };
 template <typename T, typename TX, typename TY, int DEPTH, bool G>
-__global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH == 1 ? 4 : 2)) void k_zgemm(ZL l0, const Group* grp) {
+__global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH == 1 ? 4 : 2)) void k_zgemm(ZL l0, GroupP grp) {
   Blk bk;
   const ZL& L = member<G>(l0, grp, bk);
   const ZArgs& a = L.a;
@@ -669,7 +669,7 @@ struct HL {
   Panel pn;
 };
 template <int DEPTH, bool G>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HL l0, const Group* grp) {
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz(HL l0, GroupP grp) {
   Blk bk;
   const HL& L = member<G>(l0, grp, bk);
   const HArgs& a = L.a;
@@ -864,7 +864,7 @@ struct HSL {
   Panel p1, p2;
 };
 template <int DEPTH, bool G>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HSL l0, const Group* grp) {
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_helmholtz_split(HSL l0, GroupP grp) {
   Blk bk;
   const HSL& L = member<G>(l0, grp, bk);
   const HArgs& a = L.a;
@@ -980,7 +980,7 @@ struct RHL {
   Panel pd, p1, p2;
 };
 template <int DEPTH, bool SPLIT, bool G>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHL l0, const Group* grp) {
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_rhs_helmholtz(RHL l0, GroupP grp) {
   Blk bk;
   const RHL& L = member<G>(l0, grp, bk);
   const RHArgs& a = L.a;
@@ -1232,7 +1232,7 @@ struct PL {
   Panel pd, p1, p2;
 };
 template <int DEPTH, bool G>
-__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PL l0, const Group* grp) {
+__global__ __launch_bounds__(kNT, DEPTH == 1 ? 4 : 2) void k_pressure_chain(PL l0, GroupP grp) {
   Blk bk;
   const PL& L = member<G>(l0, grp, bk);
   const PArgs& a = L.a;
@@ -1473,7 +1473,7 @@ struct ZRowsA {
   double2* top;
 };
 template <bool G>
-__global__ __launch_bounds__(kNT) void k_zrows_wall(ZRowsA a0, const Group* grp) {
+__global__ __launch_bounds__(kNT) void k_zrows_wall(ZRowsA a0, GroupP grp) {
   Blk bk;
   const ZRowsA& A = member<G>(a0, grp, bk);
   const double* __restrict__ op = A.op;
@@ -1523,7 +1523,7 @@ struct WallMeanA {
   double* out;
 };
 template <bool G>
-__global__ __launch_bounds__(384) void k_wall_mean(WallMeanA a0, const Group* grp) {
+__global__ __launch_bounds__(384) void k_wall_mean(WallMeanA a0, GroupP grp) {
   Blk bk;
   const WallMeanA& A = member<G>(a0, grp, bk);
   const double* __restrict__ op = A.op;

@@ -105,8 +105,9 @@ class Ensemble:
         self.group.step(times, alts)
         for st in self.steppers:
             st.toggle()
+        feats = self.group.features()
         for m in range(len(self.solvers)):
-            self.feats[m] = self.solvers[m].features()
+            self.feats[m] = feats[m]
 
     def run_grouped(self, nsteps):
         t0 = time.perf_counter()

@@ -40,7 +40,7 @@ EXPORTS = [
     "tbd_set_variant", "tbd_dft_forward", "tbd_dft_forward_noalloc", "tbd_dft_backward", "tbd_zcontract", "tbd_helmholtz",
     "tbd_project", "tbd_swirling_strength", "tbd_stats_accumulate", "tbd_stats_read", "tbd_stats_reset", "tbd_sno_create", "tbd_sno_destroy", "tbd_sno_param_count", "tbd_sno_set_params",
     "tbd_sno_get_params", "tbd_sno_get_grads", "tbd_sno_forward", "tbd_sno_loss_grad", "tbd_sno_train_step",
-    "tbd_bf16_gemm", "tbd_ensemble_create", "tbd_ensemble_step", "tbd_ensemble_launches", "tbd_ensemble_sync",
+    "tbd_bf16_gemm", "tbd_ensemble_create", "tbd_ensemble_step", "tbd_ensemble_launches", "tbd_ensemble_sync", "tbd_ensemble_features",
     "tbd_ensemble_destroy",
 ]
 
@@ -139,6 +139,7 @@ def lib():
         L.tbd_ensemble_step.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
         L.tbd_ensemble_launches.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.tbd_ensemble_sync.argtypes = [vp]
+        L.tbd_ensemble_features.argtypes = [vp, dp]
         L.tbd_ensemble_destroy.argtypes = [vp]
         L.tbd_ensemble_destroy.restype = None
         _lib = L
@@ -404,6 +405,12 @@ class GroupedEnsemble:
 
     def sync(self):
         _check(lib().tbd_ensemble_sync(self._h))
+
+    def features(self):
+        """(members, 50) features after the last step (one copy for all)."""
+        out = np.zeros((len(self.solvers), 50))
+        _check(lib().tbd_ensemble_features(self._h, _p(out)))
+        return out
 
     def close(self):
         if self._h:
