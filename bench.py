@@ -28,7 +28,11 @@ step's time at the roofline (minimal algorithmic transform bytes and
 contraction flops per step) over the measured time.  spinup_gpu_s: the
 metric's second half, GPU seconds to stationary TKE and wall shear from the
 plain noisy-laminar start and from the learned proposal on a held-out seed
-(turbid_amd.spinup).  cpu_baseline: the FP64
+(turbid_amd.spinup).  ensemble: the 16-member ensemble of uneven grids
+(64..128 squared by 65..129, mixed precision) in member-steps/s, advanced
+through grouped launches (one launch per kernel instance over all members,
+one graph), on the members' own streams, and one member at a time
+(turbid_amd.ensemble).  cpu_baseline: the FP64
 CPU restatement (oracle/) timed on the host's cores for one warm-up step and
 then --cpu-steps steps of the same workload.
 """
@@ -45,9 +49,8 @@ sys.path.insert(0, ROOT)
 
 from turbid_amd import runtime as RT   # noqa: E402
 from turbid_amd import workloads as W  # noqa: E402
+from turbid_amd.roofline import HBM_PEAK, MFMA_PEAK, step_roofline  # noqa: E402
 
-HBM_PEAK = 8.0e12          # B/s
-MFMA_PEAK = {"fp64": 78.6e12, "fp32": 157.3e12}
 
 # profiler class -> (kernel symbol, kind, peak)
 KERNELS = {
@@ -200,24 +203,6 @@ def cpu_baseline(cfg, state, nthreads, ksteps):
                       "and %.1f s setup; baseline only" % (ksteps, setup)}
 
 
-def step_roofline(cfg, mixed, ms_per_step):
-    """Whole-step roofline with the minimal algorithmic counts per RK3 step:
-    58.5 field-transforms of w*R + 2*w*S bytes and 63 z-contractions of
-    4*Nz^2*P flops (mixed: FP32 transforms, 12 FP32 and 51 FP64
-    contractions); frac = T_roof / T_measured."""
-    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
-    nyc, mx = ny // 3 * 2, (nx // 3 * 2) // 2 + 1
-    P = mx * nyc
-    R, S = nx * ny * nz, nz * P
-    w = 4 if mixed else 8
-    t_fft = 58.5 * (w * R + 2 * w * S) / HBM_PEAK
-    zf = 4.0 * nz * nz * P
-    t_z = (51 * zf / MFMA_PEAK["fp64"] + 12 * zf / MFMA_PEAK["fp32"]) if mixed else 63 * zf / MFMA_PEAK["fp64"]
-    t_roof = 1e3 * (t_fft + t_z)
-    return {"t_roof_ms": t_roof, "t_fft_ms": 1e3 * t_fft, "t_z_ms": 1e3 * t_z, "ms_per_step": ms_per_step,
-            "frac": t_roof / ms_per_step}
-
-
 SPINUP_MODEL = os.path.join(ROOT, "turbid_amd", "spinup_model.npz")
 
 
@@ -282,6 +267,7 @@ def main():
     ap.add_argument("--spinup-noise", type=float, nargs="+", default=[0.01],
                     help="noise amplitudes of the plain starts (the first is the headline): 0.01, the weakly "
                          "perturbed laminar layer; 0.1, the start of the 256^3 workload")
+    ap.add_argument("--ensemble-steps", type=int, default=20, help="timed steps of the ensemble leg; 0 skips it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -364,6 +350,13 @@ def main():
         line["spinup_gpu_s"] = spinup_leg(args.spinup_model, args.spinup_periods, args.spinup_noise)
     else:
         line["spinup_gpu_s"] = None
+    if args.ensemble_steps > 0:
+        from turbid_amd import ensemble as E
+        log("bench: ensemble of 16 members, %d steps per leg" % args.ensemble_steps)
+        line["ensemble"] = E.measure(E.member_configs(16, seed=0, precision=1), args.ensemble_steps, 3,
+                                     log=lambda m: log("bench: " + m))
+    else:
+        line["ensemble"] = None
     if rank == 0 and not args.no_cpu_baseline:
         # all the host cores this process may use: the OpenMP thread count the
         # host sets for it, else every CPU

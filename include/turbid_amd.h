@@ -163,14 +163,14 @@ int tbd_get_variant(tbd_ctx* ctx, int* bits);
 int tbd_set_variant(tbd_ctx* ctx, int bits);
 
 /* Ensemble of independent contexts advanced together (the reference runs one
- * FlatChannelSolver per realisation, solver_channel_flat.cc:117-137 being the
+ * FlatChannelSolver per realisation, solver_channel_flat.cc:195-293 being the
  * step loop of each; it has no batched form).  The members (at most 32, not
  * owned, created with tbd_create and kept alive while the ensemble exists)
  * may differ in grid, Reynolds number, settling velocity and the rest of
  * their configuration.  tbd_ensemble_step advances every member by one RK3
  * step (member m at time[m] with alternation[m], as tbd_step): each member's
- * step is recorded once per (alternation, wall-row slot, dt, previous dt) of
- * all members, the members' launches are merged into one launch per kernel
+ * step is recorded once per (alternation, wall-row slot, dt, previous dt,
+ * variant bits) of all members, the members' launches are merged into one launch per kernel
  * and launch shape at each position of the step (the workgroups of all
  * members in one grid, each with its own arguments), and the merged
  * sequence is captured into one HIP graph and replayed.  It is ordered after

@@ -143,3 +143,22 @@ def test_grouped_step_refuses_an_overlapping_member():
         grp.step([cfg["dt"]], [0])
     grp.close()
     s.close()
+
+
+def test_grouped_tuning_restores_the_members():
+    cfgs = _grouped_members()
+    ens = E.Ensemble(cfgs, grouped=True)
+    ens.step_grouped()
+    before = [s.download() for s in ens.solvers]
+    times = [st.time for st in ens.steppers]
+    bits, tried = ens.tune_grouped(steps=2)
+    assert set(tried) == {0, *E.Ensemble.TUNE_BITS}
+    assert all(np.isfinite(v) and v > 0 for v in tried.values())
+    after = [s.download() for s in ens.solvers]
+    assert [st.time for st in ens.steppers] == times
+    for (ub, pb, cb), (ua, pa, ca) in zip(before, after):
+        assert np.array_equal(ub, ua) and np.array_equal(cb, ca)
+    assert bits == [s.variant() for s in ens.solvers]
+    ens.step_grouped()
+    assert ens.finite()
+    ens.close()

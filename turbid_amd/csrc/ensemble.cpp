@@ -172,7 +172,7 @@ void Ensemble::step(const double* time, const int* alternation) {
     bool same = true;
     for (int m = 0; m < n && same; ++m)
       same = p.alt[m] == alternation[m] && p.slot[m] == m_[m]->wall_slot() && p.dt[m] == m_[m]->dt() &&
-             p.olddt[m] == m_[m]->olddt();
+             p.olddt[m] == m_[m]->olddt() && p.bits[m] == m_[m]->variant().bits();
     if (same) {
       prog = &p;
       break;
@@ -200,6 +200,7 @@ void Ensemble::step(const double* time, const int* alternation) {
       p.slot.push_back(m_[m]->wall_slot());
       p.dt.push_back(m_[m]->dt());
       p.olddt.push_back(m_[m]->olddt());
+      p.bits.push_back(m_[m]->variant().bits());
     }
     try {
       build(p, time, alternation);

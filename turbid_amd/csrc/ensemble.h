@@ -1,8 +1,8 @@
 // An ensemble of independent DNS contexts (uneven grids, Reynolds numbers,
 // settling velocities) advanced together on one stream with one launch per
 // kernel and launch shape per position of the step: each member's step is
-// recorded once per (alternation, wall-row slot, dt, previous dt) of every
-// member, the members' launch sequences are merged into grouped launches
+// recorded once per (alternation, wall-row slot, dt, previous dt, variant) of
+// every member, the members' launch sequences are merged into grouped launches
 // (group.h), and the merged sequence is captured into one HIP graph.
 #pragma once
 
@@ -46,7 +46,7 @@ class Ensemble {
     std::vector<int> members;      // member indices, in the Group's order
   };
   struct Program {
-    std::vector<int> alt, slot;
+    std::vector<int> alt, slot, bits;
     std::vector<double> dt, olddt;
     char* arena = nullptr;
     hipGraph_t graph = nullptr;

@@ -20,6 +20,7 @@ Three ways to advance them, timed side by side:
   sequential  one member at a time, each synchronised every step.
 """
 import argparse
+import copy
 import json
 import math
 import sys
@@ -29,6 +30,7 @@ import numpy as np
 
 from . import runtime as RT
 from . import workloads as W
+from .roofline import step_roofline
 
 
 def member_configs(n=16, seed=0, precision=1):
@@ -109,6 +111,56 @@ class Ensemble:
         for m in range(len(self.solvers)):
             self.feats[m] = feats[m]
 
+    # variant bits tried by tune_grouped, each set or cleared on every member
+    TUNE_BITS = ["rhs", "wide_yb", "wide_yf", "wide_xb", "wide_xf", "swap_yb", "swap_yf", "swap_xb", "swap_xf",
+                 "wide_zgemm"]
+
+    def tune_grouped(self, steps=5, log=None):
+        """Coordinate descent over the variant bits of TUNE_BITS, each applied
+        to every member at once (alike members keep merging), timing `steps`
+        grouped steps after two warm-up steps that also record the variant's
+        step.  The members' fields, steppers and features are restored
+        afterwards (uploaded, so the wall-row history restarts as on a
+        restart from files); the fastest bits are kept.  Returns (the
+        members' variant bits, {tried bit name or 0: ms per step})."""
+        saved = [s.download() for s in self.solvers]
+        steppers = copy.deepcopy(self.steppers)
+        feats = [f.copy() for f in self.feats]
+        base = [s.variant() for s in self.solvers]
+        flags = RT.Solver.VARIANT_BITS
+
+        def timed(mask_on, mask_off):
+            for s, b in zip(self.solvers, base):
+                s.set_variant((b | mask_on) & ~mask_off)
+            for _ in range(2):
+                self.step_grouped()
+            t = self.run_grouped(steps)
+            return 1e3 * t / steps
+
+        on, off = 0, 0
+        best = timed(on, off)
+        tried = {0: best}
+        for name in self.TUNE_BITS:
+            bit = flags[name]
+            if all(b & bit for b in base):
+                c_on, c_off = on & ~bit, off | bit
+            else:
+                c_on, c_off = on | bit, off & ~bit
+            ms = timed(c_on, c_off)
+            tried[name] = ms
+            if log:
+                log("ensemble: tune %s %s: %.3f ms per step (best %.3f)"
+                    % (name, "on" if c_on & bit else "off", ms, best))
+            if ms < best:
+                best, on, off = ms, c_on, c_off
+        for s, b, st in zip(self.solvers, base, saved):
+            s.set_variant((b | on) & ~off)
+            s.upload(*st)
+        self.steppers = steppers
+        self.feats = feats
+        self.variants = [s.variant() for s in self.solvers]
+        return self.variants, tried
+
     def run_grouped(self, nsteps):
         t0 = time.perf_counter()
         for _ in range(nsteps):
@@ -138,6 +190,57 @@ class Ensemble:
             s.close()
 
 
+def roofline(cfgs, ms_per_step):
+    """Sum over the members of the step roofline (minimal algorithmic bytes
+    and flops of each member's step at MI355X peaks) against the measured
+    time of one ensemble step."""
+    t = sum(step_roofline(c, c.get("precision", 0) == 1, 1.0)["t_roof_ms"] for c in cfgs)
+    return {"t_roof_ms": t, "ms_per_step": ms_per_step, "frac": t / ms_per_step}
+
+
+def measure(cfgs, steps, warmup, legs=("grouped", "streams", "sequential"), tune=False, log=None):
+    """Member-steps/s of each leg on the same member configurations (each leg
+    starts from the seeded states)."""
+    total = len(cfgs) * steps
+    out = {"members": len(cfgs), "steps": steps}
+    if "grouped" in legs:
+        ens = Ensemble(cfgs, grouped=True)
+        if tune:
+            _, tried = ens.tune_grouped(log=log)
+            out["tune_ms_per_step"] = tried
+        for _ in range(warmup):
+            ens.step_grouped()
+        tg = ens.run_grouped(steps)
+        out["grouped_steps_per_s"] = total / tg
+        out["launches_per_step"] = dict(zip(("grouped", "members"), ens.group.launches()))
+        out["roofline"] = roofline(cfgs, 1e3 * tg / steps)
+        out["variants"] = ens.variants
+        out["finite"] = ens.finite()
+        ens.close()
+        if log:
+            log("ensemble: grouped %.1f member-steps/s (%d grouped launches per step for %d member launches), "
+                "roofline %.3f" % (out["grouped_steps_per_s"], out["launches_per_step"]["grouped"],
+                                   out["launches_per_step"]["members"], out["roofline"]["frac"]))
+    if "streams" in legs or "sequential" in legs:
+        ens = Ensemble(cfgs)
+        for _ in range(warmup):
+            ens.step()
+        if "streams" in legs:
+            out["concurrent_steps_per_s"] = total / ens.run_concurrent(steps)
+        if "sequential" in legs:
+            out["sequential_steps_per_s"] = total / ens.run_sequential(steps)
+        out["finite"] = bool(out.get("finite", True) and ens.finite())
+        ens.close()
+        if log:
+            log("ensemble: streams %s, sequential %s member-steps/s"
+                % (out.get("concurrent_steps_per_s"), out.get("sequential_steps_per_s")))
+    if "grouped_steps_per_s" in out and "sequential_steps_per_s" in out:
+        out["grouped_vs_sequential"] = out["grouped_steps_per_s"] / out["sequential_steps_per_s"]
+    if "grouped_steps_per_s" in out and "concurrent_steps_per_s" in out:
+        out["grouped_vs_streams"] = out["grouped_steps_per_s"] / out["concurrent_steps_per_s"]
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--members", type=int, default=16)
@@ -145,43 +248,17 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--autotune", action="store_true", help="each member picks its kernel variant")
+    ap.add_argument("--legs", default="grouped,streams,sequential")
+    ap.add_argument("--tune", action="store_true", help="tune the grouped members' variant bits together")
     args = ap.parse_args(argv)
     cfgs = member_configs(args.members, args.seed, args.precision)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
-    total = args.members * args.steps
-    rates = {}
-    # grouped: its own members (the grouped stepper sets their variant bits)
-    ens = Ensemble(cfgs, autotune=args.autotune, grouped=True)
-    for _ in range(args.warmup):
-        ens.step_grouped()
-    tg = ens.run_grouped(args.steps)
-    rates["grouped"] = total / tg
-    launches = ens.group.launches()
-    variants = ens.variants
-    finite_g = ens.finite()
-    ens.close()
-    log("ensemble: grouped %.1f member-steps/s (%d grouped launches per step for %d member launches)"
-        % (rates["grouped"], launches[0], launches[1]))
-    ens = Ensemble(cfgs, autotune=args.autotune)
-    for _ in range(args.warmup):
-        ens.step()
-    tc = ens.run_concurrent(args.steps)
-    ts = ens.run_sequential(args.steps)
-    rates["streams"] = total / tc
-    rates["sequential"] = total / ts
-    log("ensemble: streams %.1f, sequential %.1f member-steps/s" % (rates["streams"], rates["sequential"]))
-    out = dict(metric="ensemble RK3 member-steps/s", members=args.members, steps=args.steps,
-               grouped_steps_per_s=rates["grouped"], concurrent_steps_per_s=rates["streams"],
-               sequential_steps_per_s=rates["sequential"], grouped_vs_sequential=rates["grouped"] / rates["sequential"],
-               grouped_vs_streams=rates["grouped"] / rates["streams"],
-               launches_per_step={"grouped": launches[0], "members": launches[1]},
-               precision="mixed" if args.precision == 1 else "fp64", finite=bool(ens.finite() and finite_g),
+    out = dict(metric="ensemble RK3 member-steps/s")
+    out.update(measure(cfgs, args.steps, args.warmup, tuple(args.legs.split(",")), args.tune, log))
+    out.update(precision="mixed" if args.precision == 1 else "fp64",
                grids=[[c["nx"], c["ny"], c["nz"]] for c in cfgs],
                re_delta=[round(c["re_delta"], 1) for c in cfgs],
-               settling=[float("%.3g" % c["settling_velocity"]) for c in cfgs],
-               variants=variants)
-    ens.close()
+               settling=[float("%.3g" % c["settling_velocity"]) for c in cfgs])
     print(json.dumps(out), flush=True)
 
 
