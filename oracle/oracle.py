@@ -68,7 +68,10 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        # rebuilt when missing or older than any of its sources, so an edit of
+        # the restatement is never tested through a stale library
+        srcs = [os.path.join(_HERE, f) for f in os.listdir(_HERE) if f.endswith((".cpp", ".h"))]
+        if not os.path.exists(_LIB_PATH) or any(os.path.getmtime(f) > os.path.getmtime(_LIB_PATH) for f in srcs):
             build()
         L = ctypes.CDLL(_LIB_PATH)
         dp = ctypes.POINTER(ctypes.c_double)
