@@ -17,3 +17,13 @@ def _oracle_built():
     from oracle import oracle
     oracle.build()
     yield
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _progress_channel(request):
+    """Lets helpers.say() print past pytest's output capture, so that a long
+    GPU test (an oracle reference of many steps) shows progress under -q."""
+    import helpers
+    helpers.CAPTURE = request.config.pluginmanager.getplugin("capturemanager")
+    yield
+    helpers.CAPTURE = None

@@ -6,6 +6,18 @@ import numpy as np
 from turbid_amd import workloads as W
 
 
+CAPTURE = None   # pytest's capture manager (set by conftest.py)
+
+
+def say(msg):
+    """Print a progress line to the real stdout even while pytest captures."""
+    if CAPTURE is None:
+        print(msg, flush=True)
+        return
+    with CAPTURE.global_and_fixture_disabled():
+        print(msg, flush=True)
+
+
 def rel_l2(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

@@ -16,7 +16,6 @@
 
 The oracle runs on up to 16 host threads (about 3.5 s per 256^3 step)."""
 import os
-import sys
 
 import numpy as np
 import pytest
@@ -27,7 +26,7 @@ from turbid_amd import ensemble as E
 from turbid_amd import runtime as RT
 from turbid_amd import workloads as W
 
-from helpers import field_errors, oracle_steps, rel_l2
+from helpers import field_errors, oracle_steps, rel_l2, say
 
 pytestmark = pytest.mark.gpu
 
@@ -35,8 +34,7 @@ NTHREADS = min(16, os.cpu_count() or 1)
 
 
 def _say(msg):
-    print(msg, flush=True)
-    sys.stdout.flush()
+    say(msg)
 
 
 def _gpu_steps(cfg, state, nsteps):

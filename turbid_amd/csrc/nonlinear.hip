@@ -346,7 +346,12 @@ __global__ __launch_bounds__(NT) void k_xfused2(XfArgs a) {
   const long long pstride = (long long)a.nz * plane;
   const C2<T>* gin = (const C2<T>*)a.gin;
   C2<T>* gout = (C2<T>*)a.gout;
-  const C2<T>* tw = (const C2<T>*)a.tw;
+  // the twiddles in LDS: a global twiddle load issued after this workgroup's
+  // output stores would wait for those stores (one in-order memory counter),
+  // which serialised the transforms of the fused kernel on store latency
+  __shared__ C2<T> tw[N];
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = ((const C2<T>*)a.tw)[i];
+  __syncthreads();
   const T* Ub = (const T*)a.U + (long long)z * a.ny * N;
   const T shifts[3] = {(T)a.su, (T)a.sv, (T)a.sw};
   const int t = threadIdx.x;
