@@ -66,11 +66,11 @@ KERNELS = {
     "xfused32": ("k_xfused2<float,...>", "bytes", HBM_PEAK),
     "zgemm64": ("k_zgemm<double,...>", "flops", MFMA_PEAK["fp64"]),
     "zgemm32": ("k_zgemm<float,...>", "flops", MFMA_PEAK["fp32"]),
-    "helmholtz": ("k_helmholtz<1>", "flops", MFMA_PEAK["fp64"]),
-    "helmholtz_split": ("k_helmholtz_split<1>", "flops", MFMA_PEAK["fp64"]),
-    "rhs_helmholtz": ("k_rhs_helmholtz<1, false>", "flops", MFMA_PEAK["fp64"]),
-    "rhs_helmholtz_split": ("k_rhs_helmholtz<1, true>", "flops", MFMA_PEAK["fp64"]),
-    "pressure_chain": ("k_pressure_chain<1>", "flops", MFMA_PEAK["fp64"]),
+    "helmholtz": ("k_helmholtz<1, false>", "flops", MFMA_PEAK["fp64"]),
+    "helmholtz_split": ("k_helmholtz_split<1, false>", "flops", MFMA_PEAK["fp64"]),
+    "rhs_helmholtz": ("k_rhs_helmholtz<1, false, false>", "flops", MFMA_PEAK["fp64"]),
+    "rhs_helmholtz_split": ("k_rhs_helmholtz<1, true, false>", "flops", MFMA_PEAK["fp64"]),
+    "pressure_chain": ("k_pressure_chain<1, false>", "flops", MFMA_PEAK["fp64"]),
     "products64": ("k_products_*<double>", "bytes", HBM_PEAK),
     "products32": ("k_products_*<float>", "bytes", HBM_PEAK),
     "other": ("(boundary rows, H update, filter, diagnostics, copies)", None, None),
