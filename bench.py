@@ -17,8 +17,9 @@ times the step's kernel variants on a scratch copy of the context and keeps
 the fastest (reported under `autotune`; --no-autotune keeps the default).
 
 Timing: after the warmup, K steps are timed as the library runs them (one
-captured HIP graph per step; `value`, `ms_per_step`), then K further steps
-run with HIP events around every launch on the library's stream (eager
+captured HIP graph per step; `value`, `ms_per_step`; K = 1000 by default,
+the workload's step count), then min(K, --profile-steps) further steps run
+with HIP events around every launch on the library's stream (eager
 substages), which gives each kernel's launch count, average duration and
 algorithmic work.  Roofline: the kernel with the largest total time;
 `achieved` = its algorithmic work per launch (bytes read plus written by a
@@ -81,12 +82,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run(cfg, state, steps, warmup, autotune, variant=None):
+def run(cfg, state, steps, warmup, autotune, variant=None, psteps=None):
     """Pick the kernel variant (the library's autotuner, on a scratch copy of
     the context, outside the timed region), warm up, time `steps` steps as the
-    library runs them (one captured HIP graph per step), then time the same
-    number of steps again with HIP events around every launch (eager
-    substages) for the per-kernel figures."""
+    library runs them (one captured HIP graph per step), then time `psteps`
+    further steps (default: as many) with HIP events around every launch
+    (eager substages) for the per-kernel figures."""
+    psteps = steps if psteps is None else psteps
     s = RT.Solver(cfg)
     s.upload(*state)
     tune = None
@@ -111,7 +113,7 @@ def run(cfg, state, steps, warmup, autotune, variant=None):
     s.profile(True)
     s.profile_reset()
     tp0 = time.perf_counter()
-    RT.run_steps(s, st, steps)
+    RT.run_steps(s, st, psteps)
     s.sync()
     tp1 = time.perf_counter()
     prof = {name: s.profile_read(name) for name in KERNELS}
@@ -251,7 +253,9 @@ def copy_calibration():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=1000, help="timed steps per precision (the workload's 1000)")
+    ap.add_argument("--profile-steps", type=int, default=100,
+                    help="steps with per-launch HIP events after the timed ones (at most --steps)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="both", choices=["mixed", "fp64", "both"])
     ap.add_argument("--workload", default="turb256")
@@ -263,7 +267,7 @@ def main():
                     help="run this kernel variant (tbd_set_variant bits) instead of autotuning: one value for "
                          "every precision, or FP64,MIXED")
     ap.add_argument("--spinup-model", default=SPINUP_MODEL, help="trained proposal (python -m turbid_amd.spinup build)")
-    ap.add_argument("--spinup-periods", type=int, default=6, help="cap on each spin-up run; 0 skips the leg")
+    ap.add_argument("--spinup-periods", type=int, default=8, help="cap on each spin-up run; 0 skips the leg")
     ap.add_argument("--spinup-noise", type=float, nargs="+", default=[0.01],
                     help="noise amplitudes of the plain starts (the first is the headline): 0.01, the weakly "
                          "perturbed laminar layer; 0.1, the start of the 256^3 workload")
@@ -294,18 +298,19 @@ def main():
     for prec in precs:
         cfg = dict(cfg64)
         cfg["precision"] = 1 if prec == "mixed" else 0
+        psteps = min(args.steps, args.profile_steps)
         log("bench: %s warmup %d, timing %d steps (graph), then %d steps with per-launch events"
-            % (prec, args.warmup, args.steps, args.steps))
+            % (prec, args.warmup, args.steps, psteps))
         t, tp, feat, prof, tune = run(cfg, state, args.steps, args.warmup, not args.no_autotune,
-                                      variants.get(prec))
+                                      variants.get(prec), psteps)
         if tune:
             log("bench: %s variant kept %s" % (prec, tune["kept"]))
         results[prec] = {"tune": tune, "seconds": t, "steps_per_s": args.steps / t, "ms_per_step": 1e3 * t / args.steps,
-                         "profiled_ms_per_step": 1e3 * tp / args.steps,
+                         "profiled_ms_per_step": 1e3 * tp / psteps, "psteps": psteps,
                          "roofline": roofline(prof), "prof": {k: list(v) for k, v in prof.items()},
                          "cfl": float(feat[19] * cfg["dt"]), "max_u": float(feat[0])}
         log("bench: %s %.2f steps/s (%.3f ms/step; %.3f ms/step with events)"
-            % (prec, args.steps / t, 1e3 * t / args.steps, 1e3 * tp / args.steps))
+            % (prec, args.steps / t, 1e3 * t / args.steps, 1e3 * tp / psteps))
         if not np.isfinite(feat[0]):
             log("bench: %s state is not finite" % prec)
             sys.exit(3)
@@ -334,7 +339,7 @@ def main():
         line["fp64_roofline"] = results["fp64"]["roofline"]
     if "mixed" in results and "fp64" in results:
         line["mixed_vs_fp64"] = results["mixed"]["steps_per_s"] / results["fp64"]["steps_per_s"]
-    line["kernel_ms_per_step"] = {prec: {k: round(v[1] / args.steps, 4) for k, v in r["prof"].items() if v[0]}
+    line["kernel_ms_per_step"] = {prec: {k: round(v[1] / r["psteps"], 4) for k, v in r["prof"].items() if v[0]}
                                   for prec, r in results.items()}
     line["profiled_ms_per_step"] = {prec: r["profiled_ms_per_step"] for prec, r in results.items()}
     line["class_rates"] = {prec: class_rates(r["prof"]) for prec, r in results.items()}
