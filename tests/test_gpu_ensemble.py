@@ -162,3 +162,28 @@ def test_grouped_tuning_restores_the_members():
     ens.step_grouped()
     assert ens.finite()
     ens.close()
+
+
+def test_grouped_members_of_different_physics_match_alone():
+    # a carrier-only member, a member whose scalar drives the carrier
+    # (buoyancy: no scalar overlap, other kernels) and one with erosion: their
+    # launch sequences differ in length, so the merge is ragged
+    cfgs = []
+    for m, (n, nz, kw) in enumerate([(24, 33, dict(nphase=1)),
+                                     (24, 33, dict(affected=1, froude=0.5)),
+                                     (16, 17, dict(erosion_rate=1e-4, critical_shear=1e-5))]):
+        c = W.base_config(n, n, nz, 20.0, 10.0, W.reynolds_from_stokes(600.0), W.TWO_PI / 2000.0, settling=5e-4,
+                          schmidt=0.5, precision=m % 2, **kw)
+        c.update(noise_u=0.1, noise_c=0.01, seed=70 + m)
+        cfgs.append(c)
+    ens = E.Ensemble(cfgs, grouped=True)
+    for _ in range(4):
+        ens.step_grouped()
+    got = [s.download() for s in ens.solvers]
+    ens.close()
+    for cfg, (ug, pg, cg) in zip(cfgs, got):
+        ua, pa, ca = _alone(cfg, 4)
+        assert np.array_equal(ug, ua)
+        assert np.array_equal(pg, pa)
+        if cfg["nphase"] > 1:
+            assert np.array_equal(cg, ca)
