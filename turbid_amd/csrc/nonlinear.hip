@@ -249,12 +249,15 @@ __device__ __forceinline__ void xback_regs(const C2<T>* __restrict__ in, int ny,
     const int ya = y0 + 2 * s;
     const bool direct = kk < mx;
     const int kx = direct ? kk : N - kk;
-    ra[i] = mk<T>((T)0, (T)0);
-    rb[i] = ra[i];
-    if (idx < S * N && (direct || kx < mx)) {
-      if (ya < ny) ra[i] = in[(long long)kx * ny + ya];
-      if (ya + 1 < ny) rb[i] = in[(long long)kx * ny + ya + 1];
-    }
+    // every load reads an in-bounds element (column and rows clamped) and
+    // what lies outside the transform is zeroed afterwards: the loads carry
+    // no branch, so they are all in flight together
+    const bool col = idx < S * N && (direct || kx < mx);
+    const long long kc = (long long)(kx < mx ? kx : mx - 1) * ny;
+    const C2<T> A = in[kc + (ya < ny ? ya : ny - 1)];
+    const C2<T> B = in[kc + (ya + 1 < ny ? ya + 1 : ny - 1)];
+    ra[i] = (col && ya < ny) ? A : mk<T>((T)0, (T)0);
+    rb[i] = (col && ya + 1 < ny) ? B : mk<T>((T)0, (T)0);
   }
   __syncthreads();
 #pragma unroll
@@ -325,13 +328,19 @@ __device__ __forceinline__ void xfwd_regs(C2<T>* w, C2<T>* __restrict__ out, int
 template <typename T, int N1, int N2>
 __device__ __forceinline__ void load_pair(C2<T>* u, const T* __restrict__ base, int N, int ny, int ya, int k1,
                                           T shift) {
-  const T* ra = base + (long long)ya * N + k1;
+  // rows past the last are read at the last row (in bounds) and zeroed: no
+  // branch between the loads, which are then all in flight together
+  const T* ra = base + (long long)(ya < ny ? ya : ny - 1) * N + k1;
+  const T* rb = base + (long long)(ya + 1 < ny ? ya + 1 : ny - 1) * N + k1;
+  T la[N2], lb[N2];
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2) {
-    const T a = (ya < ny) ? ra[N1 * k2] + shift : (T)0;
-    const T b = (ya + 1 < ny) ? ra[N + N1 * k2] + shift : (T)0;
-    u[k2] = mk<T>(a, b);
+    la[k2] = ra[N1 * k2];
+    lb[k2] = rb[N1 * k2];
   }
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2)
+    u[k2] = mk<T>(ya < ny ? la[k2] + shift : (T)0, ya + 1 < ny ? lb[k2] + shift : (T)0);
 }
 
 template <typename T, int N1, int N2, int NT, int MODE>
