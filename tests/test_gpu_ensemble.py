@@ -141,6 +141,10 @@ def test_grouped_step_refuses_an_overlapping_member():
     grp = RT.GroupedEnsemble([s])
     with pytest.raises(RT.TurbidError, match="overlap"):
         grp.step([cfg["dt"]], [0])
+    # the fused x transforms are refused the same way, before anything runs
+    s.set_variant((s.variant() & ~RT.Solver.VARIANT_BITS["overlap"]) | RT.Solver.VARIANT_BITS["fused"])
+    with pytest.raises(RT.TurbidError, match="fused"):
+        grp.step([cfg["dt"]], [0])
     grp.close()
     s.close()
 
@@ -187,3 +191,28 @@ def test_grouped_members_of_different_physics_match_alone():
         assert np.array_equal(pg, pa)
         if cfg["nphase"] > 1:
             assert np.array_equal(cg, ca)
+
+
+@pytest.mark.skip(reason="grouped fused x transforms are switched off: the one run of this test faulted the GPU "
+                         "(illegal memory access in the grouped step) and no cause has been shown")
+def test_grouped_members_with_fused_transforms_match_alone():
+    # the fused x transforms (inverse transforms, products, forward
+    # transforms in one kernel) as grouped launches
+    cfgs = _grouped_members()
+    fused = RT.Solver.VARIANT_BITS["fused"]
+    ens = E.Ensemble(cfgs, grouped=True)
+    for s in ens.solvers:
+        s.set_variant(s.variant() | fused)
+    for _ in range(4):
+        ens.step_grouped()
+    got = [s.download() for s in ens.solvers]
+    ens.close()
+    for cfg, (ug, pg, cg) in zip(cfgs, got):
+        s = RT.Solver(cfg)
+        s.upload(*W.initial_state(cfg))
+        s.set_variant(s.variant() | fused)
+        RT.run_steps(s, RT.Stepper(cfg["dt"]), 4)
+        ua, pa, ca = s.download()
+        s.close()
+        assert np.array_equal(ug, ua)
+        assert np.array_equal(cg, ca)

@@ -720,7 +720,7 @@ void Context::record_step(double time, int alternation, Recorder& r) {
   (void)hipGraphDestroy(g);
   TBD_HIP(e);
   TBD_CHECK(nodes == 0, "Context::record_step > the step issued work outside the recorded launches "
-                        "(fused x transforms, LDS Stockham transforms or a second stream)");
+                        "(LDS Stockham transforms or a second stream)");
 }
 
 void Context::replayed_step() {

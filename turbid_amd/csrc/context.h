@@ -140,12 +140,14 @@ class Context {
   // advances (wall-row slot, previous dt) advanced as by step().  The
   // forcing is read from the device pair force_ (written before each replay).
   // Throws if anything of the step was issued on the context's stream
-  // rather than recorded (fused x transforms, the LDS Stockham transforms).
+  // rather than recorded (a second stream, the LDS Stockham transforms).
   void record_step(double time, int alternation, Recorder& r);
   // the host-side state advance of a step replayed from a recording
   void replayed_step();
   // whether record_step can record this context's step: no second stream,
-  // no fused x transforms, the register transforms in x and y, no profiling
+  // no fused x transforms (their grouped instances are switched off: a
+  // grouped step with them faulted the GPU once, cause not found), the
+  // register transforms in x and y, no profiling
   bool groupable() const {
     return !overlap_ && !fused_ && !prof.on && fft_.two_level && fft_has_two_level(d.nx) &&
            fft_has_two_level(d.ny);

@@ -165,8 +165,8 @@ void Ensemble::build(Program& p, const double* time, const int* alternation) {
 void Ensemble::step(const double* time, const int* alternation) {
   const int n = (int)m_.size();
   for (Context* c : m_)
-    TBD_CHECK(c->groupable(), "Ensemble::step > a member runs with the overlap or fused-transform variant bit, the "
-                              "LDS Stockham transforms or the profiler: clear them first");
+    TBD_CHECK(c->groupable(), "Ensemble::step > a member runs with the overlap or fused-transform variant bit, "
+                              "the LDS Stockham transforms or the profiler: clear them first");
   Program* prog = nullptr;
   for (auto& p : progs_) {
     bool same = true;

@@ -2,6 +2,7 @@
 #include "nonlinear.h"
 
 #include "fft_device.h"
+#include "group.h"
 #include "regfft.h"
 
 namespace tbd {
@@ -343,14 +344,16 @@ __device__ __forceinline__ void load_pair(C2<T>* u, const T* __restrict__ base, 
     u[k2] = mk<T>(ya < ny ? la[k2] + shift : (T)0, ya + 1 < ny ? lb[k2] + shift : (T)0);
 }
 
-template <typename T, int N1, int N2, int NT, int MODE>
-__global__ __launch_bounds__(NT) void k_xfused2(XfArgs a) {
+template <typename T, int N1, int N2, int NT, int MODE, bool G>
+__global__ __launch_bounds__(NT) void k_xfused2(XfArgs a0, GroupP grp) {
+  Blk bk;
+  const XfArgs& a = member<G>(a0, grp, bk);
   constexpr int N = N1 * N2;
   constexpr int S = xf_seqs<N1, N2, NT>();
   __shared__ __align__(16) char raw[sizeof(C2<T>) * xf_lds<N1, N2, NT>()];
   C2<T>* L = reinterpret_cast<C2<T>*>(raw);
-  const int z = blockIdx.y;
-  const int y0 = blockIdx.x * 2 * S;
+  const int z = bk.y;
+  const int y0 = bk.x * 2 * S;
   const long long plane = (long long)a.mx * a.ny;
   const long long pstride = (long long)a.nz * plane;
   const C2<T>* gin = (const C2<T>*)a.gin;
@@ -368,8 +371,8 @@ __global__ __launch_bounds__(NT) void k_xfused2(XfArgs a) {
   const int k1 = t % N1, s = t / N1;
   const int ya = y0 + 2 * s;
   if (MODE == 0) {
-    // adv_i = sum_j (U_j + shift_j) g_ij for the output i = blockIdx.z
-    const int i = blockIdx.z;
+    // adv_i = sum_j (U_j + shift_j) g_ij for the output i = the grid's z
+    const int i = bk.z;
     C2<T> acc[N2];
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) acc[k2] = mk<T>((T)0, (T)0);
@@ -447,7 +450,8 @@ static bool launch_xf2(const FftPlan& p, const XfArgs& a, hipStream_t st) {
   case NN: {                                                                                  \
     constexpr int S = xf_seqs<A, B, NT>();                                                    \
     dim3 grid(ceil_div(p.d.ny, 2 * S), p.d.nz, MODE == 0 ? a.nout : 1);                       \
-    hipLaunchKernelGGL((k_xfused2<T, A, B, NT, MODE>), grid, dim3(NT), 0, st, a);             \
+    glaunch(k_xfused2<T, A, B, NT, MODE, false>, k_xfused2<T, A, B, NT, MODE, true>, grid,    \
+            dim3(NT), 0, st, a);                                                             \
     return true;                                                                              \
   }
     TBD_XF_LENGTHS(TBD_CASE)
