@@ -455,7 +455,15 @@ __global__ __launch_bounds__(256) void k_minmax_part(MmA<T> a0, GroupP grp) {
   double* part = A.part;
   __shared__ double lo[4], hi[4];
   double a = 1e300, b = -1e300;
-  for (long long i = bk.x * 256LL + threadIdx.x; i < n; i += (long long)bk.gx * 256) {
+  // four independent loads per step of the grid-stride loop, then the rest
+  const long long stride = (long long)bk.gx * 256;
+  long long i = bk.x * 256LL + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const T v0 = x[i], v1 = x[i + stride], v2 = x[i + 2 * stride], v3 = x[i + 3 * stride];
+    a = fmin(fmin(a, (double)v0), fmin((double)v1, fmin((double)v2, (double)v3)));
+    b = fmax(fmax(b, (double)v0), fmax((double)v1, fmax((double)v2, (double)v3)));
+  }
+  for (; i < n; i += stride) {
     const double v = (double)x[i];
     a = fmin(a, v);
     b = fmax(b, v);
