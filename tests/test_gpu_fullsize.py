@@ -10,7 +10,7 @@ import pytest
 from oracle import oracle as O
 from turbid_amd import runtime as RT
 from turbid_amd import workloads as W
-from helpers import rel_l2
+from helpers import oracle_steps, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -74,9 +74,41 @@ def test_full_size_step_wide_transform_kernels(oracle_step):
     assert rel_l2(pg, po) < 1e-7
 
 
-def test_384_grid_step():
-    # the same physics on the 384 x 384 x 257 grid, FP64, with the variant the
-    # autotuner keeps there (all four transform passes wide)
+@pytest.fixture(scope="module")
+def oracle_two_steps():
+    # two steps: the convective form of the nonlinear terms, then the
+    # divergence form (the alternation toggles every step)
+    cfg = W.workload("turb256", precision=0)
+    state = W.initial_state(cfg)
+    ref = oracle_steps(cfg, state, 2, NTHREADS)
+    return cfg, state, ref
+
+
+@pytest.mark.parametrize("precision,tol,ptol", [(0, 1e-9, 1e-7), (1, 1e-5, 1e-5)])
+def test_full_size_fused_transforms_both_forms(oracle_two_steps, precision, tol, ptol):
+    # the variant the autotuner keeps at this size: x transforms fused with
+    # the products (k_xfused2, all three modes), fused right-hand side and
+    # pressure chain, scalar on a second stream, wide transform passes
+    cfg, state, (uo, po, co) = oracle_two_steps
+    c = dict(cfg)
+    c["precision"] = precision
+    g = RT.Solver(c)
+    g.upload(*state)
+    bits = RT.Solver.VARIANT_BITS
+    g.set_variant(bits["fused"] | bits["rhs"] | bits["overlap"] | RT.Solver.WIDE_FFT)
+    RT.run_steps(g, RT.Stepper(c["dt"]), 2)
+    ug, pg, cg = g.download()
+    g.close()
+    uo, ug = uo.reshape(3, -1), ug.reshape(3, -1)
+    errs = {"u": rel_l2(ug[0], uo[0]), "v": rel_l2(ug[1], uo[1]), "w": rel_l2(ug[2], uo[2]),
+            "p": rel_l2(pg, po), "c": rel_l2(cg, co)}
+    print("256x256x257, precision %d, fused transforms, two steps: %s" % (precision, errs))
+    assert max(errs["u"], errs["v"], errs["w"], errs["c"]) < tol, errs
+    assert errs["p"] < ptol, errs
+
+
+@pytest.fixture(scope="module")
+def oracle_step_384():
     cfg = W.workload("turb384", precision=0)
     state = W.initial_state(cfg)
     o = O.Oracle(cfg, nthreads=NTHREADS)
@@ -85,20 +117,32 @@ def test_384_grid_step():
     st.update(0.0)
     for s in range(3):
         o.substage(s, st.time, st.alternation)
-    uo, po, co = o.get_state()
+    ref = o.get_state()
     o.close()
-    g = RT.Solver(cfg)
+    return cfg, state, st.time, st.alternation, ref
+
+
+@pytest.mark.parametrize("precision,tol,ptol", [(0, 1e-9, 1e-7), (1, 1e-5, 1e-5)])
+def test_384_grid_step(oracle_step_384, precision, tol, ptol):
+    # the same physics on the 384 x 384 x 257 grid with the variants the
+    # autotuner keeps there: fused x transforms, every transform pass wide and
+    # in the swapped radix order (384 = 16 x 24 as 24 x 16)
+    cfg, state, time, alternation, (uo, po, co) = oracle_step_384
+    c = dict(cfg)
+    c["precision"] = precision
+    g = RT.Solver(c)
     g.upload(*state)
-    g.set_variant(g.variant() | RT.Solver.WIDE_FFT)
-    g.step(st.time, st.alternation)
+    bits = RT.Solver.VARIANT_BITS
+    g.set_variant(bits["fused"] | bits["rhs"] | bits["overlap"] | RT.Solver.WIDE_FFT | RT.Solver.SWAP_FFT)
+    g.step(time, alternation)
     ug, pg, cg = g.download()
     g.close()
     uo, ug = uo.reshape(3, -1), ug.reshape(3, -1)
     errs = {"u": rel_l2(ug[0], uo[0]), "v": rel_l2(ug[1], uo[1]), "w": rel_l2(ug[2], uo[2]),
             "p": rel_l2(pg, po), "c": rel_l2(cg, co)}
-    print("384x384x257, FP64, one step, relative L2 per field: %s" % errs)
-    assert max(errs["u"], errs["v"], errs["w"], errs["c"]) < 1e-9, errs
-    assert errs["p"] < 1e-7, errs
+    print("384x384x257, precision %d, one step, relative L2 per field: %s" % (precision, errs))
+    assert max(errs["u"], errs["v"], errs["w"], errs["c"]) < tol, errs
+    assert errs["p"] < ptol, errs
 
 
 def test_full_size_transforms():
