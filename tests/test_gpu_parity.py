@@ -109,8 +109,10 @@ def test_operators_match_oracle():
     assert rel_l2(d2g, d2o) < 1e-15
 
 
-def _run_pair(cfg, nsteps):
+def _run_pair(cfg, nsteps, bits=0):
     o, g = _pair(cfg)
+    if bits:
+        g.set_variant(g.variant() | bits)
     st = RT.Stepper(cfg["dt"])
     feat = np.zeros(50)
     for n in range(nsteps):
@@ -191,6 +193,21 @@ def test_steps_fused_path(monkeypatch):
     ug, pg, cg = g.download()
     assert rel_l2(ug, uo) < 1e-9
     assert rel_l2(cg, co) < 1e-9
+
+
+@pytest.mark.parametrize("nx", [96, 192, 12])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-9), (1, 1e-5)])
+def test_steps_fused_path_lengths(nx, precision, tol):
+    # the fused x transforms at lengths whose workgroups end in a partial row
+    # of lanes (96: 960 positions over 1024 lanes in FP64, 2016 over 2048 in
+    # FP32), two steps (both nonlinear forms) against the oracle
+    cfg = small_config(nx=nx, ny=12, nz=17, precision=precision)
+    o, g = _run_pair(cfg, 2, RT.Solver.VARIANT_BITS["fused"])
+    assert g.variant() & RT.Solver.VARIANT_BITS["fused"]
+    uo, po, co = o.get_state()
+    ug, pg, cg = g.download()
+    assert rel_l2(ug, uo) < tol
+    assert rel_l2(cg, co) < tol
 
 
 @pytest.mark.parametrize("shape", [(54, 36, 3), (256, 256, 2), (24, 48, 5)])

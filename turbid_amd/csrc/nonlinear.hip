@@ -254,7 +254,9 @@ __device__ __forceinline__ void xback_regs(const C2<T>* __restrict__ in, int ny,
     // what lies outside the transform is zeroed afterwards: the loads carry
     // no branch, so they are all in flight together
     const bool col = idx < S * N && (direct || kx < mx);
-    const long long kc = (long long)(kx < mx ? kx : mx - 1) * ny;
+    // lanes past the last sequence (idx >= S N, when NT does not divide
+    // S N) have kk > N and so kx < 0: their column is clamped too
+    const long long kc = (long long)(kx < 0 ? 0 : (kx < mx ? kx : mx - 1)) * ny;
     const C2<T> A = in[kc + (ya < ny ? ya : ny - 1)];
     const C2<T> B = in[kc + (ya + 1 < ny ? ya + 1 : ny - 1)];
     ra[i] = (col && ya < ny) ? A : mk<T>((T)0, (T)0);
