@@ -210,6 +210,31 @@ def test_steps_fused_path_lengths(nx, precision, tol):
     assert rel_l2(cg, co) < tol
 
 
+@pytest.mark.parametrize("precision,tol,ptol", [(0, 1e-12, 1e-8), (1, 1e-5, 1e-5)])
+def test_fused_x_derivative_forms_agree(precision, tol, ptol, monkeypatch):
+    # on the fused conv path the x derivatives are formed from the velocity's
+    # x-pencils (i kx after the y pass); TBD_FUSED_DX=0 keeps the form where
+    # the contraction writes i kx u_i and all nine gradients take the y pass:
+    # two steps (both nonlinear forms) agree to round-off in u and c; p, from
+    # the Neumann pressure solve, amplifies round-off (5.7e-11 measured in
+    # FP64 at Nz = 33, as in the pressure's tolerance elsewhere)
+    cfg = small_config(nx=48, ny=36, nz=33, precision=precision)
+    out = []
+    for dx in ("1", "0"):
+        monkeypatch.setenv("TBD_FUSED_DX", dx)
+        g = RT.Solver(cfg)
+        g.upload(*W.initial_state(cfg))
+        g.set_variant(g.variant() | RT.Solver.VARIANT_BITS["fused"])
+        RT.run_steps(g, RT.Stepper(cfg["dt"]), 2)
+        out.append(g.download())
+        g.close()
+    errs = [rel_l2(a, b) for a, b in zip(out[0], out[1])]
+    print("precision %d, fused x-derivative forms, relative L2 (u, p, c): %s" % (precision, errs))
+    # not bit-equal: the switch did change the path
+    assert 0.0 < max(errs[0], errs[2]) < tol
+    assert errs[1] < ptol
+
+
 @pytest.mark.parametrize("shape", [(54, 36, 3), (256, 256, 2), (24, 48, 5)])
 @pytest.mark.parametrize("stockham", ["0", "1"])
 def test_dft_both_kernel_families(shape, stockham, monkeypatch):

@@ -250,6 +250,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
     want_rhs_ = !(e[0] == '1');
     pin_rhs_ = true;
   }
+  if (const char* e = std::getenv("TBD_FUSED_DX")) fused_dx_ = std::string(e) != "0";
   dt_ = olddt_ = cfg.dt;
   const double c1[3] = {0.0 / 1.0, -5.0 / 9.0, -153.0 / 128.0};    // stepper_time_rk3.cc:29-31
   const double c2[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
@@ -311,6 +312,7 @@ Context::Context(const tbd_config& c) : cfg(c) {
   gphys_ = alloc(9 * Rt);
   aphys_ = alloc(3 * Rt);
   tmp_ = alloc(std::max((size_t)9 * 2 * ts, sizeof(double2)) * fft_tmp_elems(d, 1));
+  uxp_ = alloc((size_t)3 * 2 * ts * fft_tmp_elems(d, 1));
   for (int s = 0; s < 2; ++s)
     for (int b = 0; b < 2; ++b) {
       wall_[s][b] = (double2*)alloc(sizeof(double2) * 3 * d.P);
@@ -748,9 +750,9 @@ void Context::upload(const double* uvw, const double* p, const double* c) {
   olddt_ = dt_;
   // uvwnonlinear = DftBackward(uvw) (phase_carrier_flat.cc:233-234)
   if (cfg.precision == 1)
-    fft_backward<float, double2>(fft_, uvw_, d.S, (float*)uvwnl_, d.R, 3, 0, d.nz, (float2*)tmp_, st_);
+    fft_backward<float, double2>(fft_, uvw_, d.S, (float*)uvwnl_, d.R, 3, 0, d.nz, (float2*)uxp_, st_);
   else
-    fft_backward<double, double2>(fft_, uvw_, d.S, (double*)uvwnl_, d.R, 3, 0, d.nz, (double2*)tmp_, st_);
+    fft_backward<double, double2>(fft_, uvw_, d.S, (double*)uvwnl_, d.R, 3, 0, d.nz, (double2*)uxp_, st_);
   sync();
 }
 
@@ -855,21 +857,27 @@ void Context::carrier_predict(int stage, double time, int alternation) {
   launch(KC_OTHER, 0.0, [&] { wall_mean_rows(d1_, lda_, d.nz, uvw_, S, wallmean_, st_); });
   if (!alternation) {
     // convective form (phase_carrier_flat.cc:784-806)
+    // on the fused path with the register transforms, the x derivatives are
+    // taken from the velocity's x-pencils (i kx commutes with the y pass):
+    // the contraction writes only i ky u_i and D1 u_i, and six components
+    // take the y pass instead of nine
+    const bool dx_fused = fused_ && fused_dx_ && xfused_registers(fft_);
     ZArgs a;
-    a.mode = Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
+    a.mode = dx_fused ? Z_GRADYZ : Z_GRAD; a.nz = d.nz; a.P = d.P; a.nyc = d.nyc; set_d1<T>(a);
     a.x = uvw_; a.xs = S; a.y = grad; a.ys = S; a.kx = kx_; a.ky = ky_;
     launch(is32<T>() ? KC_Z32 : KC_Z64, 3 * zwork(), [&] { zgemm<T, double2, CT>(a, 3, st_); });
     const bool erosion_planes = cfg.nphase > 1 && cfg.scalar_bc == 1;
     if (fused_) {
-      // y^-1 of the nine gradients; x^-1, u.grad u_i, x in one kernel; y
-      ypb<T, CT>(grad, S, tmp, 9, 0, d.nz);
+      // y^-1 of the gradients; x^-1, u.grad u_i, x in one kernel; y
+      ypb<T, CT>(grad, S, tmp, dx_fused ? 6 : 9, 0, d.nz);
       // outputs go to a separate x-pencil buffer: the three outputs of a row
       // tile are written by different workgroups while the inputs are read
       CT* xout = (CT*)gphys_;
       launch(is32<T>() ? KC_XFU32 : KC_XFU64, 12 * xpb_ * sizeof(CT) + 3.0 * R * sizeof(T), [&] {
         xfused_conv<T>(fft_, tmp, uvwnl, R, (T)0, (T)0, (T)0, 3, xout,
                        erosion_planes ? (T*)dudz_bot_[stage & 1] : nullptr,
-                       erosion_planes ? (T*)dvdz_bot_[stage & 1] : nullptr, st_);
+                       erosion_planes ? (T*)dvdz_bot_[stage & 1] : nullptr, st_,
+                       dx_fused ? (const CT*)uxp_ : nullptr, dx_fused ? 2 * kPi / cfg.lx : 0.0);
       });
       ypf<T, CT>(xout, adv, S, 3, 0, d.nz);
     } else {
@@ -1030,7 +1038,7 @@ void Context::carrier_correct(int stage) {
     }
   }
   // uvwnonlinear = DftBackward(uvw) (:625-626)
-  bwd<T, double2>(uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)tmp_);
+  bwd<T, double2>(uvw_, S, (T*)uvwnl_, R, 3, 0, d.nz, (C2<T>*)uxp_);
   if (stage == 2) {   // diagnostics (:633-703)
     fill_zero(fl_, sizeof(double) * 20, st_);
     launch(KC_OTHER, 0.0, [&] {
@@ -1278,9 +1286,9 @@ void Context::project() {
     }
   olddt_ = dt_;
   if (cfg.precision == 1)
-    fft_backward<float, double2>(fft_, uvw_, d.S, (float*)uvwnl_, d.R, 3, 0, d.nz, (float2*)tmp_, st_);
+    fft_backward<float, double2>(fft_, uvw_, d.S, (float*)uvwnl_, d.R, 3, 0, d.nz, (float2*)uxp_, st_);
   else
-    fft_backward<double, double2>(fft_, uvw_, d.S, (double*)uvwnl_, d.R, 3, 0, d.nz, (double2*)tmp_, st_);
+    fft_backward<double, double2>(fft_, uvw_, d.S, (double*)uvwnl_, d.R, 3, 0, d.nz, (double2*)uxp_, st_);
   sync();
 }
 

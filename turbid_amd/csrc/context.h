@@ -220,6 +220,9 @@ class Context {
   double xpb_ = 0;          // complex elements of one x-pencil component (nz*mx*ny)
   bool opentop_ = false;
   bool fused_ = false;      // x transforms fused around the nonlinear products (TBD_FUSED=1)
+  // on the fused conv path, x derivatives from the velocity's x-pencils
+  // (TBD_FUSED_DX=0: the nine-gradient form)
+  bool fused_dx_ = true;
   double dt_ = 0, olddt_ = 0;
   double coef_[3][3];
 
@@ -287,6 +290,10 @@ class Context {
   void* aphys_ = nullptr;  // 3
   double* cphys_ = nullptr;
   void* tmp_ = nullptr;    // transform intermediate (bytes for 9 components)
+  // x-pencils of u, v, w left by their inverse transform to physical space
+  // (after every Correct and every state upload); the fused conv form takes
+  // the x derivatives from them
+  void* uxp_ = nullptr;
   // wall rows of grad p: m1/m2 (bottom, top) ping-pong, and open-top rows
   double2* wall_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [slot][bottom/top], 3P each
   double2* wall2_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // 2P each

@@ -20,9 +20,17 @@ namespace tbd {
 // bot0/bot1 (optional, nout = 3): physical bottom plane of du/dz and dv/dz.
 // With nout = 3, adv must not overlap g (the outputs of a row tile are written
 // by different workgroups); with nout = 1 it may be g itself.
+// ux (optional, nout = 3, xfused_registers(p)): the x-pencils of u, v, w
+// (the intermediate of their inverse y pass); then g holds the six
+// components du/dy, dv/dy, dw/dy, du/dz, dv/dz, dw/dz and du_i/dx is i kx
+// times ux_i, kx = dkx * (spectral x index), formed in the kernel.
 template <typename T>
 void xfused_conv(const FftPlan& p, const C2<T>* g, const T* U, long long R, T su, T sv, T sw, int nout,
-                 C2<T>* adv, T* bot0, T* bot1, hipStream_t st);
+                 C2<T>* adv, T* bot0, T* bot1, hipStream_t st, const C2<T>* ux = nullptr, double dkx = 0.0);
+
+// whether the fused kernels run as the two-level register transform at this
+// plan's x length (the form that takes ux above)
+bool xfused_registers(const FftPlan& p);
 
 // the six products uu, uv, uw, vv, vw, ww of U, transformed along x
 template <typename T>

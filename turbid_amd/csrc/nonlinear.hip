@@ -22,6 +22,10 @@ struct XfArgs {
   void* bot0 = nullptr;
   void* bot1 = nullptr;
   double su = 0, sv = 0, sw = 0;
+  // conv form with the x derivatives formed here: the x-pencils of u, v, w,
+  // and kx = dkx * (spectral x index)
+  const void* ux = nullptr;
+  double dkx = 0;
 };
 
 // inverse x transform input: zero-padded, Hermitian-completed (halfcomplex)
@@ -233,10 +237,12 @@ __host__ __device__ constexpr int xf_lds() {
 }
 
 // inverse x transform of one x-pencil plane (kx < mx, y fastest) for the
-// rows y0 .. y0+2S: task (k1, s) receives w[k2] = row pair at x = k1 + N1*k2
+// rows y0 .. y0+2S: task (k1, s) receives w[k2] = row pair at x = k1 + N1*k2.
+// With dkx != 0 the plane is first multiplied by i kx, kx = dkx * index (the
+// x derivative, as Gradient's i kx multiplier applies it in spectral space).
 template <typename T, int N1, int N2, int NT>
 __device__ __forceinline__ void xback_regs(const C2<T>* __restrict__ in, int ny, int mx, int y0, C2<T>* L,
-                                           const C2<T>* __restrict__ tw, C2<T>* w) {
+                                           const C2<T>* __restrict__ tw, C2<T>* w, double dkx = 0.0) {
   constexpr int N = N1 * N2;
   constexpr int S = xf_seqs<N1, N2, NT>();
   const int t = threadIdx.x;
@@ -273,6 +279,11 @@ __device__ __forceinline__ void xback_regs(const C2<T>* __restrict__ in, int ny,
       C2<T> z = mk<T>((T)0, (T)0);
       if (direct || kx < mx) {
         C2<T> A = ra[i], B = rb[i];
+        if (dkx != 0.0) {
+          const T k = (T)(dkx * (double)kx);
+          A = mk<T>(-(k * A.y), k * A.x);
+          B = mk<T>(-(k * B.y), k * B.x);
+        }
         if (kx == 0 || 2 * kx == N) { A.y = (T)0; B.y = (T)0; }
         z = direct ? mk<T>(A.x - B.y, A.y + B.x) : mk<T>(A.x + B.y, B.x - A.y);
       }
@@ -379,9 +390,12 @@ __global__ __launch_bounds__(NT) void k_xfused2(XfArgs a0, GroupP grp) {
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) acc[k2] = mk<T>((T)0, (T)0);
     for (int j = 0; j < 3; ++j) {
-      const int gcomp = (a.nout == 3) ? 3 * i + j : j;
+      // g_ij from g (9 components, or 3 for the scalar), or with ux set:
+      // j = 0 from ux_i times i kx, j = 1, 2 from g's components i, 3 + i
+      const C2<T>* src = a.ux ? (j == 0 ? (const C2<T>*)a.ux + i * pstride : gin + (j == 1 ? i : 3 + i) * pstride)
+                              : gin + ((a.nout == 3) ? 3 * i + j : j) * pstride;
       C2<T> w[N2];
-      xback_regs<T, N1, N2, NT>(gin + gcomp * pstride + z * plane, a.ny, a.mx, y0, L, tw, w);
+      xback_regs<T, N1, N2, NT>(src + z * plane, a.ny, a.mx, y0, L, tw, w, (a.ux && j == 0) ? a.dkx : 0.0);
       if (act) {
         if (a.bot0 && z == 0 && j == 2 && i < 2) {   // physical bottom plane of du/dz, dv/dz
           T* bot = (T*)(i == 0 ? a.bot0 : a.bot1) + (long long)ya * N + k1;
@@ -491,6 +505,7 @@ static void launch_xf(const FftPlan& p, const XfArgs& a, hipStream_t st) {
     TBD_HIP(hipGetLastError());
     return;
   }
+  TBD_CHECK(a.ux == nullptr, "xfused_conv > x derivatives from the x-pencils need the register transform length");
   const int n = p.d.nx;
   const size_t lds = sizeof(C2<T>) * ((size_t)a.nseq * lds_ld(n) + n);
   dim3 grid(ceil_div(p.d.ny, 2 * a.nseq), p.d.nz);
@@ -498,12 +513,26 @@ static void launch_xf(const FftPlan& p, const XfArgs& a, hipStream_t st) {
   TBD_HIP(hipGetLastError());
 }
 
+bool xfused_registers(const FftPlan& p) {
+  if (!p.two_level) return false;
+  switch (p.d.nx) {
+#define TBD_CASE(NN, A, B) case NN:
+    TBD_XF_LENGTHS(TBD_CASE)
+#undef TBD_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
 template <typename T>
 void xfused_conv(const FftPlan& p, const C2<T>* g, const T* U, long long R, T su, T sv, T sw, int nout, C2<T>* adv,
-                 T* bot0, T* bot1, hipStream_t st) {
+                 T* bot0, T* bot1, hipStream_t st, const C2<T>* ux, double dkx) {
   TBD_CHECK(nout == 1 || nout == 3, "xfused_conv > nout must be 1 or 3");
+  TBD_CHECK(ux == nullptr || (nout == 3 && dkx != 0.0), "xfused_conv > ux needs nout = 3 and dkx");
   XfArgs a = make_args<T>(p, R);
   a.gin = g; a.U = U; a.gout = adv; a.nout = nout;
+  a.ux = ux; a.dkx = dkx;
   a.su = su; a.sv = sv; a.sw = sw;
   a.bot0 = (nout == 3) ? bot0 : nullptr;
   a.bot1 = (nout == 3) ? bot1 : nullptr;
@@ -528,9 +557,9 @@ void xfused_cdiv(const FftPlan& p, const C2<T>* c, const T* U, long long R, T su
 }
 
 template void xfused_conv<double>(const FftPlan&, const double2*, const double*, long long, double, double, double,
-                                  int, double2*, double*, double*, hipStream_t);
+                                  int, double2*, double*, double*, hipStream_t, const double2*, double);
 template void xfused_conv<float>(const FftPlan&, const float2*, const float*, long long, float, float, float, int,
-                                 float2*, float*, float*, hipStream_t);
+                                 float2*, float*, float*, hipStream_t, const float2*, double);
 template void xfused_div6<double>(const FftPlan&, const double*, long long, double2*, hipStream_t);
 template void xfused_div6<float>(const FftPlan&, const float*, long long, float2*, hipStream_t);
 template void xfused_cdiv<double>(const FftPlan&, const double2*, const double*, long long, double, double, double,

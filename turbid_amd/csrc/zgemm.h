@@ -19,6 +19,8 @@ enum ZMode {
   Z_DIV = 3,      // y = q ((M x + i kx e0) + i ky e1)                     (Divergence, pressure rhs)
   Z_CORRECT = 4,  // y_f = e_f - cf g_f, g = (i kx x, i ky x, M x); mean w := 0; wall rows saved
   Z_GRADTMP = 5,  // y_f = (i kx e0_f + i ky e1_f) + M x_f                  (GradientTmp summed)
+  Z_GRADYZ = 6,   // y_f = i ky x_f, y_{3+f} = M x_f (f < 3): Z_GRAD without the x
+                  // derivative, which the fused x transform takes from x_f's x-pencil
 };
 
 struct ZArgs {

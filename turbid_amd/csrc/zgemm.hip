@@ -428,6 +428,7 @@ __device__ __forceinline__ EpiIn<T> epi_load(const ZArgs& a, int f, long long of
   const TX* x = zx<TX>(a, f);
   switch (a.mode) {
     case Z_GRAD:
+    case Z_GRADYZ:
       in.x = ld<T, TX>(x + off);
       break;
     case Z_RHS:
@@ -472,6 +473,10 @@ __device__ __forceinline__ void zepilogue(const ZArgs& a, int f, long long col, 
       st<TY, T>(y + (3 * f + 0) * a.ys + off, cmuli<T>(kx, xv));
       st<TY, T>(y + (3 * f + 1) * a.ys + off, cmuli<T>(ky, xv));
       st<TY, T>(y + (3 * f + 2) * a.ys + off, v);
+      break;
+    case Z_GRADYZ:
+      st<TY, T>(y + f * a.ys + off, cmuli<T>(ky, xv));
+      st<TY, T>(y + (3 + f) * a.ys + off, v);
       break;
     case Z_RHS: {
       const C2<T> h = in.e0;
@@ -553,10 +558,11 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
   const int ncol = (int)min((long long)pn.ncol, a.P - c0);
   const TX* x = zx<TX>(a, f) + c0 * nz;
   const C2<T> zero = mk<T>((T)0, (T)0);
-  // Z_GRAD on the even/odd split: the two pointwise outputs (i kx x, i ky x)
-  // are written from the load pass, so the epilogue neither re-reads x nor
-  // writes them
-  const bool grad_early = a.mode == Z_GRAD && pn.parity != 0;
+  // Z_GRAD (Z_GRADYZ) on the even/odd split: the pointwise outputs (i kx x,
+  // i ky x; i ky x) are written from the load pass, so the epilogue neither
+  // re-reads x nor writes them
+  const bool yz = a.mode == Z_GRADYZ;
+  const bool grad_early = (a.mode == Z_GRAD || yz) && pn.parity != 0;
   __shared__ double2 kv[kSplitMaxCol];   // (kx, ky) of each panel column
   if (grad_early) {
     for (int c = threadIdx.x; c < pn.ncol; c += kNT) kv[c] = kcol(a.kx, a.ky, c0 + c, a.P, a.nyc);
@@ -589,12 +595,14 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
         TY* y = (TY*)a.y;
         const T kx = (T)kv[c].x, ky = (T)kv[c].y;
         const long long off = (c0 + c) * nz + j;
-        st<TY, T>(y + (3 * f + 0) * a.ys + off, cmuli<T>(kx, r.lo));
-        st<TY, T>(y + (3 * f + 1) * a.ys + off, cmuli<T>(ky, r.lo));
+        TY* yx = y + (3 * f + 0) * a.ys;
+        TY* yy = y + (yz ? f : 3 * f + 1) * a.ys;
+        if (!yz) st<TY, T>(yx + off, cmuli<T>(kx, r.lo));
+        st<TY, T>(yy + off, cmuli<T>(ky, r.lo));
         if (j < M) {
           const long long offh = (c0 + c) * nz + nz - 1 - j;
-          st<TY, T>(y + (3 * f + 0) * a.ys + offh, cmuli<T>(kx, r.hi));
-          st<TY, T>(y + (3 * f + 1) * a.ys + offh, cmuli<T>(ky, r.hi));
+          if (!yz) st<TY, T>(yx + offh, cmuli<T>(kx, r.hi));
+          st<TY, T>(yy + offh, cmuli<T>(ky, r.hi));
         }
       }
       if (j < M) {
@@ -610,7 +618,7 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
   __syncthreads();
   const int M = (nz - 1) / 2;
   if (grad_early) {
-    TY* y = (TY*)a.y + (3 * f + 2) * a.ys + c0 * nz;
+    TY* y = (TY*)a.y + (yz ? 3 + f : 3 * f + 2) * a.ys + c0 * nz;
     for (int idx = threadIdx.x; idx < ncol * nz; idx += kNT) {
       const int c = idx / nz, row = idx - c * nz;
       // y_i = s_i + d_i, y_{N-1-i} = -(s_i - d_i) (D1 centro-antisymmetric), i <= M
@@ -642,6 +650,7 @@ __global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH
 template <typename T, typename TX, typename TY>
 void zgemm(const ZArgs& a, int nfield, hipStream_t st) {
   TBD_CHECK(a.nz >= 4 && a.lda % 4 == 0 && a.P > 0, "zgemm > bad shape");
+  TBD_CHECK(a.mode != Z_GRADYZ || nfield <= 3, "zgemm > Z_GRADYZ takes at most 3 fields");
   TBD_CHECK(a.parity == 0 ? a.lda >= a.nz : (a.nz % 2 == 1 && a.lda >= (a.nz + 1) / 2),
             "zgemm > operator layout does not match the column length");
   const Panel pn = make_panel(a.nz, sizeof(C2<T>), kLdsLimit, a.parity, a.ncg_mult);
