@@ -360,7 +360,7 @@ __device__ __forceinline__ void load_pair(C2<T>* u, const T* __restrict__ base, 
 // FP32 at lengths up to 256: three waves per SIMD (168 VGPRs; the conv and
 // div forms at 256 spill 12 / 48 bytes per lane) instead of two
 template <typename T, int N1, int N2, int NT, int MODE, bool G>
-__global__ __launch_bounds__(NT, (sizeof(T) == 4 && NT == 256 && N1 * N2 <= 256) ? 3 : 1) void k_xfused2(XfArgs a0,
+__global__ __launch_bounds__(NT, (sizeof(T) == 4 && NT == 256) ? (N1 * N2 <= 256 ? 3 : 2) : 1) void k_xfused2(XfArgs a0,
                                                                                             GroupP grp) {
   Blk bk;
   const XfArgs& a = member<G>(a0, grp, bk);
