@@ -358,8 +358,10 @@ def main():
     if args.ensemble_steps > 0:
         from turbid_amd import ensemble as E
         log("bench: ensemble of 16 members, %d steps per leg" % args.ensemble_steps)
+        # the grouped leg tunes its members' variant bits together first, as
+        # the single-grid legs run the variant the autotuner keeps
         line["ensemble"] = E.measure(E.member_configs(16, seed=0, precision=1), args.ensemble_steps, 3,
-                                     log=lambda m: log("bench: " + m))
+                                     tune=True, log=lambda m: log("bench: " + m))
     else:
         line["ensemble"] = None
     if rank == 0 and not args.no_cpu_baseline:
