@@ -544,7 +544,7 @@ struct ZL {
   Panel pn;
 };
 template <typename T, typename TX, typename TY, int DEPTH, bool G>
-__global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 6 : 4) : (DEPTH == 1 ? 4 : 2)) void k_zgemm(ZL l0, GroupP grp) {
+__global__ __launch_bounds__(kNT, sizeof(T) == 4 ? (DEPTH == 1 ? 8 : 4) : (DEPTH == 1 ? 4 : 2)) void k_zgemm(ZL l0, GroupP grp) {
   Blk bk;
   const ZL& L = member<G>(l0, grp, bk);
   const ZArgs& a = L.a;
